@@ -1,0 +1,99 @@
+"""GPU parity: batched lookups through the C ABI against the oracle.
+
+Row ids must be bit-identical to the oracle's equality-checked lower_bound
+(binary_search_index semantics, src/indexes.cuh:357-361) for every index kind,
+including misses, the smallest and largest keys, and ragged sizes.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+KINDS = ["binary_search", "btree"]
+
+
+def _column(gj, n, kb, seed, dist=1):
+    return gj.gen_build_keys(n, kb, dist, 0, seed)
+
+
+def _probes(gj, keys, m, seed, miss=0.1):
+    if m == 0:
+        return np.empty(0, dtype=keys.dtype)
+    p, _ = gj.gen_probes(keys, m, 0, 1.0, miss, 0, seed)
+    kmax = np.iinfo(keys.dtype).max
+    edge = np.array([keys[0], keys[-1], 0, kmax, max(int(keys[0]) - 1, 0), min(int(keys[-1]) + 1, kmax)],
+                    dtype=keys.dtype)
+    return np.concatenate([edge, p])
+
+
+def _run(gj, kind, keys, probes, **lp):
+    col = gj.DeviceArray.from_numpy(keys)
+    idx = gj.Index(kind, col, host_column=keys)
+    m = probes.size
+    dp = gj.DeviceArray.from_numpy(probes) if m else gj.DeviceArray(1, keys.dtype)
+    out = gj.DeviceArray(max(m, 1), np.uint32)
+    idx.lookup(dp, out, m=m, **lp)
+    gj.synchronize()
+    return out.numpy(m), idx
+
+
+@pytest.mark.parametrize("kind", KINDS)
+@pytest.mark.parametrize("kb", [4, 8])
+@pytest.mark.parametrize("n", [1, 2, 17, 1000, 65539, 1 << 20])
+def test_lookup_matches_oracle(gj, orc, gpu, kind, kb, n):
+    print(f"[case] kind={kind} kb={kb} n={n}", flush=True)
+    keys = _column(gj, n, kb, seed=0x11 + n)
+    probes = _probes(gj, keys, 20000, seed=0x12 + n)
+    got, _ = _run(gj, kind, keys, probes)
+    want = orc.lookup_lower_bound(keys, probes)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("kind", KINDS + ["no_op"])
+def test_empty_probe_batch(gj, gpu, kind):
+    keys = _column(gj, 1000, 8, seed=5)
+    got, _ = _run(gj, kind, keys, np.empty(0, dtype=np.uint64))
+    assert got.size == 0
+
+
+def test_no_op_returns_not_found(gj, gpu):
+    keys = _column(gj, 1000, 8, seed=6)
+    probes = keys[::3].copy()
+    got, _ = _run(gj, "no_op", keys, probes)
+    assert np.all(got == gj.NOT_FOUND)
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_dense_keys_rowid_equals_key(gj, gpu, kind):
+    # dense build keys (generate_dense_dataset, src/index_lookup_common.cuh:44-49): row id == key
+    n = 300007
+    keys = gj.gen_build_keys(n, 8, 0, 0, 1)
+    probes = np.random.default_rng(3).integers(0, n + 1000, size=100000).astype(np.uint64)
+    got, _ = _run(gj, kind, keys, probes)
+    want = np.where(probes < n, probes, gj.NOT_FOUND).astype(np.uint32)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_block_sizes(gj, orc, gpu, kind):
+    keys = _column(gj, 50000, 8, seed=9)
+    probes = _probes(gj, keys, 30000, seed=10)
+    want = orc.lookup_lower_bound(keys, probes)
+    for block in (64, 128, 256):
+        got, _ = _run(gj, kind, keys, probes, block_size=block)
+        assert np.array_equal(got, want), block
+
+
+def test_unsorted_column_rejected(gj, gpu):
+    keys = np.array([5, 3, 9, 11], dtype=np.uint64)
+    col = gj.DeviceArray.from_numpy(keys)
+    with pytest.raises(gj.GjError) as e:
+        gj.Index("btree", col)
+    assert e.value.status == -1
+
+
+def test_duplicates_binary_search_first_occurrence(gj, orc, gpu):
+    keys = np.array([1, 2, 2, 2, 5, 5, 9], dtype=np.uint64)
+    probes = np.array([2, 5, 9, 1, 3, 0, 10], dtype=np.uint64)
+    got, _ = _run(gj, "binary_search", keys, probes)
+    assert np.array_equal(got, orc.lookup_lower_bound(keys, probes))
