@@ -1,0 +1,233 @@
+#!/usr/bin/env python3
+"""Index-join probe benchmark on one MI355X.
+
+Workload (the configuration the metric is quoted on): a sorted build column of
+2^26 unique uint64 keys drawn uniformly from [0, 2^63), row id = position;
+2^28 uint64 probe keys drawn uniformly with replacement from the build keys
+(100 % hit).  One step = one batched lookup of all 2^28 probes through the C
+ABI (gj_index_lookup), inputs resident in HBM before timing starts.
+
+Reported:
+  value            lookups/s of the headline index (learned when available, else the B-tree)
+  roofline         HBM roofline of the headline lookup kernel: 20 algorithmic bytes per lookup
+                   (8 B probe key + 8 B indexed key + 4 B row id) / average kernel time, vs 8 TB/s
+  gather_roofline  G = independent random 8 B gathers per second measured on the same GPU in the same
+                   process (gj_gather), and lookups/s / G per index
+  cpu_baseline     the oracle's OpenMP equality-checked lower_bound (port of binary_search_index) on a
+                   bounded probe sample, on this host's cores
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBPS = 8000.0
+BYTES_PER_LOOKUP = 20  # 8 B probe + 8 B indexed key + 4 B row id
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def time_steps(fn, steps, warmup, stream, gj):
+    """Runs fn() warmup + steps times; returns (per-step ms list, total ms) from events on `stream`."""
+    for _ in range(warmup):
+        fn()
+    stream.synchronize()
+    ev = [gj.Event() for _ in range(steps + 1)]
+    ev[0].record(stream)
+    for i in range(steps):
+        fn()
+        ev[i + 1].record(stream)
+    stream.synchronize()
+    per = [ev[i].elapsed_ms(ev[i + 1]) for i in range(steps)]
+    return per, ev[0].elapsed_ms(ev[steps])
+
+
+def load_traffic(kernel_tag: str):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, copy-calibrated),
+    committed as profiles/traffic.txt lines `<tag> <bytes_per_launch>`."""
+    path = os.path.join(ROOT, "profiles", "traffic.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) >= 2 and parts[0] == kernel_tag:
+                try:
+                    return float(parts[1])
+                except ValueError:
+                    return None
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--log2-n", type=int, default=26)
+    ap.add_argument("--log2-m", type=int, default=28)
+    ap.add_argument("--indexes", default="binary_search,btree,harmonia,radix_spline,learned_plr,learned_mlp")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-sample-log2", type=int, default=24)
+    ap.add_argument("--profile-index", default="", help="run only this index (for rocprofv3 runs)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import gpujoin_amd as gj
+    from gpujoin_amd import _lib
+
+    info = gj.device_info()
+    log("device", info)
+    n = 1 << args.log2_n
+    m = 1 << args.log2_m
+    threads = 0
+    t0 = time.time()
+    keys = gj.gen_build_keys(n, 8, _lib.KEYS_UNIFORM, 63, 0x11, threads)
+    probes, expected = gj.gen_probes(keys, m, _lib.PROBES_UNIFORM, 1.0, 0.0, 63, 0x12, threads)
+    log(f"generated n={n} m={m} in {time.time() - t0:.1f}s")
+    stream = gj.Stream()
+    d_keys = gj.DeviceArray.from_numpy(keys, stream)
+    d_probes = gj.DeviceArray.from_numpy(probes, stream)
+    d_out = gj.DeviceArray(m, np.uint32)
+    stream.synchronize()
+
+    results = {}
+
+    # ---- gather roofline G --------------------------------------------------
+    best_g = None
+    for items in (1, 2, 4, 8):
+        fn = lambda: gj.gather(d_keys, d_probes, d_out, m, 256, items, stream)
+        per, _ = time_steps(fn, max(3, args.steps // 2), 1, stream, gj)
+        ms = statistics.median(per)
+        rate = m / (ms * 1e-3)
+        log(f"gather items={items}: {ms:.3f} ms  {rate / 1e9:.2f} G gathers/s")
+        if best_g is None or rate > best_g["rate"]:
+            best_g = {"rate": rate, "ms": ms, "items": items}
+    G = best_g["rate"]
+
+    # streaming copy bandwidth (1 GiB)
+    cb = 1 << 30
+    src = gj.DeviceArray(cb // 8, np.uint64)
+    dst = gj.DeviceArray(cb // 8, np.uint64)
+    src.fill(1, stream)
+    per, _ = time_steps(lambda: gj.stream_copy(src, dst, cb, stream), 5, 2, stream, gj)
+    copy_gbps = 2 * cb / (statistics.median(per) * 1e-3) / 1e9
+    del src, dst
+    log(f"stream copy: {copy_gbps:.0f} GB/s (read + write)")
+
+    # dependent pointer chase over 512 MiB (latency)
+    nxt_n = 1 << 27
+    perm = np.random.default_rng(7).permutation(nxt_n).astype(np.uint32)
+    nxt = np.empty(nxt_n, dtype=np.uint32)
+    nxt[perm] = np.roll(perm, -1)  # one cycle through all slots
+    d_next = gj.DeviceArray.from_numpy(nxt, stream)
+    chase_out = gj.DeviceArray(64, np.uint32)
+    steps_chase = 20000
+    per, _ = time_steps(lambda: gj.pointer_chase(d_next, 1, steps_chase, chase_out, stream), 2, 1, stream, gj)
+    chase_ns = statistics.median(per) * 1e6 / steps_chase
+    del d_next, nxt, perm
+    log(f"pointer chase: {chase_ns:.0f} ns per dependent access")
+
+    # ---- index lookups ------------------------------------------------------
+    names = [args.profile_index] if args.profile_index else [s for s in args.indexes.split(",") if s]
+    lp = gj.lookup_params("plain")
+    for name in names:
+        try:
+            idx = gj.Index(name, d_keys, host_column=keys, stream=stream)
+        except gj.GjError as e:
+            log(f"{name}: not built ({e})")
+            continue
+        fn = lambda: idx.lookup(d_probes, d_out, m=m, params=lp, stream=stream)
+        per, total = time_steps(fn, args.steps, args.warmup, stream, gj)
+        got = d_out.numpy(m)
+        ok = bool(np.array_equal(got, expected))
+        avg = sum(per) / len(per)
+        rate = m * args.steps / (total * 1e-3)
+        st = idx.stats()
+        results[name] = {
+            "lookups_per_s": rate, "ms_per_step": total / args.steps, "avg_kernel_ms": avg,
+            "gather_roofline_frac": rate / G, "hbm_frac": (m * BYTES_PER_LOOKUP / (avg * 1e-3)) / 1e9 / PEAK_HBM_GBPS,
+            "rowids_match": ok, "index_bytes": st["memory_bytes"], "depth": st["depth"],
+            "lds_levels": st["lds_levels"],
+        }
+        log(f"{name}: {rate / 1e9:.3f} G lookups/s, {avg:.3f} ms/launch, {rate / G * 100:.1f}% of G, match={ok}")
+        idx.close()
+        if not ok:
+            log(f"{name}: ROW ID MISMATCH ({int(np.count_nonzero(got != expected))} probes)")
+
+    # ---- CPU baseline (oracle port, bounded sample) ---------------------------
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        ms_n = 1 << args.cpu_sample_log2
+        sample = probes[:ms_n]
+        cores = os.cpu_count() or 1
+        oracle.lookup_lower_bound(keys, sample[: 1 << 20], threads=cores)  # warm-up
+        times = []
+        for _ in range(3):
+            t = time.perf_counter()
+            out_cpu = oracle.lookup_lower_bound(keys, sample, threads=cores)
+            times.append(time.perf_counter() - t)
+        ok = bool(np.array_equal(out_cpu, expected[:ms_n]))
+        cpu = {"value": ms_n / statistics.median(times), "unit": "lookups/s", "cores": cores, "kind": "port",
+               "sample": f"2^{args.cpu_sample_log2} of the 2^{args.log2_m} probes against the full 2^{args.log2_n}-key "
+                         f"column, equality-checked lower_bound, OpenMP static schedule, median of 3 after 1 warm-up; "
+                         f"row ids match: {ok}"}
+        log(f"cpu baseline: {cpu['value'] / 1e6:.1f} M lookups/s on {cores} threads")
+
+    head_order = ["learned_mlp", "learned_plr", "radix_spline", "btree", "harmonia", "binary_search"]
+    head = next((h for h in head_order if h in results and results[h]["rowids_match"]), None)
+    if head is None and results:
+        head = next(iter(results))
+    r = results.get(head, {})
+    avg_s = r.get("avg_kernel_ms", float("nan")) * 1e-3
+    achieved = m * BYTES_PER_LOOKUP / avg_s / 1e9 if r else None
+    traffic = load_traffic(f"{head}_u64_2^{args.log2_n}_2^{args.log2_m}") if head else None
+    line = {
+        "metric": "probe lookups/s + % gather roofline, learned vs B-tree, 2^26-key index, 1 GPU",
+        "value": r.get("lookups_per_s"),
+        "unit": "lookups/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r.get("ms_per_step"),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic (seeded): uniform unique u64 build keys, uniform probes",
+        "config": {"workload": f"index probe: 2^{args.log2_n} sorted unique u64 keys in [0, 2^63), "
+                               f"2^{args.log2_m} uniform u64 probes (100% hit), u32 row ids",
+                   "index": head, "approach": "plain", "device": info["name"]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBPS, "unit": "GB/s",
+                     "frac": (achieved / PEAK_HBM_GBPS) if achieved else None, "traffic": traffic},
+        "gather_roofline": {"G_gathers_per_s": G, "items_per_thread": best_g["items"],
+                            "frac": (r.get("lookups_per_s", 0) / G) if r else None},
+        "cpu_baseline": cpu,
+        "indexes": results,
+        "learned_vs_btree": (results[head]["lookups_per_s"] / results["btree"]["lookups_per_s"])
+        if head in ("learned_mlp", "learned_plr", "radix_spline") and "btree" in results else None,
+        "stream_copy_GBps": copy_gbps,
+        "pointer_chase_ns": chase_ns,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()
