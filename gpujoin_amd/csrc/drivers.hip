@@ -1,18 +1,97 @@
-// drivers.hip - the lookup drivers ("approaches") behind gj_index_lookup.
+// drivers.hip - the lookup drivers ("approaches") behind gj_index_lookup:
+// plain, bws (blockwise sorting), partitioning (windowed radix partitioning)
+// and hj (hash join: build over the column, then probe).
+#include <algorithm>
+
 #include "common.hpp"
 #include "lookup_dispatch.hpp"
 
 namespace gj {
 
+namespace {
+
+// Number of significant bits of the largest key.  The reference derives
+// ignore_bits = floor(log2(max key)) - radix_bits (src/index_lookup_new.cu:63-67),
+// which leaves the top significant bit out of the digit; using the bit width
+// keeps the digit on the top radix_bits significant bits.
+uint32_t key_bit_width(uint64_t max_key) { return max_key ? 64u - static_cast<uint32_t>(__builtin_clzll(max_key)) : 0u; }
+
+// partitioning_approach (src/index_lookup_partitioning.cu:377-513): the probes
+// are cut into windows; per window a stable radix partition on the top key bits
+// (payload = global probe index), then a lookup over the partitioned tuples that
+// scatters row ids back to probe order.  The reference keeps two streams with a
+// device-wide sync per window pair; here the windows are stream-ordered on the
+// caller's stream and the tuple count never leaves the device.
+template <class K>
+void lookup_partitioned(const gj_index& ix, const K* probes, uint64_t m, uint32_t* out, const gj_lookup_params& p,
+                        hipStream_t s) {
+  const uint32_t bits = p.radix_bits ? p.radix_bits : 10u;
+  GJ_REQUIRE(bits >= 1 && bits <= 12, GJ_ERR_INVALID, "radix_bits must be in [1, 12]");
+  const uint32_t passes = p.passes == 2 ? 2u : 1u;
+  GJ_REQUIRE(m < 0xFFFFFFFFull, GJ_ERR_OVERFLOW, "partitioning needs fewer than 2^32 - 1 probes");
+  uint32_t ignore = p.ignore_bits;
+  if (ignore == 0xFFFFFFFFu) {
+    const uint32_t w = key_bit_width(ix.max_key);
+    ignore = w > bits ? w - bits : 0u;
+  }
+  GJ_REQUIRE(ignore < 8 * sizeof(K), GJ_ERR_INVALID, "ignore_bits must be below the key width");
+  const uint64_t window = std::min<uint64_t>(p.window ? p.window : (1ull << 22), m);
+  const uint64_t fan = 1ull << bits;
+  const uint64_t cap = partition_capacity(window, sizeof(K), bits);
+  GJ_REQUIRE(cap < 0xFFFFFFFFull, GJ_ERR_OVERFLOW, "window too large");
+  // scratch: keys | payloads | start (fan + 1) | count (fan) | partitioner scratch
+  auto up = [](uint64_t b) { return (b + 255) / 256 * 256; };
+  const uint64_t o_keys = 0;
+  const uint64_t o_pay = o_keys + up(cap * sizeof(K));
+  const uint64_t o_start = o_pay + up(cap * 4);
+  const uint64_t o_count = o_start + up((fan + 1) * 8);
+  const uint64_t o_part = o_count + up(fan * 8);
+  const uint64_t bytes = o_part + up(partition_scratch(window, sizeof(K), bits, passes));
+  ix.scratch.reserve(bytes);
+  uint8_t* base = ix.scratch.as<uint8_t>();
+  K* tkeys = reinterpret_cast<K*>(base + o_keys);
+  uint32_t* tpay = reinterpret_cast<uint32_t*>(base + o_pay);
+  uint64_t* start = reinterpret_cast<uint64_t*>(base + o_start);
+  uint64_t* count = reinterpret_cast<uint64_t*>(base + o_count);
+  for (uint64_t w0 = 0; w0 < m; w0 += window) {
+    const uint64_t w = std::min<uint64_t>(window, m - w0);
+    partition_keys<K>(probes + w0, w, static_cast<uint32_t>(w0), bits, ignore, passes, tkeys, tpay, start, count,
+                      base + o_part, ix.cu, s);
+    with_index_dev<K>(ix, [&](auto d) {
+      launch_tuples(d, tkeys, tpay, start + fan, partition_capacity(w, sizeof(K), bits), out, p, s, ix.cu);
+    });
+  }
+}
+
+// hj_approach (src/index_lookup_new.cu:252-302): build a linear-probing table
+// over the column, then probe.  The reference builds on the probe keys and
+// writes tids[build_tid] = probe key, which validates only for dense keys; this
+// driver returns each probe's row id, like every other driver.
+template <class K>
+void lookup_hash_join(const gj_index& ix, const K* probes, uint64_t m, uint32_t* out, const gj_lookup_params& p,
+                      hipStream_t s) {
+  uint64_t cap = 0;
+  build_hash_table<K>(ix, ix.scratch2, cap, s);
+  HashDev<K> d{};
+  d.table = ix.scratch2.as<const HashEntry>();
+  d.mask = cap - 1;
+  launch_plain(d, probes, m, out, p, s, ix.cu);
+}
+
+}  // namespace
+
 void lookup_dispatch(const gj_index& ix, const void* probes, uint64_t m, uint32_t* out, const gj_lookup_params& p,
                      hipStream_t s) {
-  switch (p.approach) {
-    case GJ_PLAIN:
-      dispatch_key(ix.key_bytes, [&]<class K>() { lookup_plain<K>(ix, static_cast<const K*>(probes), m, out, p, s); });
-      return;
-    default:
-      throw error(GJ_ERR_UNSUPPORTED, "approach not available");
-  }
+  dispatch_key(ix.key_bytes, [&]<class K>() {
+    const K* pk = static_cast<const K*>(probes);
+    switch (p.approach) {
+      case GJ_PLAIN: with_index_dev<K>(ix, [&](auto d) { launch_plain(d, pk, m, out, p, s, ix.cu); }); return;
+      case GJ_BWS: with_index_dev<K>(ix, [&](auto d) { launch_bws(d, pk, m, out, p, s, ix.cu); }); return;
+      case GJ_PARTITIONING: lookup_partitioned<K>(ix, pk, m, out, p, s); return;
+      case GJ_HJ: lookup_hash_join<K>(ix, pk, m, out, p, s); return;
+    }
+    throw error(GJ_ERR_INVALID, "unknown approach");
+  });
 }
 
 }  // namespace gj

@@ -1,25 +1,277 @@
-// extended.hip - index kinds beyond binary search / B-tree / no-op.
+// extended.hip - construction and lookup dispatch of Harmonia, RadixSpline,
+// the learned models and the hash table.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
 #include "common.hpp"
 #include "lookup_dispatch.hpp"
+#include "rs_builder.hpp"
 
 namespace gj {
 
-uint64_t extended_memory(const gj_index&) { return 0; }
+template <class K> void build_learned(gj_index& ix, hipStream_t s);  // learned.hip
+
+namespace {
+
+unsigned grid_for(uint64_t work, unsigned block, unsigned cap = 65536) {
+  uint64_t g = ceil_div(work, block);
+  if (g == 0) g = 1;
+  if (g > cap) g = cap;
+  return static_cast<unsigned>(g);
+}
+
+// ---------------------------------------------------------------------------
+// Harmonia construction.  The level sizes follow gather_tree_info
+// (src/harmonia.cuh:133-185).  Every node of a level except the last is full
+// (32 keys, 33 children), so children and separators have closed forms:
+//   children[node i of inner level l] = first(l+1) + 33 i     (store_structure, :324-344)
+//   children[leaf node i]             = 32 i                  (populate_leaf_nodes, :297-315)
+//   key s of node i of level l        = largest key under child 33 i + s of level l+1
+//                                     = col[min(n, (33 i + s + 1) * span(l+1)) - 1]
+// with span(l) = 32 * 33^(levels below l) column keys per node
+// (get_largest_key / populate_inner_nodes, :245-295).
+// ---------------------------------------------------------------------------
+struct HmLevels {
+  uint32_t depth;
+  uint64_t first[kMaxLevels];
+  uint64_t count[kMaxLevels];
+  uint64_t keycnt[kMaxLevels];
+  uint64_t span[kMaxLevels];
+};
+
+__device__ __forceinline__ uint32_t hm_level_of(const HmLevels& L, uint64_t g) {
+  uint32_t l = 0;
+  while (l + 1 < L.depth && L.first[l + 1] <= g) ++l;
+  return l;
+}
+
+__global__ void hm_children_kernel(HmLevels L, uint64_t inner_nodes, uint64_t total, uint32_t* __restrict__ children) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t g = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; g < total; g += stride) {
+    const uint32_t l = hm_level_of(L, g);
+    uint64_t v;
+    if (l + 1 < L.depth) v = L.first[l + 1] + 33ull * (g - L.first[l]);
+    else v = 32ull * (g - inner_nodes);
+    children[g] = static_cast<uint32_t>(v);
+  }
+}
 
 template <class K>
-void build_extended(gj_index& ix, hipStream_t) {
-  throw error(GJ_ERR_UNSUPPORTED, std::string("index kind not built in this library: ") + gj_index_type_name(ix.kind));
+__global__ void hm_keys_kernel(const K* __restrict__ col, uint64_t n, HmLevels L, K* __restrict__ keys, uint64_t slots) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < slots; t += stride) {
+    const uint64_t g = t / 32, s = t % 32;
+    const uint32_t l = hm_level_of(L, g);
+    const uint64_t i = g - L.first[l];
+    const uint64_t nkc = (i + 1 < L.count[l]) ? 32ull : L.keycnt[l] - i * 32ull;
+    K v = key_max<K>();
+    if (s < nkc) {
+      const uint64_t c = 33ull * i + s;
+      const uint64_t span = L.span[l + 1];
+      const uint64_t lim = (c + 1 > n / span) ? n : (c + 1) * span;
+      v = col[lim - 1];
+    }
+    keys[t] = v;
+  }
+}
+
+template <class K>
+void build_harmonia(gj_index& ix, hipStream_t s) {
+  GJ_REQUIRE(ix.n % 32 == 0, GJ_ERR_INVALID,
+             "harmonia: when Reuse_Input_Memory is set the dataset set must be a multiple of the node size (32)");
+  // level sizes, bottom-up (gather_tree_info)
+  std::vector<uint64_t> kc, pc;
+  uint64_t prev = 0, k = ix.n;
+  for (;;) {
+    uint64_t p = (k + 31) / 32;
+    if (33 * p < prev) p += 1;
+    kc.push_back(k);
+    pc.push_back(p);
+    prev = p;
+    k = (32 * p) / 33;
+    if (p == 1) break;
+    GJ_REQUIRE(kc.size() < static_cast<size_t>(kMaxLevels), GJ_ERR_INTERNAL, "harmonia: max depth exceeded");
+  }
+  HmLevels L{};
+  L.depth = static_cast<uint32_t>(kc.size());
+  uint64_t first = 0;
+  for (uint32_t l = 0; l < L.depth; ++l) {
+    const size_t b = L.depth - 1 - l;  // bottom-up index
+    L.first[l] = first;
+    L.count[l] = pc[b];
+    L.keycnt[l] = kc[b];
+    first += pc[b];
+  }
+  uint64_t span = 32;
+  for (int l = static_cast<int>(L.depth) - 1; l >= 0; --l) {
+    L.span[l] = span;
+    span = (span > (~0ull) / 33) ? ~0ull : span * 33;
+  }
+  ix.hm_depth = L.depth;
+  ix.hm_inner_nodes = L.first[L.depth - 1];
+  ix.hm_nodes = first;
+  GJ_REQUIRE(ix.hm_nodes < 0xFFFFFFFFull && ix.n <= 0xFFFFFFFFull, GJ_ERR_OVERFLOW, "harmonia: too many nodes");
+  ix.hm_children.alloc((ix.hm_nodes + 4) * sizeof(uint32_t));  // padded for 16-byte LDS staging
+  fill_bytes(ix.hm_children.p, 0, ix.hm_children.bytes, s);
+  hipLaunchKernelGGL(hm_children_kernel, dim3(grid_for(ix.hm_nodes, 256)), dim3(256), 0, s, L, ix.hm_inner_nodes,
+                     ix.hm_nodes, ix.hm_children.as<uint32_t>());
+  GJ_HIP_CHECK(hipGetLastError());
+  if (ix.hm_inner_nodes) {
+    const uint64_t slots = ix.hm_inner_nodes * 32;
+    ix.hm_keys.alloc(slots * sizeof(K));
+    hipLaunchKernelGGL(hm_keys_kernel<K>, dim3(grid_for(slots, 256)), dim3(256), 0, s, static_cast<const K*>(ix.d_col),
+                       ix.n, L, ix.hm_keys.as<K>(), slots);
+    GJ_HIP_CHECK(hipGetLastError());
+  }
+  // top inner levels into LDS within the budget
+  const uint64_t budget = ix.params.btree_lds_bytes;
+  uint64_t bytes = 0;
+  for (uint32_t l = 0; l + 1 < L.depth; ++l) {
+    const uint64_t add = L.count[l] * (32 * sizeof(K) + 4);
+    if (bytes + add + 16 > budget) break;
+    bytes += add;
+    ix.hm_lds_levels = l + 1;
+    ix.hm_lds_nodes = L.first[l] + L.count[l];
+  }
+  ix.stats.depth = L.depth;
+  ix.stats.lds_levels = ix.hm_lds_levels;
+  const uint32_t g = ix.params.harmonia_group_lanes;
+  const bool ok = (g == 8 || g == 16) || (sizeof(K) == 4 && g == 32);
+  GJ_REQUIRE(ok, GJ_ERR_INVALID, "harmonia_group_lanes must be 8 or 16 (or 32 for 4-byte keys)");
+}
+
+// ---------------------------------------------------------------------------
+// RadixSpline: host builder, device tables, then a device pass that checks
+// every build key lies inside its own search window and widens the bound
+// until it does (the FP64 interpolation can round across the exact bound).
+// ---------------------------------------------------------------------------
+template <class K>
+__global__ void rs_check_kernel(RadixSplineDev<K, 8> d, unsigned long long* bad) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  unsigned long long cnt = 0;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < d.n; i += stride) {
+    uint64_t lo, hi;
+    d.window(d.estimate(d.col[i]), lo, hi);
+    cnt += (i < lo || i >= hi) ? 1ull : 0ull;
+  }
+  if (cnt) atomicAdd(bad, cnt);
+}
+
+std::vector<uint8_t> host_column(const gj_index& ix, hipStream_t s) {
+  std::vector<uint8_t> h(ix.n * ix.key_bytes);
+  if (ix.h_col) {
+    std::memcpy(h.data(), ix.h_col, h.size());
+  } else {
+    GJ_HIP_CHECK(hipMemcpyAsync(h.data(), ix.d_col, h.size(), hipMemcpyDeviceToHost, s));
+    GJ_HIP_CHECK(hipStreamSynchronize(s));
+  }
+  return h;
+}
+
+template <class K>
+void build_rs(gj_index& ix, hipStream_t s) {
+  const uint32_t bits = ix.params.rs_radix_bits;
+  GJ_REQUIRE(bits >= 1 && bits <= 30, GJ_ERR_INVALID, "rs_radix_bits must be in [1, 30]");
+  std::vector<uint8_t> hcol;
+  const void* hk = ix.h_col;
+  if (!hk) {
+    hcol = host_column(ix, s);
+    hk = hcol.data();
+  }
+  RadixSplineHost rs = build_radix_spline(hk, ix.n, ix.key_bytes, bits, ix.params.rs_max_error);
+  ix.rs_shift = rs.shift_bits;
+  ix.rs_points = rs.knot_x.size();
+  ix.rs_radix.alloc(rs.radix.size() * sizeof(uint32_t));
+  GJ_HIP_CHECK(hipMemcpyAsync(ix.rs_radix.p, rs.radix.data(), ix.rs_radix.bytes, hipMemcpyHostToDevice, s));
+  std::vector<K> kx(rs.knot_x.size());
+  for (size_t i = 0; i < kx.size(); ++i) kx[i] = static_cast<K>(rs.knot_x[i]);
+  ix.rs_knot_x.alloc(kx.size() * sizeof(K));
+  GJ_HIP_CHECK(hipMemcpyAsync(ix.rs_knot_x.p, kx.data(), ix.rs_knot_x.bytes, hipMemcpyHostToDevice, s));
+  ix.rs_knot_y.alloc(rs.knot_y.size() * sizeof(double));
+  GJ_HIP_CHECK(hipMemcpyAsync(ix.rs_knot_y.p, rs.knot_y.data(), ix.rs_knot_y.bytes, hipMemcpyHostToDevice, s));
+  ix.rs_err = ix.params.rs_max_error;
+  dbuf bad(sizeof(unsigned long long));
+  for (int attempt = 0; attempt < 40; ++attempt) {
+    fill_bytes(bad.p, 0, bad.bytes, s);
+    hipLaunchKernelGGL(rs_check_kernel<K>, dim3(grid_for(ix.n, 256, 8192)), dim3(256), 0, s, ix.rs_dev<K, 8>(),
+                       bad.as<unsigned long long>());
+    GJ_HIP_CHECK(hipGetLastError());
+    unsigned long long h = 0;
+    GJ_HIP_CHECK(hipMemcpyAsync(&h, bad.p, sizeof(h), hipMemcpyDeviceToHost, s));
+    GJ_HIP_CHECK(hipStreamSynchronize(s));
+    if (h == 0) break;
+    GJ_REQUIRE(attempt < 39, GJ_ERR_INTERNAL, "radix spline: search bound does not converge");
+    ix.rs_err += std::max<uint32_t>(1, ix.rs_err / 4);
+  }
+  ix.stats.spline_points = ix.rs_points;
+  ix.stats.max_window = 2 * ix.rs_err + 2;
+}
+
+// ---------------------------------------------------------------------------
+// Hash table (linear_probing_hashtable, src/linear_probing_hashtable.cuh:37-93).
+// ---------------------------------------------------------------------------
+template <class K>
+__global__ void ht_insert_kernel(const K* __restrict__ col, uint64_t n, HashEntry* table, uint64_t mask) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const unsigned long long k = static_cast<uint64_t>(col[i]);
+    uint64_t s = murmur3_64(k) & mask;
+    for (;;) {
+      const unsigned long long prev = atomicCAS(&table[s].key, ~0ull, k);
+      if (prev == ~0ull || prev == k) {
+        table[s].value = static_cast<uint32_t>(i);
+        break;
+      }
+      s = (s + 1) & mask;
+    }
+  }
+}
+
+}  // namespace
+
+// calculate_table_size, src/linear_probing_hashtable.cuh:74-79
+uint64_t hash_capacity(uint64_t n) {
+  const float hint = static_cast<float>(n) / 0.5f;
+  const uint64_t bits = static_cast<uint64_t>(std::ceil(std::log2(static_cast<double>(hint))));
+  return 1ull << bits;
+}
+
+template <class K>
+void build_hash_table(const gj_index& ix, dbuf& table, uint64_t& cap, hipStream_t s) {
+  GJ_REQUIRE(!(sizeof(K) == 8 && ix.max_key == ~0ull), GJ_ERR_INVALID,
+             "hash: the key 2^64-1 is the empty-slot marker and cannot be stored");
+  cap = hash_capacity(ix.n);
+  table.reserve(cap * sizeof(HashEntry));
+  fill_bytes(table.p, 0xFF, cap * sizeof(HashEntry), s);
+  hipLaunchKernelGGL(ht_insert_kernel<K>, dim3(grid_for(ix.n, 256)), dim3(256), 0, s, static_cast<const K*>(ix.d_col),
+                     ix.n, table.as<HashEntry>(), cap - 1);
+  GJ_HIP_CHECK(hipGetLastError());
+}
+template void build_hash_table<uint32_t>(const gj_index&, dbuf&, uint64_t&, hipStream_t);
+template void build_hash_table<uint64_t>(const gj_index&, dbuf&, uint64_t&, hipStream_t);
+
+uint64_t extended_memory(const gj_index& ix) {
+  return ix.hm_keys.bytes + ix.hm_children.bytes + ix.rs_radix.bytes + ix.rs_knot_x.bytes + ix.rs_knot_y.bytes +
+         ix.lm_leaves.bytes + ix.lm_mlp.bytes + ix.ht.bytes;
+}
+
+template <class K>
+void build_extended(gj_index& ix, hipStream_t s) {
+  switch (ix.kind) {
+    case GJ_HARMONIA: build_harmonia<K>(ix, s); break;
+    case GJ_RADIX_SPLINE: build_rs<K>(ix, s); break;
+    case GJ_LEARNED_PLR:
+    case GJ_LEARNED_MLP:
+      build_btree<K>(ix, s);
+      build_learned<K>(ix, s);
+      break;
+    case GJ_HASH: build_hash_table<K>(ix, ix.ht, ix.ht_cap, s); break;
+    default:
+      throw error(GJ_ERR_UNSUPPORTED, std::string("index kind not built: ") + gj_index_type_name(ix.kind));
+  }
 }
 template void build_extended<uint32_t>(gj_index&, hipStream_t);
 template void build_extended<uint64_t>(gj_index&, hipStream_t);
-
-template <class K>
-void dispatch_extended_plain(const gj_index& ix, const K*, uint64_t, uint32_t*, const gj_lookup_params&, hipStream_t) {
-  throw error(GJ_ERR_UNSUPPORTED, std::string("lookup not available for ") + gj_index_type_name(ix.kind));
-}
-template void dispatch_extended_plain<uint32_t>(const gj_index&, const uint32_t*, uint64_t, uint32_t*,
-                                                const gj_lookup_params&, hipStream_t);
-template void dispatch_extended_plain<uint64_t>(const gj_index&, const uint64_t*, uint64_t, uint32_t*,
-                                                const gj_lookup_params&, hipStream_t);
 
 }  // namespace gj

@@ -155,24 +155,6 @@ void build_btree(gj_index& ix, hipStream_t s) {
 template void build_btree<uint32_t>(gj_index&, hipStream_t);
 template void build_btree<uint64_t>(gj_index&, hipStream_t);
 
-// ---------------------------------------------------------------------------
-// Plain lookup dispatch
-// ---------------------------------------------------------------------------
-template <class K>
-void lookup_plain(const gj_index& ix, const K* probes, uint64_t m, uint32_t* out, const gj_lookup_params& p,
-                  hipStream_t s) {
-  switch (ix.kind) {
-    case GJ_BINARY_SEARCH: launch_plain(ix.bsearch_dev<K>(), probes, m, out, p, s, ix.cu); break;
-    case GJ_BTREE: launch_plain(ix.btree_dev<K>(), probes, m, out, p, s, ix.cu); break;
-    case GJ_NO_OP: launch_plain(NoOpDev<K>{}, probes, m, out, p, s, ix.cu); break;
-    default: dispatch_extended_plain<K>(ix, probes, m, out, p, s); break;
-  }
-}
-template void lookup_plain<uint32_t>(const gj_index&, const uint32_t*, uint64_t, uint32_t*, const gj_lookup_params&,
-                                     hipStream_t);
-template void lookup_plain<uint64_t>(const gj_index&, const uint64_t*, uint64_t, uint32_t*, const gj_lookup_params&,
-                                     hipStream_t);
-
 void build_index(gj_index& ix, hipStream_t s) {
   dispatch_key(ix.key_bytes, [&]<class K>() {
     const bool strict = !(ix.kind == GJ_BINARY_SEARCH || ix.kind == GJ_NO_OP);

@@ -6,6 +6,7 @@
 
 #include "common.hpp"
 #include "indexes.cuh"
+#include "indexes_ext.cuh"
 
 namespace gj {
 
@@ -42,8 +43,28 @@ struct gj_index {
   gj::dbuf btree_nodes;
   gj::BTreeLayout bt;
 
+  // Harmonia
+  gj::dbuf hm_keys, hm_children;
+  uint64_t hm_inner_nodes = 0, hm_lds_nodes = 0, hm_nodes = 0;
+  uint32_t hm_depth = 0, hm_lds_levels = 0;
+
+  // RadixSpline
+  gj::dbuf rs_radix, rs_knot_x, rs_knot_y;
+  uint32_t rs_shift = 0, rs_err = 0;
+  uint64_t rs_points = 0;
+
+  // learned two-stage models (B-tree above is their fallback)
+  gj::dbuf lm_leaves, lm_mlp;
+  gj::RootModel lm_root{};
+  uint32_t lm_num_leaves = 0, lm_fallback_any = 0;
+
+  // hash table
+  gj::dbuf ht;
+  uint64_t ht_cap = 0;
+
   // scratch reused by the bws / partitioning / hj drivers
   mutable gj::dbuf scratch;
+  mutable gj::dbuf scratch2;
 
   gj_index_stats stats{};
 
@@ -55,6 +76,59 @@ struct gj_index {
     d.samples = samples.as<const K>();
     d.step = sample_step;
     d.sample_bits = sample_bits;
+    return d;
+  }
+
+  template <class K, int G>
+  gj::HarmoniaDev<K, G> harmonia_dev() const {
+    gj::HarmoniaDev<K, G> d{};
+    d.col = static_cast<const K*>(d_col);
+    d.n = n;
+    d.keys = hm_keys.as<const K>();
+    d.children = hm_children.as<const uint32_t>();
+    d.inner_nodes = hm_inner_nodes;
+    d.depth = hm_depth;
+    d.lds_levels = hm_lds_levels;
+    d.lds_nodes = hm_lds_nodes;
+    return d;
+  }
+
+  template <class K, int G>
+  gj::RadixSplineDev<K, G> rs_dev() const {
+    gj::RadixSplineDev<K, G> d{};
+    d.col = static_cast<const K*>(d_col);
+    d.n = n;
+    d.radix = rs_radix.as<const uint32_t>();
+    d.knot_x = rs_knot_x.as<const K>();
+    d.knot_y = rs_knot_y.as<const double>();
+    d.min_key = static_cast<K>(min_key);
+    d.max_key = static_cast<K>(max_key);
+    d.shift_bits = rs_shift;
+    d.err = rs_err;
+    return d;
+  }
+
+  template <class K, int G>
+  gj::LearnedDev<K, G> learned_dev() const {
+    gj::LearnedDev<K, G> d{};
+    d.col = static_cast<const K*>(d_col);
+    d.n = n;
+    d.min_key = static_cast<K>(min_key);
+    d.max_key = static_cast<K>(max_key);
+    d.root = lm_root;
+    d.root.mlp = lm_mlp.as<const float>();
+    d.leaves = lm_leaves.as<const gj::LeafModel>();
+    d.num_leaves = lm_num_leaves;
+    d.fallback_any = lm_fallback_any;
+    d.btree = btree_dev<K>();
+    return d;
+  }
+
+  template <class K>
+  gj::HashDev<K> hash_dev() const {
+    gj::HashDev<K> d{};
+    d.table = ht.as<const gj::HashEntry>();
+    d.mask = ht_cap - 1;
     return d;
   }
 

@@ -1,0 +1,485 @@
+// indexes_ext.cuh - device descriptors of Harmonia, RadixSpline, the learned
+// two-stage models and the hash table (same contract as indexes.cuh).
+#pragma once
+
+#include "device.cuh"
+#include "indexes.cuh"
+
+namespace gj {
+
+// KPL consecutive keys loaded by one lane (4, 8 or 16 bytes).
+template <class K, int KPL>
+struct LaneKeys {
+  K k[KPL];
+};
+template <class K, int KPL>
+__device__ __forceinline__ LaneKeys<K, KPL> load_lane_keys(const K* p) {
+  constexpr int B = KPL * static_cast<int>(sizeof(K));
+  LaneKeys<K, KPL> r;
+  if constexpr (B == 16) {
+    const u32x4 v = *reinterpret_cast<const u32x4*>(p);
+    if constexpr (sizeof(K) == 8) {
+      r.k[0] = (static_cast<uint64_t>(v.y) << 32) | v.x;
+      r.k[1] = (static_cast<uint64_t>(v.w) << 32) | v.z;
+    } else {
+      r.k[0] = v.x; r.k[1] = v.y; r.k[2] = v.z; r.k[3] = v.w;
+    }
+  } else if constexpr (B == 8) {
+    const u32x2 v = *reinterpret_cast<const u32x2*>(p);
+    if constexpr (sizeof(K) == 8) {
+      r.k[0] = (static_cast<uint64_t>(v.y) << 32) | v.x;
+    } else {
+      r.k[0] = v.x; r.k[1] = v.y;
+    }
+  } else {
+    static_assert(B == 4, "4, 8 or 16 bytes per lane");
+    r.k[0] = *p;
+  }
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// Harmonia (harmonia_index, src/indexes.cuh:257-307; harmonia_tree,
+// src/harmonia.cuh:48-992).  Layout kept from the reference: 32 keys per node
+// (fanout 33), inner keys breadth first and padded with key_max, leaf keys =
+// the column, children[node] = first child (inner) or first row id (leaf),
+// so next = children[pos] + lb (src/harmonia.cuh:748-766).  Re-laid out for
+// CDNA4: children are uint32, the top levels (keys and children) sit in LDS
+// (the reference uses 42 KiB of __constant__, src/harmonia.cuh:45-46), and a
+// node is searched by a group of G lanes of the wave64 that fetch one
+// 128-byte line per load and read a u64 node's second line only when every
+// key of the first line is below the probe (the reference's early exit,
+// src/harmonia.cuh:581-589).
+// ---------------------------------------------------------------------------
+template <class K, int G>
+struct HarmoniaDev {
+  static constexpr int MK = 32;                                   // keys per node
+  static constexpr int LK = 128 / static_cast<int>(sizeof(K));    // keys per 128-byte line
+  static constexpr int LINES = MK / LK;                           // 2 (u64) or 1 (u32)
+  static constexpr int KPL = LK / G;                              // keys per lane per line
+  static_assert(KPL >= 1 && KPL * G == LK, "group must tile a line");
+
+  const K* col;
+  uint64_t n;
+  const K* keys;              // inner nodes x MK
+  const uint32_t* children;   // every node (inner and leaf)
+  uint64_t inner_nodes;
+  uint32_t depth;             // levels including the leaf level
+  uint32_t lds_levels;        // top inner levels staged in LDS
+  uint64_t lds_nodes;
+
+  __host__ __device__ size_t lds_key_bytes() const { return lds_nodes * MK * sizeof(K); }
+  __host__ __device__ size_t lds_bytes() const {
+    return lds_nodes ? lds_key_bytes() + ((lds_nodes * 4 + 15) / 16) * 16 : 0;
+  }
+  __device__ void stage(uint8_t* lds) const {
+    if (!lds_nodes) return;
+    stage_to_lds(lds, keys, lds_key_bytes());
+    stage_to_lds(lds + lds_key_bytes(), children, ((lds_nodes * 4 + 15) / 16) * 16);
+  }
+
+  // Count of node keys below each lane's probe (and whether the key at that
+  // position equals the probe), for the G probes of the lane's group.
+  __device__ __forceinline__ uint32_t node_search(const K* base, uint64_t my_off, K my_x, int gbase, int gl,
+                                                  bool& my_eq) const {
+    uint64_t off[G];
+    K xs[G];
+    LaneKeys<K, KPL> v[G];
+#pragma unroll
+    for (int r = 0; r < G; ++r) {
+      off[r] = shfl64(my_off, gbase + r);
+      xs[r] = shfl(my_x, gbase + r);
+      v[r] = load_lane_keys<K, KPL>(base + off[r] + gl * KPL);
+    }
+    uint32_t res = 0;
+    my_eq = false;
+#pragma unroll
+    for (int r = 0; r < G; ++r) {
+      int c = 0;
+      bool e = false;
+#pragma unroll
+      for (int j = 0; j < KPL; ++j) {
+        c += group_popc<G>(ballot(v[r].k[j] < xs[r]), gbase);
+        e |= group_popc<G>(ballot(v[r].k[j] == xs[r]), gbase) != 0;
+      }
+      if constexpr (LINES == 2) {
+        if (c == LK) {  // every key of the first line is below the probe: read the second line
+          const LaneKeys<K, KPL> w = load_lane_keys<K, KPL>(base + off[r] + LK + gl * KPL);
+          int c2 = 0;
+          bool e2 = false;
+#pragma unroll
+          for (int j = 0; j < KPL; ++j) {
+            c2 += group_popc<G>(ballot(w.k[j] < xs[r]), gbase);
+            e2 |= group_popc<G>(ballot(w.k[j] == xs[r]), gbase) != 0;
+          }
+          c += c2;
+          e = e2;
+        }
+      }
+      if (gl == r) {
+        res = static_cast<uint32_t>(c);
+        my_eq = e;
+      }
+    }
+    return res;
+  }
+
+  __device__ __forceinline__ uint32_t lookup(K x, bool active, const uint8_t* lds) const {
+    const int ln = lane();
+    const int gbase = ln & ~(G - 1);
+    const int gl = ln & (G - 1);
+    bool live = active;
+    bool eq = false;
+    uint64_t pos = 0;
+    uint32_t d = 0;
+    const K* lds_keys = reinterpret_cast<const K*>(lds);
+    const uint32_t* lds_children = reinterpret_cast<const uint32_t*>(lds + lds_key_bytes());
+    for (; d < lds_levels; ++d) {
+      const uint32_t lb = node_search(lds_keys, pos * MK, x, gbase, gl, eq);
+      const uint64_t next = static_cast<uint64_t>(lds_children[pos]) + lb;
+      pos = live ? next : 0;
+    }
+    for (; d + 1 < depth; ++d) {
+      const uint32_t first = children[pos];
+      const uint32_t lb = node_search(keys, pos * MK, x, gbase, gl, eq);
+      const uint64_t next = static_cast<uint64_t>(first) + lb;
+      pos = live ? next : 0;
+    }
+    const uint64_t leaf = live ? pos - inner_nodes : 0;
+    const uint32_t lb = node_search(col, leaf * MK, x, gbase, gl, eq);
+    return (live && eq) ? static_cast<uint32_t>(leaf * MK + lb) : kNotFound;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Keys of one 128-byte line that fall inside [lo, hi): counts of those below x,
+// of those inside the window, and the in-line index of a key equal to x.
+template <class K, int G>
+struct LineProbe {
+  static constexpr int LK = 128 / static_cast<int>(sizeof(K));
+  static constexpr int KPL = LK / G;
+  LaneKeys<K, KPL> v;
+  __device__ __forceinline__ void load(const K* col, uint64_t line, uint64_t hi, int gl) {
+    const uint64_t start = line * LK + static_cast<uint64_t>(gl * KPL);
+#pragma unroll
+    for (int j = 0; j < KPL; ++j) v.k[j] = key_max<K>();
+    if (start + KPL <= hi) {
+      v = load_lane_keys<K, KPL>(col + start);
+    } else {
+#pragma unroll
+      for (int j = 0; j < KPL; ++j)
+        if (start + j < hi) v.k[j] = col[start + j];
+    }
+  }
+  __device__ __forceinline__ void eval(uint64_t line, uint64_t lo, uint64_t hi, K x, int gbase, int gl, int& below,
+                                       int& valid, int& eqpos) const {
+    const uint64_t start = line * LK + static_cast<uint64_t>(gl * KPL);
+    below = 0;
+    valid = 0;
+    eqpos = -1;
+#pragma unroll
+    for (int j = 0; j < KPL; ++j) {
+      const bool ok = (start + j >= lo) && (start + j < hi);
+      below += group_popc<G>(ballot(ok && v.k[j] < x), gbase);
+      valid += group_popc<G>(ballot(ok), gbase);
+      const int f = group_ffs<G>(ballot(ok && v.k[j] == x), gbase);
+      if (f >= 0) eqpos = f * KPL + j;
+    }
+  }
+};
+
+// Last-mile search shared by RadixSpline and the learned models: each lane's
+// probe has a predicted position `pred` and a window [lo, hi) of the column
+// that holds the key if it is present.  The group of G lanes reads the
+// 128-byte line that contains pred (the lines of all G probes are requested
+// before any is examined) and walks one line left or right until the line
+// brackets the probe or leaves the window - one line when the prediction lands
+// in the right line.  Only keys inside [0, hi) are ever read.
+template <class K, int G>
+__device__ __forceinline__ uint32_t group_window_search(const K* col, uint64_t my_pred, uint64_t my_lo,
+                                                        uint64_t my_hi, K my_x, bool my_live, int gbase, int gl) {
+  using LP = LineProbe<K, G>;
+  constexpr int LK = LP::LK;
+  const bool ok_self = my_live && my_lo < my_hi;
+  const uint64_t live_mask = ballot(ok_self);
+  uint64_t line0_self = 0;
+  if (ok_self) {
+    const uint64_t p = my_pred < my_lo ? my_lo : (my_pred >= my_hi ? my_hi - 1 : my_pred);
+    line0_self = p / LK;
+  }
+  const uint64_t lo_self = ok_self ? my_lo : 0, hi_self = ok_self ? my_hi : 0;
+  LP first[G];
+#pragma unroll
+  for (int r = 0; r < G; ++r) {
+    const uint64_t line = shfl64(line0_self, gbase + r);
+    const uint64_t hi = shfl64(hi_self, gbase + r);
+    first[r].load(col, line, hi, gl);
+  }
+  uint32_t result = kNotFound;
+#pragma unroll
+  for (int r = 0; r < G; ++r) {
+    const bool owner_live = (live_mask >> (gbase + r)) & 1ull;  // group-uniform
+    const K x = shfl(my_x, gbase + r);
+    const uint64_t lo = shfl64(lo_self, gbase + r);
+    const uint64_t hi = shfl64(hi_self, gbase + r);
+    uint64_t line = shfl64(line0_self, gbase + r);
+    uint32_t found = kNotFound;
+    if (owner_live) {
+      const uint64_t lo_line = lo / LK, hi_line = (hi - 1) / LK;
+      int below, valid, eqpos;
+      first[r].eval(line, lo, hi, x, gbase, gl, below, valid, eqpos);
+      int dir = 0;  // -1 walking left, +1 walking right
+      for (;;) {
+        if (eqpos >= 0) {
+          found = static_cast<uint32_t>(line * LK + static_cast<uint64_t>(eqpos));
+          break;
+        }
+        if (below == valid && dir >= 0 && line < hi_line) {
+          ++line;  // every window key of the line is below x: only the right can hold it
+          dir = 1;
+        } else if (below == 0 && dir <= 0 && line > lo_line) {
+          --line;  // no window key of the line is below x: only the left can hold it
+          dir = -1;
+        } else {
+          break;
+        }
+        LP next;
+        next.load(col, line, hi, gl);
+        next.eval(line, lo, hi, x, gbase, gl, below, valid, eqpos);
+      }
+    }
+    if (gl == r) result = found;
+  }
+  return result;
+}
+
+// ---------------------------------------------------------------------------
+// RadixSpline (radix_spline_index, src/indexes.cuh:176-248; estimate
+// src/rs.cuh:118-154).  Knots are stored structure-of-arrays (keys, FP64
+// positions) so the knot search reads 8-byte keys; the estimate is the
+// reference's FP64 interpolation with an explicit fma, and the last mile is
+// the line-walking window search over [est - err, est + err + 2).
+// ---------------------------------------------------------------------------
+template <class K, int G>
+struct RadixSplineDev {
+  const K* col;
+  uint64_t n;
+  const uint32_t* radix;  // radix table
+  const K* knot_x;
+  const double* knot_y;
+  K min_key, max_key;
+  uint32_t shift_bits;
+  uint32_t err;  // search bound (>= the builder's bound; widened at build until every key is inside)
+
+  __host__ __device__ size_t lds_bytes() const { return 0; }
+  __device__ void stage(uint8_t*) const {}
+
+  __device__ __forceinline__ double estimate(K x) const {
+    if (x <= min_key) return 0.0;
+    if (x >= max_key) return static_cast<double>(n - 1);
+    const uint64_t prefix = static_cast<uint64_t>(x - min_key) >> shift_bits;
+    uint64_t lower = radix[prefix];
+    uint64_t count = static_cast<uint64_t>(radix[prefix + 1]) - lower;
+    while (count > 0) {
+      const uint64_t step = count >> 1, mid = lower + step;
+      if (knot_x[mid] < x) {
+        lower = mid + 1;
+        count -= step + 1;
+      } else {
+        count = step;
+      }
+    }
+    const K x0 = knot_x[lower - 1], x1 = knot_x[lower];
+    const double y0 = knot_y[lower - 1], y1 = knot_y[lower];
+    const double slope = (y1 - y0) / static_cast<double>(x1 - x0);
+    return __builtin_fma(static_cast<double>(x - x0), slope, y0);
+  }
+
+  __device__ __forceinline__ void window(double est, uint64_t& lo, uint64_t& hi) const {
+    const double e = static_cast<double>(err);
+    lo = (est < e) ? 0 : static_cast<uint64_t>(est - e);
+    hi = (est + e + 2.0 > static_cast<double>(n)) ? n : static_cast<uint64_t>(est + e + 2.0);
+    if (lo > hi) lo = hi;
+  }
+
+  __device__ __forceinline__ uint32_t lookup(K x, bool active, const uint8_t*) const {
+    const int ln = lane();
+    const int gbase = ln & ~(G - 1);
+    const int gl = ln & (G - 1);
+    uint64_t lo = 0, hi = 0, pred = 0;
+    if (active) {
+      const double est = estimate(x);
+      window(est, lo, hi);
+      pred = static_cast<uint64_t>(est);
+    }
+    return group_window_search<K, G>(col, pred, lo, hi, x, active, gbase, gl);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Learned two-stage position model (no reference counterpart).
+//   root:  cdf = root(x), x = (key - min_key) / (max_key - min_key) in FP32;
+//          linear (a*x + b) or a monotone 1xH MLP with ramp units
+//          clamp(w1*x + b1, 0, 1) (w1, w2 >= 0); leaf = floor(cdf * L)
+//   leaf:  pred = start + floor(slope * dkey + icpt) in FP32, dkey = key - leaf base key
+//          window = [pred + err_lo, pred + err_hi] measured over the leaf's keys
+//          with this exact FP32 code; leaves whose window exceeds the bound go
+//          to the B-tree.
+// ---------------------------------------------------------------------------
+struct LeafModel {  // 32 bytes
+  uint64_t base_key;
+  uint32_t start;
+  uint32_t count;
+  float slope;
+  float icpt;
+  int32_t err_lo;  // window = [pred + err_lo, pred + err_hi]
+  int32_t err_hi;
+};
+static_assert(sizeof(LeafModel) == 32, "leaf model is 32 bytes");
+
+constexpr int kMaxHidden = 64;
+
+struct RootModel {
+  float a, b;                 // linear root
+  float inv_range;            // 1 / (max_key - min_key) in FP32
+  uint32_t hidden;            // 0: linear root
+  const float* mlp;           // [w1(h), b1(h), w2(h), b2] in global memory (staged in LDS)
+};
+
+__device__ __forceinline__ float mlp_eval(const float* p, uint32_t h, float x) {
+  float f = p[3 * h];
+  for (uint32_t j = 0; j < h; ++j) {
+    const float a = __builtin_fmaf(p[j], x, p[h + j]);
+    f = __builtin_fmaf(p[2 * h + j], fminf(fmaxf(a, 0.0f), 1.0f), f);
+  }
+  return f;
+}
+
+template <class K>
+__device__ __forceinline__ float key_unit(K key, K min_key, float inv_range) {
+  return static_cast<float>(static_cast<uint64_t>(key - min_key)) * inv_range;
+}
+
+__device__ __forceinline__ uint32_t leaf_of(float cdf, uint32_t leaves) {
+  float l = cdf * static_cast<float>(leaves);
+  l = l < 0.0f ? 0.0f : l;
+  uint32_t li = static_cast<uint32_t>(l);
+  return li >= leaves ? leaves - 1 : li;
+}
+
+// signed FP32 distance of key from the leaf's base key
+template <class K>
+__device__ __forceinline__ float leaf_dkey(K key, uint64_t base) {
+  const uint64_t k = static_cast<uint64_t>(key);
+  return k >= base ? static_cast<float>(k - base) : -static_cast<float>(base - k);
+}
+
+__device__ __forceinline__ int64_t leaf_pred(const LeafModel& m, float dkey) {
+  float p = __builtin_fmaf(m.slope, dkey, m.icpt);
+  p = fminf(fmaxf(p, -2.0e9f), 2.0e9f);
+  return static_cast<int64_t>(m.start) + static_cast<int64_t>(floorf(p));
+}
+
+template <class K, int G>
+struct LearnedDev {
+  const K* col;
+  uint64_t n;
+  K min_key, max_key;
+  RootModel root;
+  const LeafModel* leaves;
+  uint32_t num_leaves;
+  uint32_t fallback_any;  // any leaf routed to the B-tree
+  BTreeDev<K> btree;      // fallback
+
+  __host__ __device__ size_t lds_bytes() const {
+    size_t b = btree.lds_bytes();
+    if (root.hidden) b += ((3 * root.hidden + 1) * 4 + 15) / 16 * 16;
+    return b;
+  }
+  __device__ void stage(uint8_t* lds) const {
+    btree.stage(lds);
+    if (root.hidden) stage_to_lds(lds + btree.lds_bytes(), root.mlp, ((3 * root.hidden + 1) * 4 + 15) / 16 * 16);
+  }
+
+  __device__ __forceinline__ uint32_t route(K x, const uint8_t* lds) const {
+    const float u = key_unit(x, min_key, root.inv_range);
+    float cdf;
+    if (root.hidden) {
+      cdf = mlp_eval(reinterpret_cast<const float*>(lds + btree.lds_bytes()), root.hidden, u);
+    } else {
+      cdf = __builtin_fmaf(root.a, u, root.b);
+    }
+    return leaf_of(cdf, num_leaves);
+  }
+
+  __device__ __forceinline__ uint32_t lookup(K x, bool active, const uint8_t* lds) const {
+    const int ln = lane();
+    const int gbase = ln & ~7;
+    const int gl = ln & 7;
+    bool use_tree = false;
+    uint64_t lo = 0, hi = 0, pred = 0;
+    bool live = active && x >= min_key && x <= max_key;
+    if (live) {
+      const LeafModel m = leaves[route(x, lds)];
+      if (m.err_hi < m.err_lo) {  // fallback leaf
+        use_tree = true;
+      } else if (m.count) {
+        const int64_t p = leaf_pred(m, leaf_dkey(x, m.base_key));
+        int64_t a = p + m.err_lo, b = p + m.err_hi + 1;
+        const int64_t s = m.start, e = static_cast<int64_t>(m.start) + m.count;
+        a = a < s ? s : a;
+        b = b > e ? e : b;
+        if (a < b) {
+          lo = static_cast<uint64_t>(a);
+          hi = static_cast<uint64_t>(b);
+          pred = static_cast<uint64_t>(p < a ? a : (p >= b ? b - 1 : p));
+        }
+      }
+    }
+    uint32_t r = group_window_search<K, 8>(col, pred, lo, hi, x, live && !use_tree, gbase, gl);
+    if (fallback_any && ballot(use_tree) != 0) {  // wave-uniform
+      const uint32_t t = btree.lookup(x, use_tree, lds);
+      if (use_tree) r = t;
+    }
+    return r;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Linear-probing hash table over the column (linear_probing_hashtable,
+// src/linear_probing_hashtable.cuh:14-98): capacity 2^ceil(log2(n / 0.5)),
+// slot = murmur3_64(key) & (capacity - 1), step +1, empty = all-ones key.
+// An entry is 16 bytes {u64 key, u32 row id, pad}; a probe reads one entry per
+// 16-byte load (the reference's ld.global.cs of a longlong2, :62).
+// ---------------------------------------------------------------------------
+struct HashEntry {
+  unsigned long long key;
+  uint32_t value;
+  uint32_t pad;
+};
+static_assert(sizeof(HashEntry) == 16, "16-byte entry");
+
+template <class K>
+struct HashDev {
+  const HashEntry* table;
+  uint64_t mask;
+
+  __host__ __device__ size_t lds_bytes() const { return 0; }
+  __device__ void stage(uint8_t*) const {}
+
+  __device__ __forceinline__ uint32_t lookup(K x, bool active, const uint8_t*) const {
+    if (!active) return kNotFound;
+    const uint64_t k = static_cast<uint64_t>(x);
+    uint64_t s = murmur3_64(k) & mask;
+    for (uint64_t probes = 0; probes <= mask; ++probes) {
+      const u32x4 e = *reinterpret_cast<const u32x4*>(table + s);
+      const uint64_t ek = (static_cast<uint64_t>(e.y) << 32) | e.x;
+      if (ek == k) return e.z;
+      if (ek == ~0ull) return kNotFound;
+      s = (s + 1) & mask;
+    }
+    return kNotFound;
+  }
+};
+
+}  // namespace gj

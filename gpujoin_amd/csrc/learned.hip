@@ -1,0 +1,512 @@
+// learned.hip - GPU training and calibration of the learned two-stage position
+// models (no reference counterpart; they close over the same exact last-mile
+// search as the reference's RadixSpline, src/indexes.cuh:205-215).
+//
+// Pipeline, all on the GPU over the (key, position) pairs of the sorted column:
+//   1. root: linear least squares of cdf = i / n on x = unit(key)  (reduction kernel), or
+//            a 1xH MLP f(x) = b2 + sum_k w2_k clamp(w1_k x + b1_k, 0, 1) whose ramps start
+//            at the column's quantiles (the piecewise-linear CDF through h + 1 quantiles) and
+//            are then trained with minibatch Adam by hand-written forward/backward kernels;
+//            w1 and w2 are projected to >= 0 after every step, so the MLP is monotone and
+//            every leaf owns a contiguous key range
+//   2. routing: leaf = floor(root(x) * L); leaf start positions by a boundary kernel
+//   3. leaves: per-leaf linear least squares of local position on FP32 key offsets
+//   4. calibration: the error of every build key under the exact FP32 lookup code
+//      gives each leaf its window [pred + err_lo, pred + err_hi]; leaves wider than the
+//      bound are routed to the B-tree
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <vector>
+
+#include "common.hpp"
+#include "lookup_dispatch.hpp"
+
+namespace gj {
+
+namespace {
+
+unsigned grid_for(uint64_t work, unsigned block, unsigned cap = 65536) {
+  uint64_t g = ceil_div(work, block);
+  if (g == 0) g = 1;
+  if (g > cap) g = cap;
+  return static_cast<unsigned>(g);
+}
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+template <class T>
+__device__ __forceinline__ T wave_sum(T v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+struct RootFitSums {
+  double sx, sxx, st, sxt;
+};
+
+// Per-block partial sums of x, x^2, t, x*t with x = unit(key_i), t = i / n.
+template <class K>
+__global__ void root_sums_kernel(const K* __restrict__ col, uint64_t n, K min_key, float inv_range,
+                                 RootFitSums* __restrict__ partial) {
+  double sx = 0, sxx = 0, st = 0, sxt = 0;
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const double x = key_unit(col[i], min_key, inv_range);
+    const double t = static_cast<double>(i) / static_cast<double>(n);
+    sx += x;
+    sxx += x * x;
+    st += t;
+    sxt += x * t;
+  }
+  sx = wave_sum(sx);
+  sxx = wave_sum(sxx);
+  st = wave_sum(st);
+  sxt = wave_sum(sxt);
+  __shared__ RootFitSums s[4];
+  const int w = threadIdx.x / 64;
+  if (lane() == 0) s[w] = {sx, sxx, st, sxt};
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    RootFitSums r{0, 0, 0, 0};
+    for (int i = 0; i < static_cast<int>(blockDim.x / 64); ++i) {
+      r.sx += s[i].sx;
+      r.sxx += s[i].sxx;
+      r.st += s[i].st;
+      r.sxt += s[i].sxt;
+    }
+    partial[blockIdx.x] = r;
+  }
+}
+
+// ---- MLP root: loss and gradient -------------------------------------------
+// Samples come either from explicit arrays (x, y) or from the sorted column at
+// seeded random positions (x = unit(key), y = position / n).
+template <class K>
+struct SampleSource {
+  const float* x;
+  const float* y;
+  const K* col;
+  uint64_t n;
+  K min_key;
+  float inv_range;
+  uint64_t seed;
+  __device__ __forceinline__ void get(uint64_t j, float& xs, float& ys) const {
+    if (x) {
+      xs = x[j];
+      ys = y[j];
+    } else {
+      const uint64_t r = mix64(seed + (j + 1) * 0x9E3779B97F4A7C15ull);
+      const uint64_t i = static_cast<uint64_t>((static_cast<unsigned __int128>(r) * n) >> 64);
+      xs = key_unit(col[i], min_key, inv_range);
+      ys = static_cast<float>(static_cast<double>(i) / static_cast<double>(n));
+    }
+  }
+};
+
+// grad[3H+1] += d/dp mean((f(x) - y)^2) over `count` samples; loss += mean squared error.
+template <int H, class K>
+__global__ __launch_bounds__(256) void mlp_grad_kernel(const float* __restrict__ params, SampleSource<K> src,
+                                                       uint64_t count, float* __restrict__ grad,
+                                                       float* __restrict__ loss) {
+  __shared__ float p[3 * H + 1];
+  __shared__ float red[3 * H + 2];
+  for (int i = threadIdx.x; i < 3 * H + 1; i += blockDim.x) p[i] = params[i];
+  for (int i = threadIdx.x; i < 3 * H + 2; i += blockDim.x) red[i] = 0.0f;
+  __syncthreads();
+  float gw1[H], gb1[H], gw2[H];
+#pragma unroll
+  for (int k = 0; k < H; ++k) gw1[k] = gb1[k] = gw2[k] = 0.0f;
+  float gb2 = 0.0f, l = 0.0f;
+  const float scale = 2.0f / static_cast<float>(count);
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t j = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; j < count; j += stride) {
+    float x, y;
+    src.get(j, x, y);
+    float a[H];
+    float f = p[3 * H];
+#pragma unroll
+    for (int k = 0; k < H; ++k) {
+      a[k] = __builtin_fmaf(p[k], x, p[H + k]);
+      f = __builtin_fmaf(p[2 * H + k], fminf(fmaxf(a[k], 0.0f), 1.0f), f);
+    }
+    const float r = f - y;
+    l = __builtin_fmaf(r, r, l);
+    const float g = scale * r;
+    gb2 += g;
+#pragma unroll
+    for (int k = 0; k < H; ++k) {
+      const float hk = fminf(fmaxf(a[k], 0.0f), 1.0f);
+      gw2[k] = __builtin_fmaf(g, hk, gw2[k]);
+      if (a[k] > 0.0f && a[k] < 1.0f) {  // the ramp passes gradient only on its slope
+        const float gw = g * p[2 * H + k];
+        gb1[k] += gw;
+        gw1[k] = __builtin_fmaf(gw, x, gw1[k]);
+      }
+    }
+  }
+  // wave sums, then block sums in LDS, then one global atomic per parameter per block
+#pragma unroll
+  for (int k = 0; k < H; ++k) {
+    gw1[k] = wave_sum(gw1[k]);
+    gb1[k] = wave_sum(gb1[k]);
+    gw2[k] = wave_sum(gw2[k]);
+  }
+  gb2 = wave_sum(gb2);
+  l = wave_sum(l);
+  if (lane() == 0) {
+#pragma unroll
+    for (int k = 0; k < H; ++k) {
+      atomicAdd(&red[k], gw1[k]);
+      atomicAdd(&red[H + k], gb1[k]);
+      atomicAdd(&red[2 * H + k], gw2[k]);
+    }
+    atomicAdd(&red[3 * H], gb2);
+    atomicAdd(&red[3 * H + 1], l);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 3 * H + 1; i += blockDim.x) atomicAdd(&grad[i], red[i]);
+  if (threadIdx.x == 0) atomicAdd(loss, red[3 * H + 1] / static_cast<float>(count));
+}
+
+// One Adam step; w1 and w2 are projected onto >= 0 (monotone root); clears grad.
+__global__ void adam_kernel(float* __restrict__ params, float* __restrict__ grad, float* __restrict__ m1,
+                            float* __restrict__ m2, uint32_t h, float lr, int step) {
+  const uint32_t i = threadIdx.x;
+  if (i >= 3 * h + 1) return;
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  const float g = grad[i];
+  grad[i] = 0.0f;
+  m1[i] = b1 * m1[i] + (1.0f - b1) * g;
+  m2[i] = b2 * m2[i] + (1.0f - b2) * g * g;
+  const float mh = m1[i] / (1.0f - powf(b1, static_cast<float>(step)));
+  const float vh = m2[i] / (1.0f - powf(b2, static_cast<float>(step)));
+  float p = params[i] - lr * mh / (sqrtf(vh) + eps);
+  const bool nonneg = (i < h) || (i >= 2 * h && i < 3 * h);
+  if (nonneg && p < 0.0f) p = 0.0f;
+  params[i] = p;
+}
+
+template <class K>
+void launch_mlp_grad(uint32_t h, const float* params, const SampleSource<K>& src, uint64_t count, float* grad,
+                     float* loss, hipStream_t s) {
+  const unsigned blocks = grid_for(count, 256, 1024);
+  switch (h) {
+    case 8: hipLaunchKernelGGL((mlp_grad_kernel<8, K>), dim3(blocks), dim3(256), 0, s, params, src, count, grad, loss); break;
+    case 16: hipLaunchKernelGGL((mlp_grad_kernel<16, K>), dim3(blocks), dim3(256), 0, s, params, src, count, grad, loss); break;
+    case 32: hipLaunchKernelGGL((mlp_grad_kernel<32, K>), dim3(blocks), dim3(256), 0, s, params, src, count, grad, loss); break;
+    case 64: hipLaunchKernelGGL((mlp_grad_kernel<64, K>), dim3(blocks), dim3(256), 0, s, params, src, count, grad, loss); break;
+    default: throw error(GJ_ERR_INVALID, "mlp hidden width must be 8, 16, 32 or 64");
+  }
+  GJ_HIP_CHECK(hipGetLastError());
+}
+
+__global__ void mlp_forward_kernel(const float* __restrict__ params, uint32_t h, const float* __restrict__ x, uint64_t n,
+                                   float* __restrict__ out) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+    out[i] = mlp_eval(params, h, x[i]);
+}
+
+// ---- routing, leaf fit, calibration ----------------------------------------
+template <class K>
+struct Router {
+  K min_key;
+  RootModel root;
+  uint32_t leaves;
+  __device__ __forceinline__ uint32_t operator()(K key) const {
+    const float u = key_unit(key, min_key, root.inv_range);
+    const float cdf = root.hidden ? mlp_eval(root.mlp, root.hidden, u) : __builtin_fmaf(root.a, u, root.b);
+    return leaf_of(cdf, leaves);
+  }
+};
+
+// start[l] = first position routed to a leaf >= l (start[] pre-filled with n); counts
+// routing inversions (which must not occur: the root is monotone).
+template <class K>
+__global__ void leaf_bounds_kernel(const K* __restrict__ col, uint64_t n, Router<K> route, uint32_t* __restrict__ start,
+                                   unsigned long long* inversions) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint32_t li = route(col[i]);
+    const int64_t lp = (i == 0) ? -1 : static_cast<int64_t>(route(col[i - 1]));
+    if (static_cast<int64_t>(li) < lp) {
+      atomicAdd(inversions, 1ull);
+      continue;
+    }
+    for (int64_t l = lp + 1; l <= static_cast<int64_t>(li); ++l) start[l] = static_cast<uint32_t>(i);
+  }
+}
+
+// One block per leaf: least squares of local position on FP32 key offsets.
+template <class K>
+__global__ __launch_bounds__(256) void leaf_fit_kernel(const K* __restrict__ col, uint64_t n,
+                                                       const uint32_t* __restrict__ start, uint32_t leaves,
+                                                       LeafModel* __restrict__ out) {
+  const uint32_t l = blockIdx.x;
+  if (l >= leaves) return;
+  const uint64_t s = start[l];
+  const uint64_t e = start[l + 1];
+  const uint64_t cnt = e > s ? e - s : 0;
+  const uint64_t base = cnt ? static_cast<uint64_t>(col[s]) : 0;
+  double sd = 0, sdd = 0, sj = 0, sdj = 0;
+  for (uint64_t j = threadIdx.x; j < cnt; j += blockDim.x) {
+    const double d = leaf_dkey(col[s + j], base);
+    const double jj = static_cast<double>(j);
+    sd += d;
+    sdd += d * d;
+    sj += jj;
+    sdj += d * jj;
+  }
+  sd = wave_sum(sd);
+  sdd = wave_sum(sdd);
+  sj = wave_sum(sj);
+  sdj = wave_sum(sdj);
+  __shared__ double red[4][4];
+  const int w = threadIdx.x / 64;
+  if (lane() == 0) {
+    red[w][0] = sd;
+    red[w][1] = sdd;
+    red[w][2] = sj;
+    red[w][3] = sdj;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < static_cast<int>(blockDim.x / 64); ++i) {
+      sd += red[i][0];
+      sdd += red[i][1];
+      sj += red[i][2];
+      sdj += red[i][3];
+    }
+    double slope = 0, icpt = 0;
+    if (cnt > 1) {
+      const double c = static_cast<double>(cnt);
+      const double var = sdd - sd * sd / c;
+      if (var > 0) slope = (sdj - sd * sj / c) / var;
+      icpt = (sj - slope * sd) / c;
+    }
+    LeafModel m;
+    m.base_key = base;
+    m.start = static_cast<uint32_t>(s);
+    m.count = static_cast<uint32_t>(cnt);
+    m.slope = static_cast<float>(slope);
+    m.icpt = static_cast<float>(icpt);
+    m.err_lo = INT_MAX;
+    m.err_hi = INT_MIN;
+    out[l] = m;
+  }
+}
+
+// err_lo/err_hi of the leaf each key is routed to, under the exact lookup arithmetic.
+template <class K>
+__global__ void leaf_error_kernel(const K* __restrict__ col, uint64_t n, Router<K> route, LeafModel* __restrict__ lm) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const K key = col[i];
+    const uint32_t l = route(key);
+    const LeafModel m = lm[l];
+    const int64_t pred = leaf_pred(m, leaf_dkey(key, m.base_key));
+    int64_t e = static_cast<int64_t>(i) - pred;
+    e = e < INT_MIN / 2 ? INT_MIN / 2 : (e > INT_MAX / 2 ? INT_MAX / 2 : e);
+    atomicMin(&lm[l].err_lo, static_cast<int>(e));
+    atomicMax(&lm[l].err_hi, static_cast<int>(e));
+  }
+}
+
+// Empty leaves get an empty window; leaves wider than max_width go to the B-tree (err_hi < err_lo).
+__global__ void leaf_finalize_kernel(LeafModel* __restrict__ lm, uint32_t leaves, int64_t max_width) {
+  const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= leaves) return;
+  LeafModel m = lm[l];
+  if (m.count == 0) {
+    m.err_lo = 0;
+    m.err_hi = 0;
+  } else if (static_cast<int64_t>(m.err_hi) - m.err_lo + 1 > max_width) {
+    m.err_lo = 1;
+    m.err_hi = 0;
+  }
+  lm[l] = m;
+}
+
+}  // namespace
+
+template <class K>
+void build_learned(gj_index& ix, hipStream_t s) {
+  const K* col = static_cast<const K*>(ix.d_col);
+  const uint64_t n = ix.n;
+  const uint32_t bits = ix.params.learned_leaf_bits;
+  GJ_REQUIRE(bits >= 1 && bits <= 24, GJ_ERR_INVALID, "learned_leaf_bits must be in [1, 24]");
+  const uint32_t L = 1u << bits;
+  const K kmin = static_cast<K>(ix.min_key);
+  const uint64_t range = ix.max_key - ix.min_key;
+  RootModel root{};
+  root.inv_range = range ? 1.0f / static_cast<float>(range) : 0.0f;
+
+  if (ix.kind == GJ_LEARNED_PLR) {
+    const unsigned blocks = grid_for(n, 256, 2048);
+    dbuf partial(sizeof(RootFitSums) * blocks);
+    hipLaunchKernelGGL(root_sums_kernel<K>, dim3(blocks), dim3(256), 0, s, col, n, kmin, root.inv_range,
+                       partial.as<RootFitSums>());
+    GJ_HIP_CHECK(hipGetLastError());
+    std::vector<RootFitSums> h(blocks);
+    GJ_HIP_CHECK(hipMemcpyAsync(h.data(), partial.p, partial.bytes, hipMemcpyDeviceToHost, s));
+    GJ_HIP_CHECK(hipStreamSynchronize(s));
+    RootFitSums t{0, 0, 0, 0};
+    for (const auto& r : h) {
+      t.sx += r.sx;
+      t.sxx += r.sxx;
+      t.st += r.st;
+      t.sxt += r.sxt;
+    }
+    const double c = static_cast<double>(n);
+    const double var = t.sxx - t.sx * t.sx / c;
+    double a = var > 0 ? (t.sxt - t.sx * t.st / c) / var : 0.0;
+    if (a < 0) a = 0;
+    root.a = static_cast<float>(a);
+    root.b = static_cast<float>((t.st - a * t.sx) / c);
+    root.hidden = 0;
+  } else {
+    const uint32_t h = ix.params.mlp_hidden;
+    GJ_REQUIRE(h == 8 || h == 16 || h == 32 || h == 64, GJ_ERR_INVALID, "mlp_hidden must be 8, 16, 32 or 64");
+    const uint32_t np = 3 * h + 1;
+    // ramp k rises from 0 at quantile k/h of the keys to 1 at quantile (k+1)/h, weight 1/h:
+    // the initial root is the piecewise-linear CDF through h + 1 quantiles
+    std::vector<float> q(h + 1);
+    for (uint32_t k = 0; k <= h; ++k) {
+      const uint64_t i = std::min<uint64_t>(n - 1, (static_cast<uint64_t>(k) * n) / h);
+      K key;
+      GJ_HIP_CHECK(hipMemcpyAsync(&key, col + i, sizeof(K), hipMemcpyDeviceToHost, s));
+      GJ_HIP_CHECK(hipStreamSynchronize(s));
+      q[k] = static_cast<float>(static_cast<uint64_t>(key - kmin)) * root.inv_range;
+    }
+    std::vector<float> p0(np, 0.0f);
+    for (uint32_t k = 0; k < h; ++k) {
+      const float w = q[k + 1] - q[k];
+      p0[k] = w > 0.0f ? 1.0f / w : 0.0f;
+      p0[h + k] = w > 0.0f ? -q[k] / w : (q[k] <= 0.0f ? 1.0f : 0.0f);
+      p0[2 * h + k] = 1.0f / static_cast<float>(h);
+    }
+    p0[3 * h] = 0.0f;
+    ix.lm_mlp.alloc(((np * 4 + 15) / 16) * 16);
+    fill_bytes(ix.lm_mlp.p, 0, ix.lm_mlp.bytes, s);
+    GJ_HIP_CHECK(hipMemcpyAsync(ix.lm_mlp.p, p0.data(), np * 4, hipMemcpyHostToDevice, s));
+    dbuf state(4 * 64 * 4 * 4);  // grad, m1, m2, loss
+    fill_bytes(state.p, 0, state.bytes, s);
+    float* grad = state.as<float>();
+    float* m1 = grad + 256;
+    float* m2 = grad + 512;
+    float* loss = grad + 768;
+    SampleSource<K> src{nullptr, nullptr, col, n, kmin, root.inv_range, ix.params.seed};
+    const uint64_t batch = std::min<uint64_t>(ix.params.mlp_batch, std::max<uint64_t>(n, 1));
+    for (uint32_t step = 1; step <= ix.params.mlp_steps; ++step) {
+      src.seed = ix.params.seed + static_cast<uint64_t>(step) * 0x632BE59BD9B4E019ull;
+      if (step == ix.params.mlp_steps) fill_bytes(loss, 0, 16, s);
+      launch_mlp_grad<K>(h, ix.lm_mlp.as<float>(), src, batch, grad, loss, s);
+      hipLaunchKernelGGL(adam_kernel, dim3(1), dim3(256), 0, s, ix.lm_mlp.as<float>(), grad, m1, m2, h,
+                         ix.params.mlp_lr, static_cast<int>(step));
+      GJ_HIP_CHECK(hipGetLastError());
+    }
+    float hl = 0;
+    GJ_HIP_CHECK(hipMemcpyAsync(&hl, loss, sizeof(float), hipMemcpyDeviceToHost, s));
+    GJ_HIP_CHECK(hipStreamSynchronize(s));
+    ix.stats.train_loss = hl;
+    root.hidden = h;
+    root.mlp = ix.lm_mlp.as<const float>();
+  }
+
+  // routing and leaf boundaries
+  Router<K> route{kmin, root, L};
+  dbuf start((static_cast<uint64_t>(L) + 1 + 3) * sizeof(uint32_t));
+  {
+    // fill start[] with n
+    std::vector<uint32_t> init(L + 1, static_cast<uint32_t>(n));
+    GJ_HIP_CHECK(hipMemcpyAsync(start.p, init.data(), init.size() * 4, hipMemcpyHostToDevice, s));
+    dbuf inv(sizeof(unsigned long long));
+    fill_bytes(inv.p, 0, inv.bytes, s);
+    hipLaunchKernelGGL(leaf_bounds_kernel<K>, dim3(grid_for(n, 256)), dim3(256), 0, s, col, n, route,
+                       start.as<uint32_t>(), inv.as<unsigned long long>());
+    GJ_HIP_CHECK(hipGetLastError());
+    unsigned long long hinv = 0;
+    GJ_HIP_CHECK(hipMemcpyAsync(&hinv, inv.p, sizeof(hinv), hipMemcpyDeviceToHost, s));
+    GJ_HIP_CHECK(hipStreamSynchronize(s));
+    GJ_REQUIRE(hinv == 0, GJ_ERR_INTERNAL, "learned index: root model is not monotone");
+  }
+  ix.lm_leaves.alloc(sizeof(LeafModel) * L);
+  hipLaunchKernelGGL(leaf_fit_kernel<K>, dim3(L), dim3(256), 0, s, col, n, start.as<const uint32_t>(), L,
+                     ix.lm_leaves.as<LeafModel>());
+  GJ_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(leaf_error_kernel<K>, dim3(grid_for(n, 256)), dim3(256), 0, s, col, n, route,
+                     ix.lm_leaves.as<LeafModel>());
+  GJ_HIP_CHECK(hipGetLastError());
+  const int64_t max_width = 2 * static_cast<int64_t>(ix.params.learned_max_error) + 2;
+  hipLaunchKernelGGL(leaf_finalize_kernel, dim3(grid_for(L, 256)), dim3(256), 0, s, ix.lm_leaves.as<LeafModel>(), L,
+                     max_width);
+  GJ_HIP_CHECK(hipGetLastError());
+
+  // statistics
+  std::vector<LeafModel> hl(L);
+  GJ_HIP_CHECK(hipMemcpyAsync(hl.data(), ix.lm_leaves.p, ix.lm_leaves.bytes, hipMemcpyDeviceToHost, s));
+  GJ_HIP_CHECK(hipStreamSynchronize(s));
+  uint64_t fb_leaves = 0, fb_keys = 0, wsum = 0, wkeys = 0;
+  uint32_t wmax = 0;
+  for (const auto& m : hl) {
+    if (m.count == 0) continue;
+    if (m.err_hi < m.err_lo) {
+      ++fb_leaves;
+      fb_keys += m.count;
+    } else {
+      const uint32_t w = static_cast<uint32_t>(m.err_hi - m.err_lo + 1);
+      wmax = std::max(wmax, w);
+      wsum += static_cast<uint64_t>(w) * m.count;
+      wkeys += m.count;
+    }
+  }
+  ix.lm_root = root;
+  ix.lm_num_leaves = L;
+  ix.lm_fallback_any = fb_leaves ? 1u : 0u;
+  ix.stats.leaves = L;
+  ix.stats.fallback_leaves = fb_leaves;
+  ix.stats.fallback_keys = fb_keys;
+  ix.stats.max_window = wmax;
+  ix.stats.avg_window = wkeys ? static_cast<double>(wsum) / static_cast<double>(wkeys) : 0.0;
+  ix.stats.depth = 2;
+  ix.stats.lds_levels = ix.bt.lds_levels;
+}
+template void build_learned<uint32_t>(gj_index&, hipStream_t);
+template void build_learned<uint64_t>(gj_index&, hipStream_t);
+
+}  // namespace gj
+
+using namespace gj;
+
+extern "C" {
+
+int gj_mlp_loss_grad(const float* d_params, uint32_t hidden, const float* d_x, const float* d_y, uint64_t n,
+                     float* d_grad, float* d_loss, gj_stream_t stream) {
+  return guarded([&] {
+    GJ_REQUIRE(d_params && d_x && d_y && d_grad && d_loss && n > 0, GJ_ERR_INVALID, "bad argument");
+    const hipStream_t s = S(stream);
+    fill_bytes(d_grad, 0, (3 * hidden + 1) * sizeof(float), s);
+    fill_bytes(d_loss, 0, sizeof(float), s);
+    SampleSource<uint64_t> src{d_x, d_y, nullptr, n, 0, 0.0f, 0};
+    launch_mlp_grad<uint64_t>(hidden, d_params, src, n, d_grad, d_loss, s);
+  });
+}
+
+int gj_mlp_forward(const float* d_params, uint32_t hidden, const float* d_x, uint64_t n, float* d_out,
+                   gj_stream_t stream) {
+  return guarded([&] {
+    GJ_REQUIRE(d_params && d_x && d_out, GJ_ERR_INVALID, "bad argument");
+    GJ_REQUIRE(hidden >= 1 && hidden <= static_cast<uint32_t>(kMaxHidden), GJ_ERR_INVALID, "bad hidden width");
+    if (n == 0) return;
+    hipLaunchKernelGGL(mlp_forward_kernel, dim3(grid_for(n, 256)), dim3(256), 0, S(stream), d_params, hidden, d_x, n,
+                       d_out);
+    GJ_HIP_CHECK(hipGetLastError());
+  });
+}
+
+}  // extern "C"

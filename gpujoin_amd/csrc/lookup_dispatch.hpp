@@ -35,18 +35,37 @@ void launch_plain(const Dev& d, const K* probes, uint64_t m, uint32_t* out, cons
   GJ_HIP_CHECK(hipGetLastError());
 }
 
+// total_max bounds the tuple count; d_total (optional) holds the exact count on the device.
 template <class Dev, class K>
-void launch_tuples(const Dev& d, const K* keys, const uint32_t* payload, uint64_t total, uint32_t* out,
-                   const gj_lookup_params& p, hipStream_t s, int cu) {
-  if (total == 0) return;
+void launch_tuples(const Dev& d, const K* keys, const uint32_t* payload, const uint64_t* d_total, uint64_t total_max,
+                   uint32_t* out, const gj_lookup_params& p, hipStream_t s, int cu) {
+  if (total_max == 0) return;
   const unsigned block = lookup_block(p);
   const size_t lds = d.lds_bytes();
   const void* fn = reinterpret_cast<const void*>(&lookup_tuples_kernel<Dev, K>);
   uint64_t blocks = p.grid_blocks ? p.grid_blocks : static_cast<uint64_t>(blocks_per_cu(fn, block, lds)) * cu;
-  blocks = std::min<uint64_t>(blocks, ceil_div(total, block));
+  blocks = std::min<uint64_t>(blocks, ceil_div(total_max, block));
+  if (blocks >= 8) blocks = blocks / 8 * 8;  // whole XCD rounds for the XCD-major chunk map
   blocks = std::max<uint64_t>(blocks, 1);
   hipLaunchKernelGGL((lookup_tuples_kernel<Dev, K>), dim3(static_cast<unsigned>(blocks)), dim3(block), lds, s, d,
-                     keys, payload, total, out);
+                     keys, payload, d_total, total_max, out);
+  GJ_HIP_CHECK(hipGetLastError());
+}
+
+constexpr int kBwsTile = 2048;
+
+template <class Dev, class K>
+void launch_bws(const Dev& d, const K* probes, uint64_t m, uint32_t* out, const gj_lookup_params& p, hipStream_t s,
+                int cu) {
+  if (m == 0) return;
+  const unsigned block = lookup_block(p);
+  const size_t lds = (d.lds_bytes() + 15) / 16 * 16 + kBwsTile * (sizeof(K) + sizeof(uint32_t));
+  const void* fn = reinterpret_cast<const void*>(&lookup_bws_kernel<Dev, K, kBwsTile>);
+  uint64_t blocks = p.grid_blocks ? p.grid_blocks : static_cast<uint64_t>(blocks_per_cu(fn, block, lds)) * cu;
+  blocks = std::min<uint64_t>(blocks, ceil_div(m, kBwsTile));
+  blocks = std::max<uint64_t>(blocks, 1);
+  hipLaunchKernelGGL((lookup_bws_kernel<Dev, K, kBwsTile>), dim3(static_cast<unsigned>(blocks)), dim3(block), lds, s,
+                     d, probes, m, out);
   GJ_HIP_CHECK(hipGetLastError());
 }
 
@@ -54,18 +73,45 @@ void build_index(gj_index& ix, hipStream_t s);
 template <class K> void build_btree(gj_index& ix, hipStream_t s);
 template <class K> void build_extended(gj_index& ix, hipStream_t s);
 
+// Calls f(descriptor) with the device descriptor of the index (every kind).
+template <class K, class F>
+void with_index_dev(const gj_index& ix, F&& f) {
+  switch (ix.kind) {
+    case GJ_BINARY_SEARCH: f(ix.bsearch_dev<K>()); return;
+    case GJ_BTREE: f(ix.btree_dev<K>()); return;
+    case GJ_NO_OP: f(NoOpDev<K>{}); return;
+    case GJ_HARMONIA:
+      switch (ix.params.harmonia_group_lanes) {
+        case 8: f(ix.harmonia_dev<K, 8>()); return;
+        case 16: f(ix.harmonia_dev<K, 16>()); return;
+        case 32:
+          if constexpr (sizeof(K) == 4) {
+            f(ix.harmonia_dev<K, 32>());
+            return;
+          }
+          break;
+      }
+      throw error(GJ_ERR_INVALID, "unsupported harmonia_group_lanes");
+    case GJ_RADIX_SPLINE: f(ix.rs_dev<K, 8>()); return;
+    case GJ_LEARNED_PLR:
+    case GJ_LEARNED_MLP: f(ix.learned_dev<K, 8>()); return;
+    case GJ_HASH: f(ix.hash_dev<K>()); return;
+  }
+  throw error(GJ_ERR_INVALID, "unknown index kind");
+}
+
+// Hash table over the column into `table` (capacity written to cap).
 template <class K>
-void lookup_plain(const gj_index& ix, const K* probes, uint64_t m, uint32_t* out, const gj_lookup_params& p,
-                  hipStream_t s);
+void build_hash_table(const gj_index& ix, dbuf& table, uint64_t& cap, hipStream_t s);
+uint64_t hash_capacity(uint64_t n);
+
+// Radix partitioner (partition.hip).
+uint64_t partition_capacity(uint64_t m, int key_bytes, uint32_t bits);
+uint64_t partition_scratch(uint64_t m, int key_bytes, uint32_t bits, uint32_t passes);
 template <class K>
-void lookup_tuples(const gj_index& ix, const K* keys, const uint32_t* payload, uint64_t total, uint32_t* out,
-                   const gj_lookup_params& p, hipStream_t s);
-template <class K>
-void dispatch_extended_plain(const gj_index& ix, const K* probes, uint64_t m, uint32_t* out,
-                             const gj_lookup_params& p, hipStream_t s);
-template <class K>
-void dispatch_extended_tuples(const gj_index& ix, const K* keys, const uint32_t* payload, uint64_t total,
-                              uint32_t* out, const gj_lookup_params& p, hipStream_t s);
+void partition_keys(const K* keys, uint64_t m, uint32_t payload_base, uint32_t bits, uint32_t ignore, uint32_t passes,
+                    K* out_keys, uint32_t* out_payload, uint64_t* start, uint64_t* count, uint8_t* scratch, int cu,
+                    hipStream_t s);
 
 // Bytes of index-private device memory held by the extended kinds.
 uint64_t extended_memory(const gj_index& ix);

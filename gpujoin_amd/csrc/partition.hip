@@ -1,0 +1,303 @@
+// partition.hip - radix partitioner (gj_partition).
+//
+// Replaces the fast-interconnects pair gpu_contiguous_prefix_sum_int64 +
+// gpu_chunked_laswwc_radix_partition_int64_int64 that run_on_stream launches
+// (src/index_lookup_partitioning.cu:300-366; declarations
+// src/gpu_radix_partition.cuh:8-45), whose sources are absent.  Semantics kept:
+// partition p of key k is (k >> ignore_bits) & (2^radix_bits - 1); every
+// partition starts on a 128-byte boundary (the reference pads partitions by
+// GPU_CACHE_LINE_SIZE, src/partitioned_relation.hpp:7-10); payload = the
+// tuple's position in the input.  Structure of arrays instead of Tuple{key,
+// payload}, and a stable order inside each partition.
+//
+// Kernels (no cooperative launch, no device-wide barrier):
+//   hist     one block per contiguous chunk, LDS histogram of 2^radix_bits u32 bins
+//   scan_p   per partition: column prefix over the blocks and the partition total
+//   scan_s   one block: aligned exclusive prefix over partitions -> partition starts
+//   scatter  one block per chunk; waves take turns in order, each assigning stable
+//            positions from LDS running offsets (wave-level peer masks by ballots)
+#include <algorithm>
+
+#include "common.hpp"
+#include "lookup_dispatch.hpp"
+
+namespace gj {
+
+constexpr int kPartBlock = 256;
+constexpr uint32_t kMaxRadixBits = 12;
+
+__host__ __device__ __forceinline__ uint32_t part_align(int key_bytes) { return 128u / static_cast<uint32_t>(key_bytes); }
+
+template <class K>
+__device__ __forceinline__ uint32_t digit_of(K k, uint32_t shift, uint32_t mask) {
+  return static_cast<uint32_t>(static_cast<uint64_t>(k) >> shift) & mask;
+}
+
+template <class K>
+__global__ __launch_bounds__(kPartBlock) void part_hist_kernel(const K* __restrict__ keys, uint64_t m, uint64_t chunk,
+                                                               uint32_t shift, uint32_t bits,
+                                                               uint32_t* __restrict__ hist) {
+  extern __shared__ uint32_t h[];
+  const uint32_t fan = 1u << bits;
+  for (uint32_t p = threadIdx.x; p < fan; p += blockDim.x) h[p] = 0;
+  __syncthreads();
+  const uint64_t lo = static_cast<uint64_t>(blockIdx.x) * chunk;
+  const uint64_t hi = std::min<uint64_t>(m, lo + chunk);
+  for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) atomicAdd(&h[digit_of(load_stream(keys + i), shift, fan - 1)], 1u);
+  __syncthreads();
+  for (uint32_t p = threadIdx.x; p < fan; p += blockDim.x) hist[static_cast<uint64_t>(blockIdx.x) * fan + p] = h[p];
+}
+
+// hist[b][p] <- sum_{b' < b} hist[b'][p]; count[p] <- total
+__global__ void part_scan_blocks_kernel(uint32_t* __restrict__ hist, uint32_t blocks, uint32_t fan,
+                                        uint64_t* __restrict__ count) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= fan) return;
+  uint64_t run = 0;
+  for (uint32_t b = 0; b < blocks; ++b) {
+    const uint32_t v = hist[static_cast<uint64_t>(b) * fan + p];
+    hist[static_cast<uint64_t>(b) * fan + p] = static_cast<uint32_t>(run);
+    run += v;
+  }
+  count[p] = run;
+}
+
+// start[p] = align_up(start[p-1] + count[p-1], align); one block of 1024 threads.
+__global__ __launch_bounds__(1024) void part_scan_starts_kernel(const uint64_t* __restrict__ count, uint32_t fan,
+                                                                uint32_t align, uint64_t* __restrict__ start) {
+  __shared__ uint64_t s[1024];
+  const uint32_t per = (fan + blockDim.x - 1) / blockDim.x;
+  const uint32_t p0 = threadIdx.x * per;
+  uint64_t local = 0;
+  for (uint32_t q = 0; q < per; ++q) {
+    const uint32_t p = p0 + q;
+    if (p < fan) local += (count[p] + align - 1) / align * align;
+  }
+  s[threadIdx.x] = local;
+  __syncthreads();
+  for (uint32_t off = 1; off < blockDim.x; off <<= 1) {  // Hillis-Steele inclusive scan
+    const uint64_t v = threadIdx.x >= off ? s[threadIdx.x - off] : 0;
+    __syncthreads();
+    s[threadIdx.x] += v;
+    __syncthreads();
+  }
+  uint64_t run = s[threadIdx.x] - local;
+  for (uint32_t q = 0; q < per; ++q) {
+    const uint32_t p = p0 + q;
+    if (p < fan) {
+      start[p] = run;
+      run += (count[p] + align - 1) / align * align;
+    }
+  }
+  if (threadIdx.x == blockDim.x - 1) start[fan] = s[threadIdx.x];  // padded total
+}
+
+// Marks the padding slots after every partition with payload 0xFFFFFFFF and key_max.
+template <class K>
+__global__ void part_pad_kernel(const uint64_t* __restrict__ start, const uint64_t* __restrict__ count, uint32_t fan,
+                                uint32_t align, K* __restrict__ keys, uint32_t* __restrict__ payload) {
+  const uint32_t p = blockIdx.x;
+  if (p >= fan) return;
+  const uint64_t end = start[p] + count[p];
+  const uint64_t stop = start[p] + (count[p] + align - 1) / align * align;
+  for (uint64_t i = end + threadIdx.x; i < stop; i += blockDim.x) {
+    keys[i] = key_max<K>();
+    payload[i] = 0xFFFFFFFFu;
+  }
+}
+
+// Stable scatter.  in_payload == nullptr: payload = payload_base + input position.
+template <class K>
+__global__ __launch_bounds__(kPartBlock) void part_scatter_kernel(
+    const K* __restrict__ keys, const uint32_t* __restrict__ in_payload, uint32_t payload_base, uint64_t m,
+    uint64_t chunk, uint32_t shift, uint32_t bits, const uint32_t* __restrict__ block_off,
+    const uint64_t* __restrict__ start, K* __restrict__ out_keys, uint32_t* __restrict__ out_payload) {
+  extern __shared__ uint32_t run[];  // running output position per partition (relative to 0)
+  const uint32_t fan = 1u << bits;
+  for (uint32_t p = threadIdx.x; p < fan; p += blockDim.x)
+    run[p] = static_cast<uint32_t>(start[p]) + block_off[static_cast<uint64_t>(blockIdx.x) * fan + p];
+  __syncthreads();
+  const uint64_t lo = static_cast<uint64_t>(blockIdx.x) * chunk;
+  const uint64_t hi = std::min<uint64_t>(m, lo + chunk);
+  const int w = threadIdx.x / kWave;
+  const int nw = blockDim.x / kWave;
+  const int ln = lane();
+  const uint64_t lt = (ln == 0) ? 0ull : (~0ull >> (64 - ln));  // lanes below this one
+  for (uint64_t tile = lo; tile < hi; tile += static_cast<uint64_t>(blockDim.x)) {
+    const uint64_t i = tile + static_cast<uint64_t>(threadIdx.x);
+    const bool valid = i < hi;
+    K k = K(0);
+    uint32_t pay = 0;
+    if (valid) {
+      k = load_stream(keys + i);
+      pay = in_payload ? load_stream(in_payload + i) : payload_base + static_cast<uint32_t>(i);
+    }
+    const uint32_t d = digit_of(k, shift, fan - 1);
+    // lanes of this wave holding the same digit
+    uint64_t peers = ballot(valid);
+    for (uint32_t b = 0; b < bits; ++b) {
+      const uint64_t set = ballot((d >> b) & 1u);
+      peers &= ((d >> b) & 1u) ? set : ~set;
+    }
+    const uint32_t rank = static_cast<uint32_t>(__popcll(peers & lt));
+    const bool leader = valid && (peers & lt) == 0;
+    for (int turn = 0; turn < nw; ++turn) {
+      if (turn == w && valid) {
+        const uint32_t pos = run[d] + rank;
+        out_keys[pos] = k;
+        out_payload[pos] = pay;
+      }
+      __syncthreads();
+      if (turn == w && leader) run[d] += static_cast<uint32_t>(__popcll(peers));
+      __syncthreads();
+    }
+  }
+}
+
+namespace {
+
+struct PartPlan {
+  uint32_t fan, blocks;
+  uint64_t chunk;
+};
+
+PartPlan plan(uint64_t m, uint32_t bits, int cu) {
+  PartPlan p;
+  p.fan = 1u << bits;
+  // a few blocks per CU; each block's histogram row lives in scratch
+  uint64_t want = static_cast<uint64_t>(cu) * 4;
+  const uint64_t min_chunk = 4096;
+  uint64_t chunk = std::max<uint64_t>(min_chunk, ceil_div(m, want));
+  chunk = ceil_div(chunk, kPartBlock) * kPartBlock;
+  p.chunk = chunk;
+  p.blocks = static_cast<uint32_t>(std::max<uint64_t>(1, ceil_div(m, chunk)));
+  return p;
+}
+
+// histograms | (2 passes: first-pass keys and payloads) | partition starts for gj_partition
+uint64_t scratch_for(uint64_t m, int key_bytes, uint32_t bits, uint32_t passes, int cu) {
+  const PartPlan p = plan(m, bits, cu);
+  uint64_t b = static_cast<uint64_t>(p.blocks) * p.fan * 4 + 256;
+  if (passes == 2) b += (m + 32) * (key_bytes + 4) + 512;  // first-pass keys + payloads
+  b += ((1ull << bits) + 2) * 8 + 16;
+  return b;
+}
+
+}  // namespace
+
+// One stable partition pass with digit (k >> shift) & (2^bits - 1).  `align`
+// pads partition starts (1 = unpadded).  Writes start[fan] and count[fan].
+template <class K>
+void partition_pass(const K* keys, const uint32_t* in_payload, uint32_t payload_base, uint64_t m, uint32_t shift,
+                    uint32_t bits, uint32_t align, K* out_keys, uint32_t* out_payload, uint64_t* start,
+                    uint64_t* count, uint32_t* hist, int cu, hipStream_t s) {
+  const PartPlan p = plan(m, bits, cu);
+  const size_t lds = p.fan * sizeof(uint32_t);
+  hipLaunchKernelGGL(part_hist_kernel<K>, dim3(p.blocks), dim3(kPartBlock), lds, s, keys, m, p.chunk, shift, bits,
+                     hist);
+  GJ_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(part_scan_blocks_kernel, dim3((p.fan + 255) / 256), dim3(256), 0, s, hist, p.blocks, p.fan, count);
+  GJ_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(part_scan_starts_kernel, dim3(1), dim3(1024), 0, s, count, p.fan, align, start);
+  GJ_HIP_CHECK(hipGetLastError());
+  if (align > 1) {
+    hipLaunchKernelGGL(part_pad_kernel<K>, dim3(p.fan), dim3(64), 0, s, start, count, p.fan, align, out_keys,
+                       out_payload);
+    GJ_HIP_CHECK(hipGetLastError());
+  }
+  hipLaunchKernelGGL(part_scatter_kernel<K>, dim3(p.blocks), dim3(kPartBlock), lds, s, keys, in_payload, payload_base,
+                     m, p.chunk, shift, bits, hist, start, out_keys, out_payload);
+  GJ_HIP_CHECK(hipGetLastError());
+}
+
+uint64_t partition_capacity(uint64_t m, int key_bytes, uint32_t bits) {
+  return m + (static_cast<uint64_t>(1) << bits) * part_align(key_bytes);
+}
+
+uint64_t partition_scratch(uint64_t m, int key_bytes, uint32_t bits, uint32_t passes) {
+  return scratch_for(m, key_bytes, bits, passes, device_cu_count());
+}
+
+// Full partitioning into `out` (1 or 2 passes); scratch must hold partition_scratch() bytes;
+// start must hold 2^bits + 1 entries (start[2^bits] = padded total).
+template <class K>
+void partition_keys(const K* keys, uint64_t m, uint32_t payload_base, uint32_t bits, uint32_t ignore, uint32_t passes,
+                    K* out_keys, uint32_t* out_payload, uint64_t* start, uint64_t* count, uint8_t* scratch, int cu,
+                    hipStream_t s) {
+  const PartPlan p = plan(m, bits, cu);
+  uint32_t* hist = reinterpret_cast<uint32_t*>(scratch);
+  const uint32_t align = part_align(sizeof(K));
+  if (passes == 1 || bits < 2) {
+    partition_pass<K>(keys, nullptr, payload_base, m, ignore, bits, align, out_keys, out_payload, start, count, hist,
+                      cu, s);
+    return;
+  }
+  // pass 1: the high half of the digit, unpadded, into scratch; pass 2: the full digit.
+  const uint32_t b_hi = (bits + 1) / 2;
+  uint8_t* tmp = scratch + ceil_div(static_cast<uint64_t>(p.blocks) * p.fan * 4 + 256, 256) * 256;
+  K* tk = reinterpret_cast<K*>(tmp);
+  uint32_t* tp = reinterpret_cast<uint32_t*>(tmp + ceil_div((m + 32) * sizeof(K), 256) * 256);
+  partition_pass<K>(keys, nullptr, payload_base, m, ignore + (bits - b_hi), b_hi, 1, tk, tp, start, count, hist, cu, s);
+  partition_pass<K>(tk, tp, 0, m, ignore, bits, align, out_keys, out_payload, start, count, hist, cu, s);
+}
+template void partition_keys<uint32_t>(const uint32_t*, uint64_t, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t*,
+                                       uint32_t*, uint64_t*, uint64_t*, uint8_t*, int, hipStream_t);
+template void partition_keys<uint64_t>(const uint64_t*, uint64_t, uint32_t, uint32_t, uint32_t, uint32_t, uint64_t*,
+                                       uint32_t*, uint64_t*, uint64_t*, uint8_t*, int, hipStream_t);
+
+}  // namespace gj
+
+using namespace gj;
+
+extern "C" {
+
+uint64_t gj_partition_capacity(uint64_t m, int key_bytes, uint32_t radix_bits) {
+  if ((key_bytes != 4 && key_bytes != 8) || radix_bits > kMaxRadixBits) return 0;
+  return partition_capacity(m, key_bytes, radix_bits);
+}
+
+uint64_t gj_partition_scratch_bytes(uint64_t m, int key_bytes, const gj_partition_params* p) {
+  if (!p || (key_bytes != 4 && key_bytes != 8) || p->radix_bits > kMaxRadixBits) return 0;
+  uint64_t r = 0;
+  const int st = guarded([&] { r = partition_scratch(m, key_bytes, p->radix_bits, p->passes == 2 ? 2 : 1); });
+  return st == GJ_OK ? r : 0;
+}
+
+int gj_partition(const void* d_keys, uint64_t m, int key_bytes, const gj_partition_params* params,
+                 gj_partitioned* out, void* d_scratch, uint64_t scratch_bytes, gj_stream_t stream) {
+  return guarded([&] {
+    GJ_REQUIRE(params && out, GJ_ERR_INVALID, "null argument");
+    GJ_REQUIRE(key_bytes == 4 || key_bytes == 8, GJ_ERR_INVALID, "key_bytes must be 4 or 8");
+    const uint32_t bits = params->radix_bits;
+    GJ_REQUIRE(bits >= 1 && bits <= kMaxRadixBits, GJ_ERR_INVALID, "radix_bits must be in [1, 12]");
+    GJ_REQUIRE(params->ignore_bits < static_cast<uint32_t>(8 * key_bytes), GJ_ERR_INVALID,
+               "ignore_bits must be below the key width");
+    GJ_REQUIRE(params->passes == 1 || params->passes == 2, GJ_ERR_INVALID, "passes must be 1 or 2");
+    GJ_REQUIRE(m < 0xFFFFFFFFull, GJ_ERR_OVERFLOW, "partitioning needs m < 2^32 - 1 (u32 payloads)");
+    GJ_REQUIRE(out->d_offsets && out->d_counts, GJ_ERR_INVALID, "null offsets / counts");
+    const uint64_t cap = partition_capacity(m, key_bytes, bits);
+    GJ_REQUIRE(out->capacity >= cap, GJ_ERR_INVALID, "gj_partitioned.capacity < gj_partition_capacity()");
+    GJ_REQUIRE(cap < 0xFFFFFFFFull, GJ_ERR_OVERFLOW, "partitioned relation exceeds 2^32 tuples");
+    const int cu = device_cu_count();
+    const uint64_t need = scratch_for(m, key_bytes, bits, params->passes, cu);
+    GJ_REQUIRE(d_scratch && scratch_bytes >= need, GJ_ERR_INVALID, "scratch smaller than gj_partition_scratch_bytes()");
+    const hipStream_t s = S(stream);
+    if (m == 0) {
+      fill_bytes(out->d_offsets, 0, (1ull << bits) * 8, s);
+      fill_bytes(out->d_counts, 0, (1ull << bits) * 8, s);
+      return;
+    }
+    GJ_REQUIRE(d_keys && out->d_keys && out->d_payloads, GJ_ERR_INVALID, "null buffer");
+    // partition starts (2^bits + 1 entries, the last is the padded total) live at the end of scratch
+    uint8_t* sc = static_cast<uint8_t*>(d_scratch);
+    uint64_t* start = reinterpret_cast<uint64_t*>(sc + need - ((1ull << bits) + 1) * 8 - 8);
+    start = reinterpret_cast<uint64_t*>(reinterpret_cast<uintptr_t>(start) & ~static_cast<uintptr_t>(7));
+    dispatch_key(key_bytes, [&]<class K>() {
+      partition_keys<K>(static_cast<const K*>(d_keys), m, 0, bits, params->ignore_bits, params->passes,
+                        static_cast<K*>(out->d_keys), out->d_payloads, start, out->d_counts, sc, cu, s);
+    });
+    GJ_HIP_CHECK(hipMemcpyAsync(out->d_offsets, start, (1ull << bits) * 8, hipMemcpyDeviceToDevice, s));
+  });
+}
+
+}  // extern "C"

@@ -48,6 +48,10 @@ int blocks_per_cu(const void* fn, unsigned block, size_t lds) {
 __global__ void fill_kernel(uint8_t* __restrict__ d, uint8_t v, uint64_t bytes) {
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
   const uint64_t i0 = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (reinterpret_cast<uintptr_t>(d) & 15u) {  // unaligned target: byte stores
+    for (uint64_t i = i0; i < bytes; i += stride) d[i] = v;
+    return;
+  }
   const uint64_t words = bytes / 16;
   const uint32_t w = 0x01010101u * v;
   const u32x4 pattern = {w, w, w, w};
@@ -58,7 +62,6 @@ __global__ void fill_kernel(uint8_t* __restrict__ d, uint8_t v, uint64_t bytes) 
 
 void fill_bytes(void* d, int value, uint64_t bytes, hipStream_t s) {
   if (bytes == 0) return;
-  GJ_REQUIRE((reinterpret_cast<uintptr_t>(d) & 15u) == 0, GJ_ERR_INVALID, "fill target must be 16-byte aligned");
   uint64_t blocks = ceil_div(bytes / 16 + 1, 256);
   if (blocks > 4096) blocks = 4096;
   hipLaunchKernelGGL(fill_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, static_cast<uint8_t*>(d),

@@ -268,8 +268,9 @@ int gj_autotune_lookup(const gj_index* index, const void* d_probe_sample, uint64
                        gj_lookup_params* best, double* best_lookups_per_s, gj_stream_t stream);
 
 /* ---- Learned-model kernels exposed for gradient checks (no reference counterpart)
- * MLP root f(x) = w2 . relu(w1 * x + b1) + b2 with hidden width h, params laid
- * out [w1(h), b1(h), w2(h), b2(1)] in FP32.  Loss = mean((f(x) - y)^2).
+ * MLP root f(x) = b2 + sum_k w2_k * ramp(w1_k * x + b1_k), ramp(a) = min(max(a, 0), 1),
+ * hidden width h (8, 16, 32 or 64), params laid out [w1(h), b1(h), w2(h), b2(1)] in
+ * FP32.  Loss = mean((f(x) - y)^2).
  * Writes the loss and d loss / d params (3h + 1 floats). */
 int gj_mlp_loss_grad(const float* d_params, uint32_t hidden, const float* d_x, const float* d_y, uint64_t n,
                      float* d_grad, float* d_loss, gj_stream_t stream);

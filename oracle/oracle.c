@@ -846,17 +846,17 @@ uint64_t orc_splitmix64(uint64_t seed, uint64_t i) {
 
 /* ------------------------------------------------------------------------
  * Learned-index MLP root (no reference counterpart):
- *   f(x) = sum_j w2_j * relu(w1_j * x + b1_j) + b2,  params [w1(h), b1(h), w2(h), b2]
+ *   f(x) = sum_j w2_j * ramp(w1_j * x + b1_j) + b2,  ramp(a) = min(max(a, 0), 1),
+ *   params [w1(h), b1(h), w2(h), b2]
  *   loss = mean((f(x) - y)^2); gradients accumulated in double.
  * ------------------------------------------------------------------------ */
+static double orc_ramp(double a) { return a < 0 ? 0 : (a > 1 ? 1 : a); }
+
 void orc_mlp_forward(const float* p, uint32_t h, const float* x, uint64_t n, float* out) {
   const float *w1 = p, *b1 = p + h, *w2 = p + 2 * h, b2 = p[3 * h];
   for (uint64_t i = 0; i < n; ++i) {
     double f = b2;
-    for (uint32_t j = 0; j < h; ++j) {
-      double a = (double)w1[j] * x[i] + b1[j];
-      if (a > 0) f += (double)w2[j] * a;
-    }
+    for (uint32_t j = 0; j < h; ++j) f += (double)w2[j] * orc_ramp((double)w1[j] * x[i] + b1[j]);
     out[i] = (float)f;
   }
 }
@@ -867,18 +867,15 @@ double orc_mlp_loss_grad(const float* p, uint32_t h, const float* x, const float
   double loss = 0;
   for (uint64_t i = 0; i < n; ++i) {
     double f = b2;
-    for (uint32_t j = 0; j < h; ++j) {
-      double a = (double)w1[j] * x[i] + b1[j];
-      if (a > 0) f += (double)w2[j] * a;
-    }
+    for (uint32_t j = 0; j < h; ++j) f += (double)w2[j] * orc_ramp((double)w1[j] * x[i] + b1[j]);
     double r = f - y[i];
     loss += r * r;
     double g = 2.0 * r / (double)n;
     grad[3 * h] += g;
     for (uint32_t j = 0; j < h; ++j) {
       double a = (double)w1[j] * x[i] + b1[j];
-      if (a > 0) {
-        grad[2 * h + j] += g * a;
+      grad[2 * h + j] += g * orc_ramp(a);
+      if (a > 0 && a < 1) {
         grad[j] += g * w2[j] * x[i];
         grad[h + j] += g * w2[j];
       }

@@ -1,0 +1,98 @@
+"""GPU parity for the radix partitioner and the lookup drivers (approaches).
+
+The partitioner is compared with the oracle's stable partition (counts, the key
+and payload sequence of every partition, 128-byte aligned starts, padding
+markers).  Every approach must return the oracle's row ids for every index kind:
+the output of the partitioned, blockwise-sorted and hash-join drivers is
+independent of the partition layout.
+"""
+import numpy as np
+import pytest
+
+from gpujoin_amd import _lib
+
+pytestmark = pytest.mark.gpu
+
+KINDS = ["binary_search", "btree", "harmonia", "radix_spline", "learned_plr", "learned_mlp", "hash", "no_op"]
+
+
+@pytest.mark.parametrize("kb", [4, 8])
+@pytest.mark.parametrize("bits,passes", [(1, 1), (4, 1), (8, 1), (10, 1), (12, 1), (8, 2), (12, 2)])
+@pytest.mark.parametrize("m", [0, 1, 1000, 100003])
+def test_partition_matches_oracle(gj, orc, gpu, kb, bits, passes, m):
+    print(f"[case] partition kb={kb} bits={bits} passes={passes} m={m}", flush=True)
+    rng = np.random.default_rng(bits * 7 + m)
+    dt = np.uint64 if kb == 8 else np.uint32
+    keys = rng.integers(0, np.iinfo(dt).max, size=m, dtype=dt)
+    ignore = 8 * kb - bits - 3
+    d = gj.DeviceArray.from_numpy(keys) if m else gj.DeviceArray(1, dt)
+    dk, dp, off, cnt = gj.partition(d, m, bits, ignore, passes)
+    gj.synchronize()
+    counts, ok, op = orc.partition(keys, bits, ignore)
+    got_cnt = cnt.numpy()
+    assert np.array_equal(got_cnt, counts)
+    if m == 0:
+        return
+    offs = off.numpy()
+    align = 128 // kb
+    assert np.all(offs % align == 0)
+    assert np.all(offs[1:] >= offs[:-1] + counts[:-1])
+    gk, gp = dk.numpy(), dp.numpy()
+    starts = np.concatenate([[0], np.cumsum(counts)[:-1]]).astype(np.int64)
+    for p in np.nonzero(counts)[0]:
+        o, c, s = int(offs[p]), int(counts[p]), int(starts[p])
+        assert np.array_equal(gk[o:o + c], ok[s:s + c]), p
+        assert np.array_equal(gp[o:o + c], op[s:s + c]), p
+        pad_end = o + (c + align - 1) // align * align
+        assert np.all(gp[o + c:pad_end] == 0xFFFFFFFF)
+
+
+def test_partition_rejects_bad_args(gj, gpu):
+    keys = gj.DeviceArray(16, np.uint64)
+    with pytest.raises(gj.GjError):
+        gj.partition(keys, 16, 13, 0)
+    with pytest.raises(gj.GjError):
+        gj.partition(keys, 16, 4, 64)
+
+
+def _setup(gj, kind, n, kb, dist=_lib.KEYS_UNIFORM):
+    n = (n // 32) * 32  # harmonia needs a multiple of 32
+    keys = gj.gen_build_keys(n, kb, dist, 0, 0x21 + kb)
+    col = gj.DeviceArray.from_numpy(keys)
+    params = gj.index_params(learned_leaf_bits=12, mlp_steps=100)
+    return keys, col, gj.Index(kind, col, host_column=keys, params=params)
+
+
+@pytest.mark.parametrize("kind", KINDS)
+@pytest.mark.parametrize("kb", [4, 8])
+@pytest.mark.parametrize("approach,extra", [
+    ("bws", {}),
+    ("partitioning", {"radix_bits": 8, "window": 1 << 14}),
+    ("partitioning", {"radix_bits": 12, "passes": 2, "window": 100000}),
+    ("partitioning", {"radix_bits": 10}),
+    ("hj", {}),
+])
+def test_approaches_match_oracle(gj, orc, gpu, kind, kb, approach, extra):
+    print(f"[case] {kind} kb={kb} {approach} {extra}", flush=True)
+    keys, col, idx = _setup(gj, kind, 1 << 18, kb)
+    probes, _ = gj.gen_probes(keys, 300007, _lib.PROBES_UNIFORM, 1.0, 0.1, 0, 0x22)
+    dp = gj.DeviceArray.from_numpy(probes)
+    out = gj.DeviceArray(probes.size, np.uint32)
+    out.fill(0x5A)
+    idx.lookup(dp, out, approach=approach, **extra)
+    gj.synchronize()
+    want = orc.lookup_lower_bound(keys, probes)
+    if kind == "no_op" and approach != "hj":
+        want = np.full_like(want, gj.NOT_FOUND)
+    assert np.array_equal(out.numpy(), want)
+
+
+def test_partitioned_zipf_probes(gj, orc, gpu):
+    keys, col, idx = _setup(gj, "btree", 1 << 20, 8)
+    probes, exp = gj.gen_probes(keys, 1 << 20, _lib.PROBES_ZIPF, 1.0, 0.0, 0, 0x23)
+    dp = gj.DeviceArray.from_numpy(probes)
+    out = gj.DeviceArray(probes.size, np.uint32)
+    idx.lookup(dp, out, approach="partitioning", radix_bits=10, window=1 << 18)
+    gj.synchronize()
+    assert np.array_equal(out.numpy(), exp)
+    assert np.array_equal(orc.lookup_lower_bound(keys, probes), exp)

@@ -1,0 +1,173 @@
+"""GPU parity for Harmonia, RadixSpline, the learned models and the hash table.
+
+Every row id is compared bit for bit with the oracle's equality-checked
+lower_bound on the same seeded inputs (hits, misses, smallest/largest keys).
+"""
+import numpy as np
+import pytest
+
+from gpujoin_amd import _lib
+
+pytestmark = pytest.mark.gpu
+
+
+def _probes(gj, keys, m, seed, miss=0.1):
+    p, _ = gj.gen_probes(keys, m, _lib.PROBES_UNIFORM, 1.0, miss, 0, seed)
+    kmax = np.iinfo(keys.dtype).max
+    edge = np.array([keys[0], keys[-1], 0, kmax - 1, max(int(keys[0]) - 1, 0), min(int(keys[-1]) + 1, kmax - 1)],
+                    dtype=keys.dtype)
+    return np.concatenate([edge, p])
+
+
+def _lookup(gj, kind, keys, probes, params=None, **lp):
+    col = gj.DeviceArray.from_numpy(keys)
+    idx = gj.Index(kind, col, host_column=keys, params=params)
+    dp = gj.DeviceArray.from_numpy(probes)
+    out = gj.DeviceArray(probes.size, np.uint32)
+    idx.lookup(dp, out, **lp)
+    gj.synchronize()
+    res = out.numpy()
+    st = idx.stats()
+    idx.close()
+    return res, st
+
+
+# ---- Harmonia ---------------------------------------------------------------
+
+@pytest.mark.parametrize("kb", [4, 8])
+@pytest.mark.parametrize("n", [32, 64, 32 * 33, 32 * 34, 32 * 33 * 33 + 32 * 7, 1 << 20])
+def test_harmonia_matches_oracle(gj, orc, gpu, kb, n):
+    print(f"[case] harmonia kb={kb} n={n}", flush=True)
+    keys = gj.gen_build_keys(n, kb, _lib.KEYS_UNIFORM, 0, 100 + n)
+    probes = _probes(gj, keys, 20000, 200 + n)
+    want = orc.lookup_lower_bound(keys, probes)
+    for g in ([8, 16] if kb == 8 else [8, 16, 32]):
+        got, st = _lookup(gj, "harmonia", keys, probes, params=gj.index_params(harmonia_group_lanes=g))
+        assert np.array_equal(got, want), f"group lanes {g}"
+    # the reference Harmonia built over the same column has the same depth and answers
+    h = orc.Harmonia(keys, 33)
+    assert st["depth"] == h.depth()
+    assert np.array_equal(h.lookup(probes), want)
+
+
+def test_harmonia_requires_multiple_of_32(gj, gpu):
+    keys = gj.gen_build_keys(1000, 8, _lib.KEYS_UNIFORM, 0, 7)
+    col = gj.DeviceArray.from_numpy(keys)
+    with pytest.raises(gj.GjError) as e:
+        gj.Index("harmonia", col)
+    assert e.value.status == _lib.GJ_ERR_INVALID
+
+
+def test_harmonia_dense_rowid_equals_key(gj, gpu):
+    n = 32 * 40000
+    keys = gj.gen_build_keys(n, 8, _lib.KEYS_DENSE, 0, 1)
+    probes = np.random.default_rng(5).integers(0, n + 500, size=50000).astype(np.uint64)
+    got, _ = _lookup(gj, "harmonia", keys, probes)
+    assert np.array_equal(got, np.where(probes < n, probes, gj.NOT_FOUND).astype(np.uint32))
+
+
+# ---- RadixSpline ------------------------------------------------------------
+
+@pytest.mark.parametrize("kb", [4, 8])
+@pytest.mark.parametrize("err", [8, 32, 64])
+@pytest.mark.parametrize("dist", [_lib.KEYS_UNIFORM, _lib.KEYS_LOGNORMAL, _lib.KEYS_CLUSTERED])
+def test_radix_spline_matches_oracle(gj, orc, gpu, kb, err, dist):
+    n = 200003
+    print(f"[case] radix_spline kb={kb} err={err} dist={dist}", flush=True)
+    keys = gj.gen_build_keys(n, kb, dist, 0, 300 + err + dist)
+    probes = _probes(gj, keys, 30000, 400 + err)
+    want = orc.lookup_lower_bound(keys, probes)
+    got, st = _lookup(gj, "radix_spline", keys, probes, params=gj.index_params(rs_max_error=err, rs_radix_bits=18))
+    assert np.array_equal(got, want)
+    ref = orc.RadixSpline(keys, 18, err)
+    px, _, _ = ref.arrays()
+    assert st["spline_points"] == px.size  # same GreedySplineCorridor knot count as the oracle's builder
+    assert ref.max_abs_error() <= err + 1
+    assert np.array_equal(ref.lookup(probes), want)
+
+
+@pytest.mark.parametrize("bits", [1, 4, 12])
+def test_radix_spline_radix_bits(gj, orc, gpu, bits):
+    keys = gj.gen_build_keys(50000, 8, _lib.KEYS_UNIFORM, 0, 17)
+    probes = _probes(gj, keys, 20000, 18)
+    got, _ = _lookup(gj, "radix_spline", keys, probes, params=gj.index_params(rs_radix_bits=bits, rs_max_error=16))
+    assert np.array_equal(got, orc.lookup_lower_bound(keys, probes))
+
+
+# ---- learned two-stage models -------------------------------------------------
+
+@pytest.mark.parametrize("kind", ["learned_plr", "learned_mlp"])
+@pytest.mark.parametrize("dist", [_lib.KEYS_UNIFORM, _lib.KEYS_LOGNORMAL, _lib.KEYS_CLUSTERED])
+@pytest.mark.parametrize("kb", [4, 8])
+def test_learned_matches_oracle(gj, orc, gpu, kind, dist, kb):
+    n = 1 << 18
+    print(f"[case] {kind} dist={dist} kb={kb}", flush=True)
+    keys = gj.gen_build_keys(n, kb, dist, 0, 500 + dist)
+    probes = _probes(gj, keys, 40000, 600 + dist)
+    want = orc.lookup_lower_bound(keys, probes)
+    params = gj.index_params(learned_leaf_bits=10, learned_max_error=16, mlp_steps=150)
+    got, st = _lookup(gj, kind, keys, probes, params=params)
+    assert np.array_equal(got, want)
+    assert st["leaves"] == 1 << 10
+    assert st["max_window"] <= 2 * 16 + 2
+
+
+def test_learned_fallback_is_exact(gj, orc, gpu):
+    # a tiny bound forces most leaves onto the B-tree fallback; answers stay exact
+    keys = gj.gen_build_keys(1 << 17, 8, _lib.KEYS_LOGNORMAL, 0, 77)
+    probes = _probes(gj, keys, 30000, 78)
+    got, st = _lookup(gj, "learned_plr", keys, probes,
+                      params=gj.index_params(learned_leaf_bits=6, learned_max_error=1))
+    assert st["fallback_leaves"] > 0
+    assert np.array_equal(got, orc.lookup_lower_bound(keys, probes))
+
+
+def test_mlp_root_beats_linear_root_on_skewed_keys(gj, gpu):
+    # what the trained network gains: on lognormal keys the linear root crowds most
+    # keys into a few leaves; the trained monotone MLP root spreads them, so far fewer
+    # keys need the B-tree fallback and the mean last-mile window shrinks
+    keys = gj.gen_build_keys(1 << 20, 8, _lib.KEYS_LOGNORMAL, 63, 0x31)
+    params = gj.index_params(learned_leaf_bits=12, learned_max_error=32)
+    col = gj.DeviceArray.from_numpy(keys)
+    plr = gj.Index("learned_plr", col, host_column=keys, params=params).stats()
+    mlp = gj.Index("learned_mlp", col, host_column=keys, params=params).stats()
+    print(f"[gain] plr fallback_keys={plr['fallback_keys']} avg_window={plr['avg_window']:.1f}; "
+          f"mlp fallback_keys={mlp['fallback_keys']} avg_window={mlp['avg_window']:.1f} loss={mlp['train_loss']:.2e}",
+          flush=True)
+    assert mlp["fallback_keys"] < plr["fallback_keys"]
+
+
+def test_mlp_gradients_match_oracle(gj, orc, gpu):
+    rng = np.random.default_rng(11)
+    h = 32
+    p = rng.normal(0, 0.5, size=3 * h + 1).astype(np.float32)
+    x = rng.random(4096).astype(np.float32)
+    y = np.sqrt(x).astype(np.float32)
+    dp, dx, dy = gj.DeviceArray.from_numpy(p), gj.DeviceArray.from_numpy(x), gj.DeviceArray.from_numpy(y)
+    dg = gj.DeviceArray(3 * h + 1, np.float32)
+    dl = gj.DeviceArray(1, np.float32)
+    gj.mlp_loss_grad(dp, h, dx, dy, dg, dl)
+    gj.synchronize()
+    loss_ref, grad_ref = orc.mlp_loss_grad(p, h, x, y)
+    # FP32 accumulation on the GPU vs double on the CPU: 1e-4 relative to the gradient scale
+    tol = 1e-4 * max(1.0, float(np.abs(grad_ref).max()))
+    assert np.allclose(dg.numpy(), grad_ref, rtol=1e-3, atol=tol)
+    assert abs(float(dl.numpy()[0]) - loss_ref) <= 1e-4 * max(1.0, loss_ref)
+    out = gj.DeviceArray(x.size, np.float32)
+    gj.mlp_forward(dp, h, dx, out)
+    gj.synchronize()
+    assert np.allclose(out.numpy(), orc.mlp_forward(p, h, x), rtol=1e-5, atol=1e-5)
+
+
+# ---- hash table ---------------------------------------------------------------
+
+@pytest.mark.parametrize("kb", [4, 8])
+@pytest.mark.parametrize("n", [1, 1000, 1 << 20])
+def test_hash_matches_oracle(gj, orc, gpu, kb, n):
+    keys = gj.gen_build_keys(n, kb, _lib.KEYS_UNIFORM, 0, 900 + n)
+    probes = _probes(gj, keys, 30000, 901 + n)
+    want = orc.lookup_lower_bound(keys, probes)
+    got, st = _lookup(gj, "hash", keys, probes)
+    assert np.array_equal(got, want)
+    assert np.array_equal(orc.hash_join(keys, probes), want)
+    assert st["memory_bytes"] >= orc.hash_capacity(n) * 16
