@@ -1,0 +1,460 @@
+// batch.hip - fact-dimension join-aggregate and batched execution plans.
+//
+// gj_join_aggregate replaces the Q14 probe kernels: the filter / lookup /
+// aggregate of hj_probe_kernel (src/tpch/tpch_14_hj.cu:21-71, the correct
+// shipdate predicate lo <= d < hi, :39-42) and ij_plain_kernel
+// (src/tpch/tpch_14_ij.cu:34-85), with the lane-refill idea of
+// ij_np_lane_refill_scan_lookup (src/tpch/tpch_14_ij.cu:464-566): qualifying
+// rows are queued per wave in LDS and looked up 64 at a time, so the
+// cooperative index lookup always runs on a full wave although only ~1 % of
+// the fact rows pass the filter.  Only the date column is read for every row;
+// fkey, price and discount are read for qualifying rows only.  Sums are exact
+// int64: wave shuffle reduction, then one 64-bit atomic per wave.
+//
+// Plans run a fixed list of uneven batches: one launch per batch, the same
+// launches captured once into a hipGraph, or one persistent kernel whose waves
+// pull (batch, tile) work items from a device counter until all are done.
+#include <algorithm>
+#include <vector>
+
+#include "common.hpp"
+#include "lookup_dispatch.hpp"
+
+namespace gj {
+
+constexpr int kAggBlock = 256;
+constexpr int kAggQueue = 128;              // queued qualifying rows per wave
+constexpr uint64_t kAggTileRows = 8192;     // fact rows per persistent work item
+constexpr uint64_t kLookupTileRows = 16384; // probes per persistent work item
+
+struct FactBatchDev {
+  const int32_t* fkey;
+  const int32_t* date;
+  const int64_t* price;
+  const int8_t* disc;
+  uint64_t rows;
+  uint64_t tile_start;  // first work item of this batch (persistent plans)
+  gj_agg_result* result;
+};
+
+struct ProbeBatchDev {
+  const void* probes;
+  uint32_t* out;
+  uint64_t m;
+  uint64_t tile_start;
+};
+
+struct AggAcc {
+  int64_t num = 0, den = 0;
+  uint64_t matches = 0;
+};
+
+__device__ __forceinline__ int64_t wave_sum_i64(int64_t v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Per-wave LDS queue of qualifying rows (dimension key, summand).
+struct AggQueue {
+  uint32_t* key;
+  int64_t* val;
+  int count;  // wave-uniform
+};
+
+// Looks up the 64 (or fewer, at a flush) newest queued rows on the full wave.
+template <class Dev>
+__device__ __forceinline__ void agg_drain(const Dev& d, const uint8_t* lds, AggQueue& q, int take,
+                                          const uint8_t* __restrict__ flag, AggAcc& acc) {
+  const int ln = lane();
+  const bool active = ln < take;
+  const int slot = q.count - take + ln;
+  const uint32_t k = active ? q.key[slot] : 0u;
+  const int64_t s = active ? q.val[slot] : 0;
+  const uint32_t r = d.lookup(k, active, lds);
+  if (active && r != kNotFound) {
+    acc.den += s;
+    if (flag[r]) acc.num += s;
+    acc.matches += 1;
+  }
+  q.count -= take;
+}
+
+// Rows [r0, r1) of one fact batch, 64 consecutive rows per wave step.
+template <class Dev>
+__device__ __forceinline__ void agg_rows(const Dev& d, const uint8_t* lds, AggQueue& q, const FactBatchDev& b,
+                                         uint64_t r0, uint64_t r1, uint64_t step, int32_t lo, int32_t hi,
+                                         const uint8_t* __restrict__ flag, AggAcc& acc) {
+  const int ln = lane();
+  const uint64_t lt = (ln == 0) ? 0ull : (~0ull >> (64 - ln));
+  for (uint64_t base = r0; base < r1; base += step) {
+    const uint64_t i = base + static_cast<uint64_t>(ln);
+    bool qual = false;
+    if (i < r1) {
+      const int32_t dt = load_stream(b.date + i);
+      qual = dt >= lo && dt < hi;
+    }
+    const uint64_t mask = ballot(qual);
+    if (mask == 0) continue;
+    if (qual) {
+      const int pos = q.count + __popcll(mask & lt);
+      q.key[pos] = static_cast<uint32_t>(b.fkey[i]);
+      q.val[pos] = b.price[i] * (100 - static_cast<int64_t>(b.disc[i]));
+    }
+    q.count += __popcll(mask);
+    __builtin_amdgcn_wave_barrier();
+    if (q.count >= kWave) agg_drain(d, lds, q, kWave, flag, acc);
+  }
+}
+
+template <class Dev>
+__device__ __forceinline__ void agg_flush(const Dev& d, const uint8_t* lds, AggQueue& q,
+                                          const uint8_t* __restrict__ flag, AggAcc& acc, gj_agg_result* out) {
+  while (q.count > 0) agg_drain(d, lds, q, q.count < kWave ? q.count : kWave, flag, acc);
+  const int64_t num = wave_sum_i64(acc.num);
+  const int64_t den = wave_sum_i64(acc.den);
+  const int64_t mt = wave_sum_i64(static_cast<int64_t>(acc.matches));
+  if (lane() == 0 && (num | den | mt) != 0) {
+    atomicAdd(reinterpret_cast<unsigned long long*>(&out->numerator), static_cast<unsigned long long>(num));
+    atomicAdd(reinterpret_cast<unsigned long long*>(&out->denominator), static_cast<unsigned long long>(den));
+    atomicAdd(reinterpret_cast<unsigned long long*>(&out->matches), static_cast<unsigned long long>(mt));
+  }
+  acc = AggAcc{};
+}
+
+template <class Dev>
+__device__ __forceinline__ AggQueue wave_queue(uint8_t* lds_tail) {
+  const int w = threadIdx.x / kWave;
+  AggQueue q;
+  q.key = reinterpret_cast<uint32_t*>(lds_tail) + w * kAggQueue;
+  q.val = reinterpret_cast<int64_t*>(lds_tail + (kAggBlock / kWave) * kAggQueue * 4) + w * kAggQueue;
+  q.count = 0;
+  return q;
+}
+
+inline size_t agg_queue_bytes() { return (kAggBlock / kWave) * kAggQueue * (4 + 8); }
+
+// One batch, grid-stride over 64-row steps.
+template <class Dev>
+__global__ __launch_bounds__(kAggBlock) void join_agg_kernel(Dev d, FactBatchDev b, int32_t lo, int32_t hi,
+                                                             const uint8_t* __restrict__ flag) {
+  extern __shared__ __attribute__((aligned(16))) uint8_t lds[];
+  d.stage(lds);
+  const size_t ilds = (d.lds_bytes() + 15) / 16 * 16;
+  AggQueue q = wave_queue<Dev>(lds + ilds);
+  __syncthreads();
+  const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (blockDim.x / kWave);
+  const uint64_t wave = static_cast<uint64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
+  AggAcc acc;
+  agg_rows(d, lds, q, b, wave * kWave, b.rows, waves * kWave, lo, hi, flag, acc);
+  agg_flush(d, lds, q, flag, acc, b.result);
+}
+
+__device__ __forceinline__ uint32_t find_batch(const uint64_t* tile_start, uint32_t nb, uint64_t t) {
+  uint32_t lo = 0, hi = nb;  // last b with tile_start[b] <= t
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (tile_start[mid] <= t) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Persistent join-aggregate: waves pull (batch, tile) items until the counter passes total_tiles.
+template <class Dev>
+__global__ __launch_bounds__(kAggBlock) void join_agg_persistent_kernel(Dev d, const FactBatchDev* __restrict__ batches,
+                                                                        const uint64_t* __restrict__ tile_start,
+                                                                        uint32_t nb, uint64_t total_tiles, int32_t lo,
+                                                                        int32_t hi, const uint8_t* __restrict__ flag,
+                                                                        unsigned long long* counter) {
+  extern __shared__ __attribute__((aligned(16))) uint8_t lds[];
+  d.stage(lds);
+  const size_t ilds = (d.lds_bytes() + 15) / 16 * 16;
+  AggQueue q = wave_queue<Dev>(lds + ilds);
+  __syncthreads();
+  AggAcc acc;
+  for (;;) {
+    unsigned long long t = 0;
+    if (lane() == 0) t = atomicAdd(counter, 1ull);
+    t = shfl64(t, 0);
+    if (t >= total_tiles) break;
+    const uint32_t bi = find_batch(tile_start, nb, t);
+    const FactBatchDev b = batches[bi];
+    const uint64_t r0 = (t - b.tile_start) * kAggTileRows;
+    const uint64_t r1 = std::min<uint64_t>(b.rows, r0 + kAggTileRows);
+    agg_rows(d, lds, q, b, r0, r1, kWave, lo, hi, flag, acc);
+    agg_flush(d, lds, q, flag, acc, b.result);
+  }
+}
+
+// Persistent lookups over many probe batches.
+template <class Dev, class K>
+__global__ __launch_bounds__(256) void lookup_persistent_kernel(Dev d, const ProbeBatchDev* __restrict__ batches,
+                                                                const uint64_t* __restrict__ tile_start, uint32_t nb,
+                                                                uint64_t total_tiles, unsigned long long* counter) {
+  extern __shared__ __attribute__((aligned(16))) uint8_t lds[];
+  d.stage(lds);
+  __syncthreads();
+  for (;;) {
+    unsigned long long t = 0;
+    if (lane() == 0) t = atomicAdd(counter, 1ull);
+    t = shfl64(t, 0);
+    if (t >= total_tiles) break;
+    const uint32_t bi = find_batch(tile_start, nb, t);
+    const ProbeBatchDev b = batches[bi];
+    const K* probes = static_cast<const K*>(b.probes);
+    const uint64_t r0 = (t - b.tile_start) * kLookupTileRows;
+    const uint64_t r1 = std::min<uint64_t>(b.m, r0 + kLookupTileRows);
+    for (uint64_t base = r0; base < r1; base += kWave) {
+      const uint64_t i = base + static_cast<uint64_t>(lane());
+      const bool active = i < r1;
+      const K x = active ? load_stream(probes + i) : K(0);
+      const uint32_t r = d.lookup(x, active, lds);
+      if (active) store_stream(b.out + i, r);
+    }
+  }
+}
+
+namespace {
+
+template <class Dev>
+void launch_join_agg(const Dev& d, const FactBatchDev& b, int32_t lo, int32_t hi, const uint8_t* flag, int cu,
+                     hipStream_t s) {
+  if (b.rows == 0) return;
+  const size_t lds = (d.lds_bytes() + 15) / 16 * 16 + agg_queue_bytes();
+  const void* fn = reinterpret_cast<const void*>(&join_agg_kernel<Dev>);
+  uint64_t blocks = static_cast<uint64_t>(blocks_per_cu(fn, kAggBlock, lds)) * cu;
+  blocks = std::min<uint64_t>(blocks, ceil_div(b.rows, kAggBlock * 4));
+  blocks = std::max<uint64_t>(blocks, 1);
+  hipLaunchKernelGGL((join_agg_kernel<Dev>), dim3(static_cast<unsigned>(blocks)), dim3(kAggBlock), lds, s, d, b, lo, hi,
+                     flag);
+  GJ_HIP_CHECK(hipGetLastError());
+}
+
+void check_agg_index(const gj_index& ix) {
+  GJ_REQUIRE(ix.key_bytes == 4, GJ_ERR_INVALID, "join-aggregate needs an index over 4-byte keys");
+  GJ_REQUIRE(ix.kind != GJ_NO_OP, GJ_ERR_INVALID, "join-aggregate needs a real index");
+}
+
+FactBatchDev to_dev(const gj_fact_batch& b, gj_agg_result* res) {
+  GJ_REQUIRE(b.rows == 0 || (b.d_fkey && b.d_date && b.d_price && b.d_discount), GJ_ERR_INVALID, "null fact column");
+  return FactBatchDev{b.d_fkey, b.d_date, b.d_price, b.d_discount, b.rows, 0, res};
+}
+
+}  // namespace
+
+}  // namespace gj
+
+using namespace gj;
+
+struct gj_batch_plan {
+  const gj_index* ix = nullptr;
+  bool aggregate = false;
+  gj_batch_mode mode = GJ_BATCH_LAUNCH;
+  // host copies
+  std::vector<FactBatchDev> fact;
+  std::vector<ProbeBatchDev> probe;
+  std::vector<uint64_t> tile_start;
+  uint64_t total_tiles = 0;
+  int32_t lo = 0, hi = 0;
+  const uint8_t* flag = nullptr;
+  gj_agg_result* results = nullptr;
+  // device copies (persistent)
+  dbuf d_batches, d_tile_start, d_counter;
+  // graph
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~gj_batch_plan() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
+namespace gj {
+namespace {
+
+// Enqueues one execution of the plan (no graph replay) on s.
+void enqueue_plan(gj_batch_plan& pl, hipStream_t s) {
+  const gj_index& ix = *pl.ix;
+  if (pl.aggregate) {
+    fill_bytes(pl.results, 0, pl.fact.size() * sizeof(gj_agg_result), s);
+    with_index_dev<uint32_t>(ix, [&](auto d) {
+      using Dev = decltype(d);
+      if (pl.mode == GJ_BATCH_PERSISTENT) {
+        fill_bytes(pl.d_counter.p, 0, 16, s);
+        if (pl.total_tiles == 0) return;
+        const size_t lds = (d.lds_bytes() + 15) / 16 * 16 + agg_queue_bytes();
+        const void* fn = reinterpret_cast<const void*>(&join_agg_persistent_kernel<Dev>);
+        uint64_t blocks = static_cast<uint64_t>(blocks_per_cu(fn, kAggBlock, lds)) * ix.cu;
+        blocks = std::min<uint64_t>(blocks, ceil_div(pl.total_tiles, kAggBlock / kWave));
+        blocks = std::max<uint64_t>(blocks, 1);
+        hipLaunchKernelGGL((join_agg_persistent_kernel<Dev>), dim3(static_cast<unsigned>(blocks)), dim3(kAggBlock),
+                           lds, s, d, pl.d_batches.as<const FactBatchDev>(), pl.d_tile_start.as<const uint64_t>(),
+                           static_cast<uint32_t>(pl.fact.size()), pl.total_tiles, pl.lo, pl.hi, pl.flag,
+                           pl.d_counter.as<unsigned long long>());
+        GJ_HIP_CHECK(hipGetLastError());
+      } else {
+        for (const auto& b : pl.fact) launch_join_agg(d, b, pl.lo, pl.hi, pl.flag, ix.cu, s);
+      }
+    });
+    return;
+  }
+  dispatch_key(ix.key_bytes, [&]<class K>() {
+    with_index_dev<K>(ix, [&](auto d) {
+      using Dev = decltype(d);
+      if (pl.mode == GJ_BATCH_PERSISTENT) {
+        fill_bytes(pl.d_counter.p, 0, 16, s);
+        if (pl.total_tiles == 0) return;
+        const size_t lds = d.lds_bytes();
+        const void* fn = reinterpret_cast<const void*>(&lookup_persistent_kernel<Dev, K>);
+        uint64_t blocks = static_cast<uint64_t>(blocks_per_cu(fn, 256, lds)) * ix.cu;
+        blocks = std::min<uint64_t>(blocks, ceil_div(pl.total_tiles, 4));
+        blocks = std::max<uint64_t>(blocks, 1);
+        hipLaunchKernelGGL((lookup_persistent_kernel<Dev, K>), dim3(static_cast<unsigned>(blocks)), dim3(256), lds, s,
+                           d, pl.d_batches.as<const ProbeBatchDev>(), pl.d_tile_start.as<const uint64_t>(),
+                           static_cast<uint32_t>(pl.probe.size()), pl.total_tiles,
+                           pl.d_counter.as<unsigned long long>());
+        GJ_HIP_CHECK(hipGetLastError());
+      } else {
+        gj_lookup_params lp;
+        gj_lookup_params_default(&lp);
+        for (const auto& b : pl.probe) launch_plain(d, static_cast<const K*>(b.probes), b.m, b.out, lp, s, ix.cu);
+      }
+    });
+  });
+}
+
+void finish_plan(gj_batch_plan& pl, hipStream_t s) {
+  if (pl.mode == GJ_BATCH_PERSISTENT) {
+    const size_t nb = pl.aggregate ? pl.fact.size() : pl.probe.size();
+    const size_t esz = pl.aggregate ? sizeof(FactBatchDev) : sizeof(ProbeBatchDev);
+    pl.d_batches.alloc(std::max<size_t>(nb * esz, 16));
+    pl.d_tile_start.alloc(std::max<size_t>(nb * 8, 16));
+    pl.d_counter.alloc(16);
+    const void* src = pl.aggregate ? static_cast<const void*>(pl.fact.data()) : static_cast<const void*>(pl.probe.data());
+    if (nb) {
+      GJ_HIP_CHECK(hipMemcpyAsync(pl.d_batches.p, src, nb * esz, hipMemcpyHostToDevice, s));
+      GJ_HIP_CHECK(hipMemcpyAsync(pl.d_tile_start.p, pl.tile_start.data(), nb * 8, hipMemcpyHostToDevice, s));
+    }
+    GJ_HIP_CHECK(hipStreamSynchronize(s));
+  } else if (pl.mode == GJ_BATCH_GRAPH) {
+    // capture needs a non-null stream
+    hipStream_t cs = s;
+    hipStream_t own = nullptr;
+    if (!cs) {
+      GJ_HIP_CHECK(hipStreamCreateWithFlags(&own, hipStreamNonBlocking));
+      cs = own;
+    }
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      try {
+        enqueue_plan(pl, cs);
+      } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(cs, &g);
+        if (g) (void)hipGraphDestroy(g);
+        if (own) (void)hipStreamDestroy(own);
+        throw;
+      }
+      e = hipStreamEndCapture(cs, &pl.graph);
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&pl.exec, pl.graph, nullptr, nullptr, 0);
+    if (own) (void)hipStreamDestroy(own);
+    GJ_HIP_CHECK(e);
+  }
+}
+
+}  // namespace
+}  // namespace gj
+
+extern "C" {
+
+int gj_join_aggregate(const gj_index* index, const gj_fact_batch* batch, const gj_agg_params* params,
+                      gj_agg_result* d_result, gj_stream_t stream) {
+  return guarded([&] {
+    GJ_REQUIRE(index && batch && params && d_result, GJ_ERR_INVALID, "null argument");
+    check_agg_index(*index);
+    GJ_REQUIRE(params->d_flag, GJ_ERR_INVALID, "null flag column");
+    const FactBatchDev b = to_dev(*batch, d_result);
+    with_index_dev<uint32_t>(*index, [&](auto d) {
+      launch_join_agg(d, b, params->date_lo, params->date_hi, params->d_flag, index->cu, S(stream));
+    });
+  });
+}
+
+int64_t gj_promo_revenue(int64_t numerator, int64_t denominator) {
+  // process_results, src/tpch/tpch_14_common.cu:124-127
+  numerator *= 1000;
+  denominator /= 1000;
+  if (denominator == 0) return 0;
+  return 100 * numerator / denominator;
+}
+
+int gj_lookup_plan_create(const gj_index* index, const gj_probe_batch* h_batches, uint32_t num_batches,
+                          gj_batch_mode mode, gj_stream_t stream, gj_batch_plan** out) {
+  return guarded([&] {
+    GJ_REQUIRE(index && out && (num_batches == 0 || h_batches), GJ_ERR_INVALID, "null argument");
+    GJ_REQUIRE(mode >= GJ_BATCH_LAUNCH && mode <= GJ_BATCH_PERSISTENT, GJ_ERR_INVALID, "unknown batch mode");
+    *out = nullptr;
+    auto pl = std::make_unique<gj_batch_plan>();
+    pl->ix = index;
+    pl->mode = mode;
+    uint64_t tiles = 0;
+    for (uint32_t i = 0; i < num_batches; ++i) {
+      const gj_probe_batch& b = h_batches[i];
+      GJ_REQUIRE(b.m == 0 || (b.d_probe_keys && b.d_rowids), GJ_ERR_INVALID, "null probe batch buffer");
+      pl->probe.push_back(ProbeBatchDev{b.d_probe_keys, b.d_rowids, b.m, tiles});
+      pl->tile_start.push_back(tiles);
+      tiles += ceil_div(b.m, kLookupTileRows);
+    }
+    pl->total_tiles = tiles;
+    finish_plan(*pl, S(stream));
+    *out = pl.release();
+  });
+}
+
+int gj_aggregate_plan_create(const gj_index* index, const gj_fact_batch* h_batches, uint32_t num_batches,
+                             const gj_agg_params* params, gj_agg_result* d_results, gj_batch_mode mode,
+                             gj_stream_t stream, gj_batch_plan** out) {
+  return guarded([&] {
+    GJ_REQUIRE(index && out && params && d_results && (num_batches == 0 || h_batches), GJ_ERR_INVALID,
+               "null argument");
+    GJ_REQUIRE(mode >= GJ_BATCH_LAUNCH && mode <= GJ_BATCH_PERSISTENT, GJ_ERR_INVALID, "unknown batch mode");
+    check_agg_index(*index);
+    GJ_REQUIRE(params->d_flag, GJ_ERR_INVALID, "null flag column");
+    *out = nullptr;
+    auto pl = std::make_unique<gj_batch_plan>();
+    pl->ix = index;
+    pl->aggregate = true;
+    pl->mode = mode;
+    pl->lo = params->date_lo;
+    pl->hi = params->date_hi;
+    pl->flag = params->d_flag;
+    pl->results = d_results;
+    uint64_t tiles = 0;
+    for (uint32_t i = 0; i < num_batches; ++i) {
+      FactBatchDev b = to_dev(h_batches[i], d_results + i);
+      b.tile_start = tiles;
+      pl->fact.push_back(b);
+      pl->tile_start.push_back(tiles);
+      tiles += ceil_div(b.rows, kAggTileRows);
+    }
+    pl->total_tiles = tiles;
+    finish_plan(*pl, S(stream));
+    *out = pl.release();
+  });
+}
+
+int gj_batch_plan_run(gj_batch_plan* plan, gj_stream_t stream) {
+  return guarded([&] {
+    GJ_REQUIRE(plan, GJ_ERR_INVALID, "null plan");
+    if (plan->mode == GJ_BATCH_GRAPH) {
+      GJ_HIP_CHECK(hipGraphLaunch(plan->exec, S(stream)));
+    } else {
+      enqueue_plan(*plan, S(stream));
+    }
+  });
+}
+
+void gj_batch_plan_destroy(gj_batch_plan* plan) { delete plan; }
+
+}  // extern "C"

@@ -1,0 +1,160 @@
+"""GPU parity for the fact-dimension join-aggregate, batched plans and the autotuner.
+
+Integer aggregates are exact: num, den and the match count equal the oracle's
+int64 sums (the Q14 shape of src/tpch/tpch_14_hj.cu:38-58), for every index
+kind over the dimension keys.  Plans run uneven batches by per-batch launches,
+a replayed hipGraph and a persistent kernel; every replay is checked.
+"""
+import numpy as np
+import pytest
+
+from gpujoin_amd import _lib
+
+pytestmark = pytest.mark.gpu
+
+AGG_KINDS = ["binary_search", "btree", "harmonia", "radix_spline", "learned_plr", "learned_mlp", "hash"]
+
+
+def _dimension(gj, rows, seed=0x41):
+    key, flag = gj.gen_dimension(rows, seed)
+    skey, sflag = gj.sort_dimension(key, flag)
+    return skey, sflag
+
+
+def _fact(gj, rows, dim_rows, seed=0x42):
+    return gj.gen_fact(rows, dim_rows, 2557, 90000, 10500000, 10, seed)
+
+
+def _agg_index(gj, kind, skey):
+    col = gj.DeviceArray.from_numpy(skey)
+    return col, gj.Index(kind, col, host_column=skey, params=gj.index_params(learned_leaf_bits=10, mlp_steps=100))
+
+
+@pytest.mark.parametrize("kind", AGG_KINDS)
+def test_join_aggregate_matches_oracle(gj, orc, gpu, kind):
+    dim_rows = 32 * 1000
+    skey, sflag = _dimension(gj, dim_rows)
+    # a quarter of the fact keys miss the dimension (keys above dim_rows)
+    fkey, date, price, disc = _fact(gj, 2_000_003, dim_rows + dim_rows // 3)
+    lo, hi = 1000, 1030
+    col, idx = _agg_index(gj, kind, skey)
+    d = [gj.DeviceArray.from_numpy(a) for a in (fkey, date, price, disc)]
+    dflag = gj.DeviceArray.from_numpy(sflag)
+    res = gj.DeviceArray(4, np.int64)
+    res.fill(0)
+    gj.join_aggregate(idx, *d, dflag, lo, hi, res)
+    gj.synchronize()
+    num, den, mt = (int(v) for v in res.numpy()[:3])
+    onum, oden, omt = orc.join_aggregate(skey, sflag, fkey, date, price, disc, lo, hi)
+    assert (num, den, mt) == (onum, oden, omt)
+    assert gj.promo_revenue(num, den) == orc.promo_revenue(onum, oden)
+
+
+def test_promo_revenue_closed_form(gj, orc):
+    # process_results (src/tpch/tpch_14_common.cu:124-127): 100 * (num*1000) / (den/1000)
+    for num, den in [(0, 0), (1, 999), (16380778, 100000000), (123456789012, 987654321098)]:
+        assert gj.promo_revenue(num, den) == orc.promo_revenue(num, den)
+    assert gj.promo_revenue(1, 2) == 0
+
+
+def test_join_aggregate_all_rows_closed_form(gj, gpu):
+    # every fact row qualifies and every dimension row is flagged: num == den == sum(price*(100-disc))
+    dim_rows = 4096
+    skey = np.arange(1, dim_rows + 1, dtype=np.uint32)
+    sflag = np.ones(dim_rows, dtype=np.uint8)
+    fkey, date, price, disc = _fact(gj, 100000, dim_rows, seed=9)
+    col, idx = _agg_index(gj, "btree", skey)
+    d = [gj.DeviceArray.from_numpy(a) for a in (fkey, date, price, disc)]
+    res = gj.DeviceArray(4, np.int64)
+    res.fill(0)
+    gj.join_aggregate(idx, *d, gj.DeviceArray.from_numpy(sflag), -1, 1 << 30, res)
+    gj.synchronize()
+    want = int(np.sum(price.astype(np.int64) * (100 - disc.astype(np.int64))))
+    num, den, mt = (int(v) for v in res.numpy()[:3])
+    assert num == den == want and mt == fkey.size
+
+
+def _log_uniform_sizes(rng, total_max, lo=1e3, hi=1e6):
+    sizes = []
+    while sum(sizes) < total_max:
+        sizes.append(int(np.exp(rng.uniform(np.log(lo), np.log(hi)))))
+    return sizes
+
+
+@pytest.mark.parametrize("mode", [_lib.BATCH_LAUNCH, _lib.BATCH_GRAPH, _lib.BATCH_PERSISTENT])
+def test_aggregate_plan_batches(gj, orc, gpu, mode):
+    dim_rows = 32 * 2000
+    skey, sflag = _dimension(gj, dim_rows)
+    rng = np.random.default_rng(mode)
+    sizes = _log_uniform_sizes(rng, 3_000_000)
+    rows = sum(sizes)
+    fkey, date, price, disc = _fact(gj, rows, dim_rows, seed=0x43)
+    col, idx = _agg_index(gj, "learned_plr", skey)
+    d = [gj.DeviceArray.from_numpy(a) for a in (fkey, date, price, disc)]
+    dflag = gj.DeviceArray.from_numpy(sflag)
+    lo, hi = 500, 530
+    batches, off = [], 0
+    for sz in sizes:
+        batches.append((d[0].offset(off), d[1].offset(off), d[2].offset(off), d[3].offset(off), sz))
+        off += sz
+    res = gj.DeviceArray(4 * len(batches), np.int64)
+    plan = gj.AggregatePlan(idx, batches, dflag, lo, hi, res, mode)
+    want, off = [], 0
+    for sz in sizes:
+        s = slice(off, off + sz)
+        want.append(orc.join_aggregate(skey, sflag, fkey[s], date[s], price[s], disc[s], lo, hi))
+        off += sz
+    want = np.array(want, dtype=np.int64)
+    for rep in range(3):
+        plan.run()
+        gj.synchronize()
+        got = res.numpy().reshape(-1, 4)[:, :3]
+        assert np.array_equal(got, want), f"replay {rep}"
+        # eager work between replays
+        eager = gj.DeviceArray(4, np.int64)
+        eager.fill(0)
+        gj.join_aggregate(idx, *d, dflag, lo, hi, eager, rows=sizes[0])
+        gj.synchronize()
+        assert np.array_equal(eager.numpy()[:3], want[0])
+
+
+@pytest.mark.parametrize("mode", [_lib.BATCH_LAUNCH, _lib.BATCH_GRAPH, _lib.BATCH_PERSISTENT])
+@pytest.mark.parametrize("kind", ["btree", "learned_plr", "harmonia"])
+def test_lookup_plan_batches(gj, orc, gpu, mode, kind):
+    keys = gj.gen_build_keys(32 * 8192, 8, _lib.KEYS_UNIFORM, 63, 0x51)
+    col = gj.DeviceArray.from_numpy(keys)
+    idx = gj.Index(kind, col, host_column=keys, params=gj.index_params(learned_leaf_bits=10))
+    rng = np.random.default_rng(7 + mode)
+    sizes = _log_uniform_sizes(rng, 2_000_000) + [0, 1]
+    m = sum(sizes)
+    probes, _ = gj.gen_probes(keys, m, _lib.PROBES_UNIFORM, 1.0, 0.1, 63, 0x52)
+    dp = gj.DeviceArray.from_numpy(probes)
+    out = gj.DeviceArray(m, np.uint32)
+    batches, off = [], 0
+    for sz in sizes:
+        batches.append((dp.offset(off), out.offset(off), sz))
+        off += sz
+    plan = gj.LookupPlan(idx, batches, mode)
+    want = orc.lookup_lower_bound(keys, probes)
+    for rep in range(3):
+        out.fill(0x33)
+        plan.run()
+        gj.synchronize()
+        assert np.array_equal(out.numpy(), want), f"replay {rep}"
+
+
+def test_autotune_returns_exact_config(gj, orc, gpu):
+    keys = gj.gen_build_keys(1 << 20, 8, _lib.KEYS_UNIFORM, 63, 0x61)
+    col = gj.DeviceArray.from_numpy(keys)
+    idx = gj.Index("btree", col, host_column=keys)
+    probes, exp = gj.gen_probes(keys, 1 << 20, _lib.PROBES_UNIFORM, 1.0, 0.0, 63, 0x62)
+    dp = gj.DeviceArray.from_numpy(probes)
+    scratch = gj.DeviceArray(probes.size, np.uint32)
+    best, rate = gj.autotune(idx, dp, scratch)
+    print(f"[autotune] approach={best.approach} block={best.block_size} bits={best.radix_bits} "
+          f"window={best.window} passes={best.passes} rate={rate / 1e9:.2f} G/s", flush=True)
+    assert rate > 0
+    out = gj.DeviceArray(probes.size, np.uint32)
+    idx.lookup(dp, out, params=best)
+    gj.synchronize()
+    assert np.array_equal(out.numpy(), exp)
