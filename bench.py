@@ -83,6 +83,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample-log2", type=int, default=24)
     ap.add_argument("--profile-index", default="", help="run only this index (for rocprofv3 runs)")
+    ap.add_argument("--approach", default="auto", help="auto (plain + autotuned), plain, bws, partitioning or hj")
+    ap.add_argument("--tune-log2", type=int, default=24, help="probe sample size of the autotuner")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -146,30 +148,56 @@ def main():
 
     # ---- index lookups ------------------------------------------------------
     names = [args.profile_index] if args.profile_index else [s for s in args.indexes.split(",") if s]
-    lp = gj.lookup_params("plain")
+    approach_names = {0: "plain", 1: "bws", 2: "partitioning", 3: "hj"}
+    tune_m = min(m, 1 << args.tune_log2)
     for name in names:
+        t_build = time.time()
         try:
             idx = gj.Index(name, d_keys, host_column=keys, stream=stream)
         except gj.GjError as e:
             log(f"{name}: not built ({e})")
             continue
-        fn = lambda: idx.lookup(d_probes, d_out, m=m, params=lp, stream=stream)
-        per, total = time_steps(fn, args.steps, args.warmup, stream, gj)
-        got = d_out.numpy(m)
-        ok = bool(np.array_equal(got, expected))
-        avg = sum(per) / len(per)
-        rate = m * args.steps / (total * 1e-3)
+        build_s = time.time() - t_build
         st = idx.stats()
-        results[name] = {
-            "lookups_per_s": rate, "ms_per_step": total / args.steps, "avg_kernel_ms": avg,
-            "gather_roofline_frac": rate / G, "hbm_frac": (m * BYTES_PER_LOOKUP / (avg * 1e-3)) / 1e9 / PEAK_HBM_GBPS,
-            "rowids_match": ok, "index_bytes": st["memory_bytes"], "depth": st["depth"],
-            "lds_levels": st["lds_levels"],
-        }
-        log(f"{name}: {rate / 1e9:.3f} G lookups/s, {avg:.3f} ms/launch, {rate / G * 100:.1f}% of G, match={ok}")
+        configs = [("plain", gj.lookup_params("plain"))]
+        if args.approach == "auto":
+            best, rate_s = gj.autotune(idx, d_probes, d_out, m=tune_m, stream=stream)
+            log(f"{name}: autotune on 2^{args.tune_log2} probes -> {approach_names[best.approach]} "
+                f"bits={best.radix_bits} window={best.window} passes={best.passes} ({rate_s / 1e9:.2f} G/s)")
+            if best.approach != 0 or best.block_size != 256:
+                configs.append(("tuned", best))
+        elif args.approach != "plain":
+            configs = [(args.approach, gj.lookup_params(args.approach))]
+        entry = {"index_bytes": st["memory_bytes"], "depth": st["depth"], "lds_levels": st["lds_levels"],
+                 "build_s": build_s}
+        for k in ("spline_points", "leaves", "fallback_leaves", "fallback_keys", "max_window", "avg_window",
+                  "train_loss"):
+            if st[k]:
+                entry[k] = st[k]
+        best_rate = 0.0
+        for label, lp in configs:
+            fn = lambda: idx.lookup(d_probes, d_out, m=m, params=lp, stream=stream)
+            per, total = time_steps(fn, args.steps, args.warmup, stream, gj)
+            got = d_out.numpy(m)
+            ok = bool(np.array_equal(got, expected))
+            avg = sum(per) / len(per)
+            rate = m * args.steps / (total * 1e-3)
+            res = {"approach": approach_names[lp.approach], "radix_bits": lp.radix_bits, "window": lp.window,
+                   "passes": lp.passes, "block": lp.block_size, "lookups_per_s": rate,
+                   "ms_per_step": total / args.steps, "avg_launch_ms": avg, "gather_roofline_frac": rate / G,
+                   "rowids_match": ok}
+            entry[label] = res
+            log(f"{name} [{label} {res['approach']}]: {rate / 1e9:.3f} G lookups/s, {avg:.3f} ms/step, "
+                f"{rate / G * 100:.1f}% of G, match={ok}")
+            if not ok:
+                log(f"{name}: ROW ID MISMATCH ({int(np.count_nonzero(got != expected))} probes)")
+            if ok and rate > best_rate:
+                best_rate = rate
+                entry.update({"lookups_per_s": rate, "ms_per_step": total / args.steps, "avg_launch_ms": avg,
+                              "best": label, "gather_roofline_frac": rate / G, "rowids_match": True})
+        entry.setdefault("rowids_match", False)
+        results[name] = entry
         idx.close()
-        if not ok:
-            log(f"{name}: ROW ID MISMATCH ({int(np.count_nonzero(got != expected))} probes)")
 
     # ---- CPU baseline (oracle port, bounded sample) ---------------------------
     cpu = None
@@ -196,7 +224,7 @@ def main():
     if head is None and results:
         head = next(iter(results))
     r = results.get(head, {})
-    avg_s = r.get("avg_kernel_ms", float("nan")) * 1e-3
+    avg_s = r.get("avg_launch_ms", float("nan")) * 1e-3
     achieved = m * BYTES_PER_LOOKUP / avg_s / 1e9 if r else None
     traffic = load_traffic(f"{head}_u64_2^{args.log2_n}_2^{args.log2_m}") if head else None
     line = {
