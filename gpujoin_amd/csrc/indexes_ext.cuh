@@ -91,34 +91,45 @@ struct HarmoniaDev {
       xs[r] = shfl(my_x, gbase + r);
       v[r] = load_lane_keys<K, KPL>(base + off[r] + gl * KPL);
     }
+    int c[G];
+    bool e[G];
+#pragma unroll
+    for (int r = 0; r < G; ++r) {
+      c[r] = 0;
+      e[r] = false;
+#pragma unroll
+      for (int j = 0; j < KPL; ++j) {
+        c[r] += group_popc<G>(ballot(v[r].k[j] < xs[r]), gbase);
+        e[r] |= group_popc<G>(ballot(v[r].k[j] == xs[r]), gbase) != 0;
+      }
+    }
+    if constexpr (LINES == 2) {
+      // every key of the first line is below the probe: read the second line; all such
+      // rounds request their second line before any is examined (one more latency, not G)
+#pragma unroll
+      for (int r = 0; r < G; ++r)
+        if (c[r] == LK) v[r] = load_lane_keys<K, KPL>(base + off[r] + LK + gl * KPL);
+#pragma unroll
+      for (int r = 0; r < G; ++r) {
+        if (c[r] != LK) continue;
+        int c2 = 0;
+        bool e2 = false;
+#pragma unroll
+        for (int j = 0; j < KPL; ++j) {
+          c2 += group_popc<G>(ballot(v[r].k[j] < xs[r]), gbase);
+          e2 |= group_popc<G>(ballot(v[r].k[j] == xs[r]), gbase) != 0;
+        }
+        c[r] += c2;
+        e[r] = e2;
+      }
+    }
     uint32_t res = 0;
     my_eq = false;
 #pragma unroll
     for (int r = 0; r < G; ++r) {
-      int c = 0;
-      bool e = false;
-#pragma unroll
-      for (int j = 0; j < KPL; ++j) {
-        c += group_popc<G>(ballot(v[r].k[j] < xs[r]), gbase);
-        e |= group_popc<G>(ballot(v[r].k[j] == xs[r]), gbase) != 0;
-      }
-      if constexpr (LINES == 2) {
-        if (c == LK) {  // every key of the first line is below the probe: read the second line
-          const LaneKeys<K, KPL> w = load_lane_keys<K, KPL>(base + off[r] + LK + gl * KPL);
-          int c2 = 0;
-          bool e2 = false;
-#pragma unroll
-          for (int j = 0; j < KPL; ++j) {
-            c2 += group_popc<G>(ballot(w.k[j] < xs[r]), gbase);
-            e2 |= group_popc<G>(ballot(w.k[j] == xs[r]), gbase) != 0;
-          }
-          c += c2;
-          e = e2;
-        }
-      }
       if (gl == r) {
-        res = static_cast<uint32_t>(c);
-        my_eq = e;
+        res = static_cast<uint32_t>(c[r]);
+        my_eq = e[r];
       }
     }
     return res;
@@ -152,104 +163,126 @@ struct HarmoniaDev {
 };
 
 // ---------------------------------------------------------------------------
-// Keys of one 128-byte line that fall inside [lo, hi): counts of those below x,
-// of those inside the window, and the in-line index of a key equal to x.
-template <class K, int G>
+// Last-mile window search shared by RadixSpline and the learned models.
+// Each lane's probe has a predicted position `pred` and a window [lo, hi) of
+// the column that holds the key if it is present.  Groups of kWinG = 4 lanes
+// read whole 128-byte lines (32 B per lane); a group serves its 4 probes in 4
+// rounds.  A round starts at the line holding pred and walks one line left or
+// right until the line brackets the probe or leaves the window.  The search
+// runs in phases: in each phase every unresolved round of every group requests
+// its next line before any line is examined, so a wave pays one memory latency
+// per phase (one, occasionally two or three) rather than one per walking round.
+// Positions are 32-bit (n < 2^32), which keeps the per-round state small and
+// the wave count per SIMD high.  Only keys inside [0, hi) are read.
+// ---------------------------------------------------------------------------
+constexpr int kWinG = 4;
+
+template <class K>
 struct LineProbe {
-  static constexpr int LK = 128 / static_cast<int>(sizeof(K));
-  static constexpr int KPL = LK / G;
-  LaneKeys<K, KPL> v;
-  __device__ __forceinline__ void load(const K* col, uint64_t line, uint64_t hi, int gl) {
-    const uint64_t start = line * LK + static_cast<uint64_t>(gl * KPL);
-#pragma unroll
-    for (int j = 0; j < KPL; ++j) v.k[j] = key_max<K>();
+  static constexpr int LK = 128 / static_cast<int>(sizeof(K));  // keys per line
+  static constexpr int KPL = LK / kWinG;                         // keys per lane (32 bytes)
+  K k[KPL];
+  __device__ __forceinline__ void load(const K* col, uint32_t line, uint32_t hi, int gl) {
+    const uint32_t start = line * LK + static_cast<uint32_t>(gl * KPL);
     if (start + KPL <= hi) {
-      v = load_lane_keys<K, KPL>(col + start);
+      const u32x4 a = *reinterpret_cast<const u32x4*>(col + start);
+      const u32x4 b = *reinterpret_cast<const u32x4*>(col + start + KPL / 2);
+      if constexpr (sizeof(K) == 8) {
+        k[0] = (static_cast<uint64_t>(a.y) << 32) | a.x;
+        k[1] = (static_cast<uint64_t>(a.w) << 32) | a.z;
+        k[2] = (static_cast<uint64_t>(b.y) << 32) | b.x;
+        k[3] = (static_cast<uint64_t>(b.w) << 32) | b.z;
+      } else {
+        k[0] = a.x; k[1] = a.y; k[2] = a.z; k[3] = a.w;
+        k[4] = b.x; k[5] = b.y; k[6] = b.z; k[7] = b.w;
+      }
     } else {
 #pragma unroll
-      for (int j = 0; j < KPL; ++j)
-        if (start + j < hi) v.k[j] = col[start + j];
+      for (int j = 0; j < KPL; ++j) k[j] = (start + j < hi) ? col[start + j] : key_max<K>();
     }
   }
-  __device__ __forceinline__ void eval(uint64_t line, uint64_t lo, uint64_t hi, K x, int gbase, int gl, int& below,
+  // counts of window keys below x and of window keys in the line; in-line index of x or -1
+  __device__ __forceinline__ void eval(uint32_t line, uint32_t lo, uint32_t hi, K x, int gbase, int gl, int& below,
                                        int& valid, int& eqpos) const {
-    const uint64_t start = line * LK + static_cast<uint64_t>(gl * KPL);
+    const uint32_t start = line * LK + static_cast<uint32_t>(gl * KPL);
     below = 0;
     valid = 0;
     eqpos = -1;
 #pragma unroll
     for (int j = 0; j < KPL; ++j) {
       const bool ok = (start + j >= lo) && (start + j < hi);
-      below += group_popc<G>(ballot(ok && v.k[j] < x), gbase);
-      valid += group_popc<G>(ballot(ok), gbase);
-      const int f = group_ffs<G>(ballot(ok && v.k[j] == x), gbase);
+      below += group_popc<kWinG>(ballot(ok && k[j] < x), gbase);
+      valid += group_popc<kWinG>(ballot(ok), gbase);
+      const int f = group_ffs<kWinG>(ballot(ok && k[j] == x), gbase);
       if (f >= 0) eqpos = f * KPL + j;
     }
   }
 };
 
-// Last-mile search shared by RadixSpline and the learned models: each lane's
-// probe has a predicted position `pred` and a window [lo, hi) of the column
-// that holds the key if it is present.  The group of G lanes reads the
-// 128-byte line that contains pred (the lines of all G probes are requested
-// before any is examined) and walks one line left or right until the line
-// brackets the probe or leaves the window - one line when the prediction lands
-// in the right line.  Only keys inside [0, hi) are ever read.
-template <class K, int G>
-__device__ __forceinline__ uint32_t group_window_search(const K* col, uint64_t my_pred, uint64_t my_lo,
-                                                        uint64_t my_hi, K my_x, bool my_live, int gbase, int gl) {
-  using LP = LineProbe<K, G>;
-  constexpr int LK = LP::LK;
+template <class K>
+__device__ __forceinline__ uint32_t group_window_search(const K* col, uint32_t my_pred, uint32_t my_lo,
+                                                        uint32_t my_hi, K my_x, bool my_live) {
+  using LP = LineProbe<K>;
+  constexpr int G = kWinG;
+  constexpr uint32_t LK = LP::LK;
+  const int ln = lane();
+  const int gbase = ln & ~(G - 1);
+  const int gl = ln & (G - 1);
   const bool ok_self = my_live && my_lo < my_hi;
   const uint64_t live_mask = ballot(ok_self);
-  uint64_t line0_self = 0;
+  uint32_t line0_self = 0;
   if (ok_self) {
-    const uint64_t p = my_pred < my_lo ? my_lo : (my_pred >= my_hi ? my_hi - 1 : my_pred);
+    const uint32_t p = my_pred < my_lo ? my_lo : (my_pred >= my_hi ? my_hi - 1 : my_pred);
     line0_self = p / LK;
   }
-  const uint64_t lo_self = ok_self ? my_lo : 0, hi_self = ok_self ? my_hi : 0;
-  LP first[G];
+  const uint32_t lo_self = ok_self ? my_lo : 0u, hi_self = ok_self ? my_hi : 0u;
+  // per-round state, replicated in every lane of the group (group-uniform)
+  K xs[G];
+  uint32_t lo[G], hi[G], line[G], found[G];
+  int dir[G];  // 0: first line, -1 walking left, +1 walking right
+  bool open[G];
 #pragma unroll
   for (int r = 0; r < G; ++r) {
-    const uint64_t line = shfl64(line0_self, gbase + r);
-    const uint64_t hi = shfl64(hi_self, gbase + r);
-    first[r].load(col, line, hi, gl);
+    open[r] = (live_mask >> (gbase + r)) & 1ull;
+    xs[r] = shfl(my_x, gbase + r);
+    lo[r] = shfl32(lo_self, gbase + r);
+    hi[r] = shfl32(hi_self, gbase + r);
+    line[r] = shfl32(line0_self, gbase + r);
+    dir[r] = 0;
+    found[r] = kNotFound;
+  }
+  for (;;) {
+    LP pc[G];
+#pragma unroll
+    for (int r = 0; r < G; ++r)
+      if (open[r]) pc[r].load(col, line[r], hi[r], gl);
+#pragma unroll
+    for (int r = 0; r < G; ++r) {
+      if (!open[r]) continue;
+      int below, valid, eqpos;
+      pc[r].eval(line[r], lo[r], hi[r], xs[r], gbase, gl, below, valid, eqpos);
+      if (eqpos >= 0) {
+        found[r] = line[r] * LK + static_cast<uint32_t>(eqpos);
+        open[r] = false;
+      } else if (below == valid && dir[r] >= 0 && line[r] < (hi[r] - 1) / LK) {
+        ++line[r];  // every window key of the line is below x: only the right can hold it
+        dir[r] = 1;
+      } else if (below == 0 && dir[r] <= 0 && line[r] > lo[r] / LK) {
+        --line[r];  // no window key of the line is below x: only the left can hold it
+        dir[r] = -1;
+      } else {
+        open[r] = false;  // x falls between two window keys (or past the window): absent
+      }
+    }
+    bool any = false;
+#pragma unroll
+    for (int r = 0; r < G; ++r) any |= open[r];
+    if (ballot(any) == 0) break;  // wave-uniform exit
   }
   uint32_t result = kNotFound;
 #pragma unroll
-  for (int r = 0; r < G; ++r) {
-    const bool owner_live = (live_mask >> (gbase + r)) & 1ull;  // group-uniform
-    const K x = shfl(my_x, gbase + r);
-    const uint64_t lo = shfl64(lo_self, gbase + r);
-    const uint64_t hi = shfl64(hi_self, gbase + r);
-    uint64_t line = shfl64(line0_self, gbase + r);
-    uint32_t found = kNotFound;
-    if (owner_live) {
-      const uint64_t lo_line = lo / LK, hi_line = (hi - 1) / LK;
-      int below, valid, eqpos;
-      first[r].eval(line, lo, hi, x, gbase, gl, below, valid, eqpos);
-      int dir = 0;  // -1 walking left, +1 walking right
-      for (;;) {
-        if (eqpos >= 0) {
-          found = static_cast<uint32_t>(line * LK + static_cast<uint64_t>(eqpos));
-          break;
-        }
-        if (below == valid && dir >= 0 && line < hi_line) {
-          ++line;  // every window key of the line is below x: only the right can hold it
-          dir = 1;
-        } else if (below == 0 && dir <= 0 && line > lo_line) {
-          --line;  // no window key of the line is below x: only the left can hold it
-          dir = -1;
-        } else {
-          break;
-        }
-        LP next;
-        next.load(col, line, hi, gl);
-        next.eval(line, lo, hi, x, gbase, gl, below, valid, eqpos);
-      }
-    }
-    if (gl == r) result = found;
-  }
+  for (int r = 0; r < G; ++r)
+    if (gl == r) result = found[r];
   return result;
 }
 
@@ -303,16 +336,14 @@ struct RadixSplineDev {
   }
 
   __device__ __forceinline__ uint32_t lookup(K x, bool active, const uint8_t*) const {
-    const int ln = lane();
-    const int gbase = ln & ~(G - 1);
-    const int gl = ln & (G - 1);
     uint64_t lo = 0, hi = 0, pred = 0;
     if (active) {
       const double est = estimate(x);
       window(est, lo, hi);
       pred = static_cast<uint64_t>(est);
     }
-    return group_window_search<K, G>(col, pred, lo, hi, x, active, gbase, gl);
+    return group_window_search<K>(col, static_cast<uint32_t>(pred), static_cast<uint32_t>(lo),
+                                  static_cast<uint32_t>(hi), x, active);
   }
 };
 
@@ -413,9 +444,6 @@ struct LearnedDev {
   }
 
   __device__ __forceinline__ uint32_t lookup(K x, bool active, const uint8_t* lds) const {
-    const int ln = lane();
-    const int gbase = ln & ~7;
-    const int gl = ln & 7;
     bool use_tree = false;
     uint64_t lo = 0, hi = 0, pred = 0;
     bool live = active && x >= min_key && x <= max_key;
@@ -436,7 +464,8 @@ struct LearnedDev {
         }
       }
     }
-    uint32_t r = group_window_search<K, 8>(col, pred, lo, hi, x, live && !use_tree, gbase, gl);
+    uint32_t r = group_window_search<K>(col, static_cast<uint32_t>(pred), static_cast<uint32_t>(lo),
+                                        static_cast<uint32_t>(hi), x, live && !use_tree);
     if (fallback_any && ballot(use_tree) != 0) {  // wave-uniform
       const uint32_t t = btree.lookup(x, use_tree, lds);
       if (use_tree) r = t;
