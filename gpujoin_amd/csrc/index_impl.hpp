@@ -108,9 +108,9 @@ struct gj_index {
     return d;
   }
 
-  template <class K, int G>
-  gj::LearnedDev<K, G> learned_dev() const {
-    gj::LearnedDev<K, G> d{};
+  template <class K, bool FB>
+  gj::LearnedDev<K, FB> learned_dev() const {
+    gj::LearnedDev<K, FB> d{};
     d.col = static_cast<const K*>(d_col);
     d.n = n;
     d.min_key = static_cast<K>(min_key);
@@ -119,7 +119,6 @@ struct gj_index {
     d.root.mlp = lm_mlp.as<const float>();
     d.leaves = lm_leaves.as<const gj::LeafModel>();
     d.num_leaves = lm_num_leaves;
-    d.fallback_any = lm_fallback_any;
     d.btree = btree_dev<K>();
     return d;
   }

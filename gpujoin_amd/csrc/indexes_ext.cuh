@@ -172,8 +172,14 @@ struct HarmoniaDev {
 // runs in phases: in each phase every unresolved round of every group requests
 // its next line before any line is examined, so a wave pays one memory latency
 // per phase (one, occasionally two or three) rather than one per walking round.
-// Positions are 32-bit (n < 2^32), which keeps the per-round state small and
-// the wave count per SIMD high.  Only keys inside [0, hi) are read.
+//
+// A round's line is sorted, so the group only needs two facts about it for the
+// owning lane: how many of its keys are below the probe (`below`, which is the
+// in-line offset of the probe's lower bound) and whether the key at that offset
+// equals the probe.  The owning lane alone holds its probe's state (line,
+// window, direction), which keeps the registers per lane, and hence the waves
+// per SIMD, where a latency-bound search needs them.  Positions are 32-bit
+// (n < 2^32); keys at or past n read as the all-ones fill, which sorts last.
 // ---------------------------------------------------------------------------
 constexpr int kWinG = 4;
 
@@ -182,9 +188,9 @@ struct LineProbe {
   static constexpr int LK = 128 / static_cast<int>(sizeof(K));  // keys per line
   static constexpr int KPL = LK / kWinG;                         // keys per lane (32 bytes)
   K k[KPL];
-  __device__ __forceinline__ void load(const K* col, uint32_t line, uint32_t hi, int gl) {
-    const uint32_t start = line * LK + static_cast<uint32_t>(gl * KPL);
-    if (start + KPL <= hi) {
+  __device__ __forceinline__ void load(const K* col, uint64_t n, uint32_t line, int gl) {
+    const uint64_t start = static_cast<uint64_t>(line) * LK + static_cast<uint64_t>(gl * KPL);
+    if (start + KPL <= n) {
       const u32x4 a = *reinterpret_cast<const u32x4*>(col + start);
       const u32x4 b = *reinterpret_cast<const u32x4*>(col + start + KPL / 2);
       if constexpr (sizeof(K) == 8) {
@@ -198,29 +204,24 @@ struct LineProbe {
       }
     } else {
 #pragma unroll
-      for (int j = 0; j < KPL; ++j) k[j] = (start + j < hi) ? col[start + j] : key_max<K>();
+      for (int j = 0; j < KPL; ++j) k[j] = (start + j < n) ? col[start + j] : key_max<K>();
     }
   }
-  // counts of window keys below x and of window keys in the line; in-line index of x or -1
-  __device__ __forceinline__ void eval(uint32_t line, uint32_t lo, uint32_t hi, K x, int gbase, int gl, int& below,
-                                       int& valid, int& eqpos) const {
-    const uint32_t start = line * LK + static_cast<uint32_t>(gl * KPL);
+  // keys of the group's line below x, and whether the line holds x
+  __device__ __forceinline__ void eval(K x, int gbase, uint32_t& below, bool& eq) const {
     below = 0;
-    valid = 0;
-    eqpos = -1;
+    bool e = false;
 #pragma unroll
     for (int j = 0; j < KPL; ++j) {
-      const bool ok = (start + j >= lo) && (start + j < hi);
-      below += group_popc<kWinG>(ballot(ok && k[j] < x), gbase);
-      valid += group_popc<kWinG>(ballot(ok), gbase);
-      const int f = group_ffs<kWinG>(ballot(ok && k[j] == x), gbase);
-      if (f >= 0) eqpos = f * KPL + j;
+      below += static_cast<uint32_t>(group_popc<kWinG>(ballot(k[j] < x), gbase));
+      e |= group_bits<kWinG>(ballot(k[j] == x), gbase) != 0;
     }
+    eq = e;
   }
 };
 
 template <class K>
-__device__ __forceinline__ uint32_t group_window_search(const K* col, uint32_t my_pred, uint32_t my_lo,
+__device__ __forceinline__ uint32_t group_window_search(const K* col, uint64_t n, uint32_t my_pred, uint32_t my_lo,
                                                         uint32_t my_hi, K my_x, bool my_live) {
   using LP = LineProbe<K>;
   constexpr int G = kWinG;
@@ -228,62 +229,55 @@ __device__ __forceinline__ uint32_t group_window_search(const K* col, uint32_t m
   const int ln = lane();
   const int gbase = ln & ~(G - 1);
   const int gl = ln & (G - 1);
-  const bool ok_self = my_live && my_lo < my_hi;
-  const uint64_t live_mask = ballot(ok_self);
-  uint32_t line0_self = 0;
-  if (ok_self) {
+  bool open = my_live && my_lo < my_hi;
+  uint32_t line = 0, found = kNotFound;
+  if (open) {
     const uint32_t p = my_pred < my_lo ? my_lo : (my_pred >= my_hi ? my_hi - 1 : my_pred);
-    line0_self = p / LK;
+    line = p / LK;
   }
-  const uint32_t lo_self = ok_self ? my_lo : 0u, hi_self = ok_self ? my_hi : 0u;
-  // per-round state, replicated in every lane of the group (group-uniform)
-  K xs[G];
-  uint32_t lo[G], hi[G], line[G], found[G];
-  int dir[G];  // 0: first line, -1 walking left, +1 walking right
-  bool open[G];
-#pragma unroll
-  for (int r = 0; r < G; ++r) {
-    open[r] = (live_mask >> (gbase + r)) & 1ull;
-    xs[r] = shfl(my_x, gbase + r);
-    lo[r] = shfl32(lo_self, gbase + r);
-    hi[r] = shfl32(hi_self, gbase + r);
-    line[r] = shfl32(line0_self, gbase + r);
-    dir[r] = 0;
-    found[r] = kNotFound;
-  }
+  const uint32_t first_line = my_lo / LK, last_line = open ? (my_hi - 1) / LK : 0u;
+  int dir = 0;  // 0: first line, -1 walking left, +1 walking right
   for (;;) {
+    const uint64_t open_mask = ballot(open);
+    if (open_mask == 0) break;  // wave-uniform exit
     LP pc[G];
 #pragma unroll
-    for (int r = 0; r < G; ++r)
-      if (open[r]) pc[r].load(col, line[r], hi[r], gl);
+    for (int r = 0; r < G; ++r) {
+      const uint32_t l = shfl32(line, gbase + r);
+      if ((open_mask >> (gbase + r)) & 1ull) pc[r].load(col, n, l, gl);
+    }
+    uint32_t below = 0;
+    bool eq = false;
 #pragma unroll
     for (int r = 0; r < G; ++r) {
-      if (!open[r]) continue;
-      int below, valid, eqpos;
-      pc[r].eval(line[r], lo[r], hi[r], xs[r], gbase, gl, below, valid, eqpos);
-      if (eqpos >= 0) {
-        found[r] = line[r] * LK + static_cast<uint32_t>(eqpos);
-        open[r] = false;
-      } else if (below == valid && dir[r] >= 0 && line[r] < (hi[r] - 1) / LK) {
-        ++line[r];  // every window key of the line is below x: only the right can hold it
-        dir[r] = 1;
-      } else if (below == 0 && dir[r] <= 0 && line[r] > lo[r] / LK) {
-        --line[r];  // no window key of the line is below x: only the left can hold it
-        dir[r] = -1;
-      } else {
-        open[r] = false;  // x falls between two window keys (or past the window): absent
+      if (!((open_mask >> (gbase + r)) & 1ull)) continue;
+      uint32_t b;
+      bool e;
+      pc[r].eval(shfl(my_x, gbase + r), gbase, b, e);
+      if (gl == r) {
+        below = b;
+        eq = e;
       }
     }
-    bool any = false;
-#pragma unroll
-    for (int r = 0; r < G; ++r) any |= open[r];
-    if (ballot(any) == 0) break;  // wave-uniform exit
+    if (open) {
+      const uint32_t base = line * LK;
+      const uint32_t a = my_lo > base ? my_lo - base : 0u;               // window slots of this line: [a, c)
+      const uint32_t c = my_hi - base < LK ? my_hi - base : LK;
+      if (eq && static_cast<uint64_t>(base) + below < n) {
+        found = base + below;  // the line holds x at its lower-bound offset
+        open = false;
+      } else if (below >= c && dir >= 0 && line < last_line) {
+        ++line;  // every window key of the line is below x: only the right can hold it
+        dir = 1;
+      } else if (below <= a && dir <= 0 && line > first_line) {
+        --line;  // no window key of the line is below x: only the left can hold it
+        dir = -1;
+      } else {
+        open = false;  // x falls between two window keys (or past the window): absent
+      }
+    }
   }
-  uint32_t result = kNotFound;
-#pragma unroll
-  for (int r = 0; r < G; ++r)
-    if (gl == r) result = found[r];
-  return result;
+  return found;
 }
 
 // ---------------------------------------------------------------------------
@@ -342,7 +336,7 @@ struct RadixSplineDev {
       window(est, lo, hi);
       pred = static_cast<uint64_t>(est);
     }
-    return group_window_search<K>(col, static_cast<uint32_t>(pred), static_cast<uint32_t>(lo),
+    return group_window_search<K>(col, n, static_cast<uint32_t>(pred), static_cast<uint32_t>(lo),
                                   static_cast<uint32_t>(hi), x, active);
   }
 };
@@ -411,7 +405,9 @@ __device__ __forceinline__ int64_t leaf_pred(const LeafModel& m, float dkey) {
   return static_cast<int64_t>(m.start) + static_cast<int64_t>(floorf(p));
 }
 
-template <class K, int G>
+// FB: some leaf is routed to the B-tree.  The common case (no such leaf)
+// compiles without the B-tree code, so the kernel keeps fewer registers.
+template <class K, bool FB>
 struct LearnedDev {
   const K* col;
   uint64_t n;
@@ -419,24 +415,24 @@ struct LearnedDev {
   RootModel root;
   const LeafModel* leaves;
   uint32_t num_leaves;
-  uint32_t fallback_any;  // any leaf routed to the B-tree
-  BTreeDev<K> btree;      // fallback
+  BTreeDev<K> btree;  // fallback (used when FB)
 
+  __host__ __device__ size_t mlp_offset() const { return FB ? btree.lds_bytes() : 0; }
   __host__ __device__ size_t lds_bytes() const {
-    size_t b = btree.lds_bytes();
+    size_t b = mlp_offset();
     if (root.hidden) b += ((3 * root.hidden + 1) * 4 + 15) / 16 * 16;
     return b;
   }
   __device__ void stage(uint8_t* lds) const {
-    btree.stage(lds);
-    if (root.hidden) stage_to_lds(lds + btree.lds_bytes(), root.mlp, ((3 * root.hidden + 1) * 4 + 15) / 16 * 16);
+    if constexpr (FB) btree.stage(lds);
+    if (root.hidden) stage_to_lds(lds + mlp_offset(), root.mlp, ((3 * root.hidden + 1) * 4 + 15) / 16 * 16);
   }
 
   __device__ __forceinline__ uint32_t route(K x, const uint8_t* lds) const {
     const float u = key_unit(x, min_key, root.inv_range);
     float cdf;
     if (root.hidden) {
-      cdf = mlp_eval(reinterpret_cast<const float*>(lds + btree.lds_bytes()), root.hidden, u);
+      cdf = mlp_eval(reinterpret_cast<const float*>(lds + mlp_offset()), root.hidden, u);
     } else {
       cdf = __builtin_fmaf(root.a, u, root.b);
     }
@@ -449,7 +445,7 @@ struct LearnedDev {
     bool live = active && x >= min_key && x <= max_key;
     if (live) {
       const LeafModel m = leaves[route(x, lds)];
-      if (m.err_hi < m.err_lo) {  // fallback leaf
+      if (m.err_hi < m.err_lo) {  // fallback leaf (only built when FB)
         use_tree = true;
       } else if (m.count) {
         const int64_t p = leaf_pred(m, leaf_dkey(x, m.base_key));
@@ -464,11 +460,13 @@ struct LearnedDev {
         }
       }
     }
-    uint32_t r = group_window_search<K>(col, static_cast<uint32_t>(pred), static_cast<uint32_t>(lo),
+    uint32_t r = group_window_search<K>(col, n, static_cast<uint32_t>(pred), static_cast<uint32_t>(lo),
                                         static_cast<uint32_t>(hi), x, live && !use_tree);
-    if (fallback_any && ballot(use_tree) != 0) {  // wave-uniform
-      const uint32_t t = btree.lookup(x, use_tree, lds);
-      if (use_tree) r = t;
+    if constexpr (FB) {
+      if (ballot(use_tree) != 0) {  // wave-uniform
+        const uint32_t t = btree.lookup(x, use_tree, lds);
+        if (use_tree) r = t;
+      }
     }
     return r;
   }

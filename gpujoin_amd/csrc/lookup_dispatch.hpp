@@ -94,7 +94,12 @@ void with_index_dev(const gj_index& ix, F&& f) {
       throw error(GJ_ERR_INVALID, "unsupported harmonia_group_lanes");
     case GJ_RADIX_SPLINE: f(ix.rs_dev<K, 8>()); return;
     case GJ_LEARNED_PLR:
-    case GJ_LEARNED_MLP: f(ix.learned_dev<K, 8>()); return;
+    case GJ_LEARNED_MLP:
+      if (ix.lm_fallback_any)
+        f(ix.learned_dev<K, true>());
+      else
+        f(ix.learned_dev<K, false>());
+      return;
     case GJ_HASH: f(ix.hash_dev<K>()); return;
   }
   throw error(GJ_ERR_INVALID, "unknown index kind");
