@@ -57,6 +57,7 @@ struct gj_index {
   gj::dbuf lm_leaves, lm_mlp;
   gj::RootModel lm_root{};
   uint32_t lm_num_leaves = 0, lm_fallback_any = 0;
+  int32_t lm_err_lo = 0, lm_err_hi = 0;  // learned window bounds over all leaves
 
   // hash table
   gj::dbuf ht;
@@ -119,6 +120,8 @@ struct gj_index {
     d.root.mlp = lm_mlp.as<const float>();
     d.leaves = lm_leaves.as<const gj::LeafModel>();
     d.num_leaves = lm_num_leaves;
+    d.err_lo = lm_err_lo;
+    d.err_hi = lm_err_hi;
     d.btree = btree_dev<K>();
     return d;
   }

@@ -10,10 +10,11 @@
 //            w1 and w2 are projected to >= 0 after every step, so the MLP is monotone and
 //            every leaf owns a contiguous key range
 //   2. routing: leaf = floor(root(x) * L); leaf start positions by a boundary kernel
-//   3. leaves: per-leaf linear least squares of local position on FP32 key offsets
+//   3. leaves: per-leaf linear least squares of local position on FP32 key offsets,
+//      packed into the 16-byte lookup model (intercept rounded into the start)
 //   4. calibration: the error of every build key under the exact FP32 lookup code
-//      gives each leaf its window [pred + err_lo, pred + err_hi]; leaves wider than the
-//      bound are routed to the B-tree
+//      gives each leaf its error range; leaves wider than the bound are routed to the
+//      B-tree, and the extremes over the other leaves form the lookup window
 #include <algorithm>
 #include <climits>
 #include <cmath>
@@ -242,11 +243,17 @@ __global__ void leaf_bounds_kernel(const K* __restrict__ col, uint64_t n, Router
   }
 }
 
+// Build-time facts of one leaf (not read by lookups).
+struct LeafFit {
+  uint32_t start, count;
+  int32_t err_lo, err_hi;  // extremes of position - pred over the leaf's keys
+};
+
 // One block per leaf: least squares of local position on FP32 key offsets.
 template <class K>
 __global__ __launch_bounds__(256) void leaf_fit_kernel(const K* __restrict__ col, uint64_t n,
                                                        const uint32_t* __restrict__ start, uint32_t leaves,
-                                                       LeafModel* __restrict__ out) {
+                                                       LeafModel* __restrict__ out, LeafFit* __restrict__ fit) {
   const uint32_t l = blockIdx.x;
   if (l >= leaves) return;
   const uint64_t s = start[l];
@@ -289,21 +296,21 @@ __global__ __launch_bounds__(256) void leaf_fit_kernel(const K* __restrict__ col
       if (var > 0) slope = (sdj - sd * sj / c) / var;
       icpt = (sj - slope * sd) / c;
     }
+    double st = static_cast<double>(s) + std::floor(icpt + 0.5);
+    st = st < 0 ? 0 : (st > static_cast<double>(n) ? static_cast<double>(n) : st);
     LeafModel m;
     m.base_key = base;
-    m.start = static_cast<uint32_t>(s);
-    m.count = static_cast<uint32_t>(cnt);
+    m.start = static_cast<uint32_t>(st);
     m.slope = static_cast<float>(slope);
-    m.icpt = static_cast<float>(icpt);
-    m.err_lo = INT_MAX;
-    m.err_hi = INT_MIN;
     out[l] = m;
+    fit[l] = LeafFit{static_cast<uint32_t>(s), static_cast<uint32_t>(cnt), INT_MAX, INT_MIN};
   }
 }
 
 // err_lo/err_hi of the leaf each key is routed to, under the exact lookup arithmetic.
 template <class K>
-__global__ void leaf_error_kernel(const K* __restrict__ col, uint64_t n, Router<K> route, LeafModel* __restrict__ lm) {
+__global__ void leaf_error_kernel(const K* __restrict__ col, uint64_t n, Router<K> route,
+                                  const LeafModel* __restrict__ lm, LeafFit* __restrict__ fit) {
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
     const K key = col[i];
@@ -312,24 +319,28 @@ __global__ void leaf_error_kernel(const K* __restrict__ col, uint64_t n, Router<
     const int64_t pred = leaf_pred(m, leaf_dkey(key, m.base_key));
     int64_t e = static_cast<int64_t>(i) - pred;
     e = e < INT_MIN / 2 ? INT_MIN / 2 : (e > INT_MAX / 2 ? INT_MAX / 2 : e);
-    atomicMin(&lm[l].err_lo, static_cast<int>(e));
-    atomicMax(&lm[l].err_hi, static_cast<int>(e));
+    atomicMin(&fit[l].err_lo, static_cast<int>(e));
+    atomicMax(&fit[l].err_hi, static_cast<int>(e));
   }
 }
 
-// Empty leaves get an empty window; leaves wider than max_width go to the B-tree (err_hi < err_lo).
-__global__ void leaf_finalize_kernel(LeafModel* __restrict__ lm, uint32_t leaves, int64_t max_width) {
+// Empty leaves predict their start (no key is routed to them); leaves wider than
+// max_width go to the B-tree (slope = NaN, error range emptied: err_hi < err_lo).
+__global__ void leaf_finalize_kernel(LeafModel* __restrict__ lm, LeafFit* __restrict__ fit, uint32_t leaves,
+                                     int64_t max_width) {
   const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= leaves) return;
-  LeafModel m = lm[l];
-  if (m.count == 0) {
-    m.err_lo = 0;
-    m.err_hi = 0;
-  } else if (static_cast<int64_t>(m.err_hi) - m.err_lo + 1 > max_width) {
-    m.err_lo = 1;
-    m.err_hi = 0;
+  LeafFit f = fit[l];
+  if (f.count == 0) {
+    f.err_lo = 0;
+    f.err_hi = 0;
+    lm[l].slope = 0.0f;
+  } else if (static_cast<int64_t>(f.err_hi) - f.err_lo + 1 > max_width) {
+    f.err_lo = 1;
+    f.err_hi = 0;
+    lm[l].slope = __builtin_nanf("");
   }
-  lm[l] = m;
+  fit[l] = f;
 }
 
 }  // namespace
@@ -436,35 +447,41 @@ void build_learned(gj_index& ix, hipStream_t s) {
     GJ_REQUIRE(hinv == 0, GJ_ERR_INTERNAL, "learned index: root model is not monotone");
   }
   ix.lm_leaves.alloc(sizeof(LeafModel) * L);
+  dbuf fit(sizeof(LeafFit) * L);
   hipLaunchKernelGGL(leaf_fit_kernel<K>, dim3(L), dim3(256), 0, s, col, n, start.as<const uint32_t>(), L,
-                     ix.lm_leaves.as<LeafModel>());
+                     ix.lm_leaves.as<LeafModel>(), fit.as<LeafFit>());
   GJ_HIP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(leaf_error_kernel<K>, dim3(grid_for(n, 256)), dim3(256), 0, s, col, n, route,
-                     ix.lm_leaves.as<LeafModel>());
+                     ix.lm_leaves.as<const LeafModel>(), fit.as<LeafFit>());
   GJ_HIP_CHECK(hipGetLastError());
   const int64_t max_width = 2 * static_cast<int64_t>(ix.params.learned_max_error) + 2;
-  hipLaunchKernelGGL(leaf_finalize_kernel, dim3(grid_for(L, 256)), dim3(256), 0, s, ix.lm_leaves.as<LeafModel>(), L,
-                     max_width);
+  hipLaunchKernelGGL(leaf_finalize_kernel, dim3(grid_for(L, 256)), dim3(256), 0, s, ix.lm_leaves.as<LeafModel>(),
+                     fit.as<LeafFit>(), L, max_width);
   GJ_HIP_CHECK(hipGetLastError());
 
-  // statistics
-  std::vector<LeafModel> hl(L);
-  GJ_HIP_CHECK(hipMemcpyAsync(hl.data(), ix.lm_leaves.p, ix.lm_leaves.bytes, hipMemcpyDeviceToHost, s));
+  // window bounds over the leaves that are not routed to the B-tree, and statistics
+  std::vector<LeafFit> hf(L);
+  GJ_HIP_CHECK(hipMemcpyAsync(hf.data(), fit.p, fit.bytes, hipMemcpyDeviceToHost, s));
   GJ_HIP_CHECK(hipStreamSynchronize(s));
   uint64_t fb_leaves = 0, fb_keys = 0, wsum = 0, wkeys = 0;
   uint32_t wmax = 0;
-  for (const auto& m : hl) {
-    if (m.count == 0) continue;
-    if (m.err_hi < m.err_lo) {
+  int32_t elo = 0, ehi = 0;
+  for (const auto& f : hf) {
+    if (f.count == 0) continue;
+    if (f.err_hi < f.err_lo) {
       ++fb_leaves;
-      fb_keys += m.count;
+      fb_keys += f.count;
     } else {
-      const uint32_t w = static_cast<uint32_t>(m.err_hi - m.err_lo + 1);
+      const uint32_t w = static_cast<uint32_t>(f.err_hi - f.err_lo + 1);
       wmax = std::max(wmax, w);
-      wsum += static_cast<uint64_t>(w) * m.count;
-      wkeys += m.count;
+      wsum += static_cast<uint64_t>(w) * f.count;
+      wkeys += f.count;
+      elo = std::min(elo, f.err_lo);
+      ehi = std::max(ehi, f.err_hi);
     }
   }
+  ix.lm_err_lo = elo;
+  ix.lm_err_hi = ehi;
   ix.lm_root = root;
   ix.lm_num_leaves = L;
   ix.lm_fallback_any = fb_leaves ? 1u : 0u;
