@@ -85,6 +85,8 @@ def main():
     ap.add_argument("--profile-index", default="", help="run only this index (for rocprofv3 runs)")
     ap.add_argument("--approach", default="auto", help="auto (plain + autotuned), plain, bws, partitioning or hj")
     ap.add_argument("--tune-log2", type=int, default=24, help="probe sample size of the autotuner")
+    ap.add_argument("--suite", action="store_true", help="measure the other workload configurations instead")
+    ap.add_argument("--quick", action="store_true", help="--suite at reduced sizes")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -145,6 +147,13 @@ def main():
     chase_ns = statistics.median(per) * 1e6 / steps_chase
     del d_next, nxt, perm
     log(f"pointer chase: {chase_ns:.0f} ns per dependent access")
+
+    if args.suite:
+        # the other workload configurations (one JSON object per line; not the headline run)
+        from gpujoin_amd import suite
+        del d_probes, d_out, d_keys
+        suite.run(gj, G, quick=args.quick, cpu=not args.no_cpu)
+        return
 
     # ---- index lookups ------------------------------------------------------
     names = [args.profile_index] if args.profile_index else [s for s in args.indexes.split(",") if s]

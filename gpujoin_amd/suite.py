@@ -1,0 +1,203 @@
+"""Measurement suite over the workload configurations beyond the headline.
+
+Run with `python bench.py --suite [--quick]`.  Every measurement checks its row ids
+or sums against the generator's ground truth (or the oracle for the aggregates)
+and records lookups/s (rows/s for the aggregate) next to the CPU baseline.
+Writes one JSON object per measurement to stdout-compatible lines via `emit`.
+
+  u32 probe      2^20 unique u32 keys from [0, 2^32), 2^24 uniform probes: row ids + checksum
+  Zipf probe     2^26 u64 keys, 2^28 probes, Zipf theta = 1.0 (rank 0 never drawn, as in the reference)
+  partitioned    fan-out 2^8 / 2^10 / 2^12, 1 and 2 passes, windows 2^16..2^22;
+                 index-size sweep 2^20..2^30 keys (B-tree and learned)
+  learned        RadixSpline err 8/32/64 and the PLR / MLP models over uniform,
+                 lognormal and clustered keys, 10 % misses; B-tree fallbacks counted
+  join-aggregate 2e6-row dimension, 6e7 fact rows, 30-day window,
+                 log-uniform batches of 1e3..1e7 rows, per-batch launches / hipGraph / persistent kernel
+"""
+from __future__ import annotations
+
+import json
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+from . import _lib
+
+
+def _time(gj, fn, stream, steps=5, warmup=2):
+    for _ in range(warmup):
+        fn()
+    stream.synchronize()
+    e0, e1 = gj.Event(), gj.Event()
+    e0.record(stream)
+    for _ in range(steps):
+        fn()
+    e1.record(stream)
+    stream.synchronize()
+    return e0.elapsed_ms(e1) / steps
+
+
+def run(gj, G, quick=False, cpu=True, emit=None):
+    import oracle
+
+    emit = emit or (lambda d: print(json.dumps(d), flush=True))
+    stream = gj.Stream()
+    cores = os.cpu_count() or 1
+    log = lambda *a: print("[suite]", *a, file=sys.stderr, flush=True)
+
+    # ---- u32 probe -------------------------------------------------------------------
+    keys = gj.gen_build_keys(1 << 20, 4, _lib.KEYS_UNIFORM, 0, 0x1)
+    probes, exp = gj.gen_probes(keys, 1 << 24, _lib.PROBES_UNIFORM, 1.0, 0.0, 0, 0x2)
+    dk, dp = gj.DeviceArray.from_numpy(keys), gj.DeviceArray.from_numpy(probes)
+    out = gj.DeviceArray(probes.size, np.uint32)
+    for kind in ("binary_search", "btree", "learned_plr"):
+        idx = gj.Index(kind, dk, host_column=keys, stream=stream)
+        ms = _time(gj, lambda: idx.lookup(dp, out, params=gj.lookup_params(), stream=stream), stream)
+        got = out.numpy()
+        rec = {"config": "u32 probe: 2^20 u32 keys, 2^24 uniform probes", "index": kind, "lookups_per_s": probes.size / ms * 1e3,
+               "ms": ms, "rowids_match": bool(np.array_equal(got, exp)),
+               "checksum": int(got.astype(np.uint64).sum()), "checksum_expected": int(exp.astype(np.uint64).sum())}
+        if cpu and kind == "binary_search":
+            t = time.perf_counter()
+            ref = oracle.lookup_lower_bound(keys, probes, threads=cores)
+            rec["cpu_lookups_per_s"] = probes.size / (time.perf_counter() - t)
+            rec["cpu_cores"] = cores
+            rec["cpu_checksum"] = oracle.checksum(ref)
+        emit(rec)
+        idx.close()
+    del dk, dp, out
+
+    n_log = 24 if quick else 26
+    m_log = 26 if quick else 28
+    n, m = 1 << n_log, 1 << m_log
+    keys = gj.gen_build_keys(n, 8, _lib.KEYS_UNIFORM, 63, 0x11)
+    dk = gj.DeviceArray.from_numpy(keys, stream)
+    out = gj.DeviceArray(m, np.uint32)
+
+    # ---- Zipf probe --------------------------------------------------------------
+    probes, exp = gj.gen_probes(keys, m, _lib.PROBES_ZIPF, 1.0, 0.0, 63, 0x13)
+    dp = gj.DeviceArray.from_numpy(probes, stream)
+    for kind in ("binary_search", "btree", "harmonia", "radix_spline", "learned_plr"):
+        idx = gj.Index(kind, dk, host_column=keys, stream=stream)
+        for appr in ("plain", "partitioning"):
+            ms = _time(gj, lambda: idx.lookup(dp, out, params=gj.lookup_params(appr), stream=stream), stream, steps=3)
+            emit({"config": f"Zipf 1.0 probe: 2^{n_log} u64 keys, 2^{m_log} probes", "index": kind, "approach": appr,
+                  "lookups_per_s": m / ms * 1e3, "gather_roofline_frac": m / ms * 1e3 / G, "ms": ms,
+                  "rowids_match": bool(np.array_equal(out.numpy(), exp))})
+        idx.close()
+    del probes, exp
+
+    # ---- partitioned sweeps -------------------------------------------------
+    probes, exp = gj.gen_probes(keys, m, _lib.PROBES_UNIFORM, 1.0, 0.0, 63, 0x21)
+    dp.upload(probes)
+    for kind in ("btree", "learned_plr"):
+        idx = gj.Index(kind, dk, host_column=keys, stream=stream)
+        for bits, passes in ((8, 1), (10, 1), (12, 1), (12, 2)):
+            for wlog in ((16, 18, 20, 22) if not quick else (18, 22)):
+                lp = gj.lookup_params("partitioning", radix_bits=bits, passes=passes, window=1 << wlog)
+                ms = _time(gj, lambda: idx.lookup(dp, out, params=lp, stream=stream), stream, steps=3, warmup=1)
+                emit({"config": f"partitioned probe: 2^{n_log} u64 keys, 2^{m_log} uniform probes", "index": kind,
+                      "fanout": 1 << bits, "passes": passes, "window": 1 << wlog, "lookups_per_s": m / ms * 1e3,
+                      "gather_roofline_frac": m / ms * 1e3 / G, "ms": ms,
+                      "rowids_match": bool(np.array_equal(out.numpy(), exp))})
+        idx.close()
+    del dk
+    sizes = (20, 22, 24, 26, 28, 30) if not quick else (20, 24)
+    for s_log in sizes:
+        k2 = gj.gen_build_keys(1 << s_log, 8, _lib.KEYS_UNIFORM, 63, 0x22 + s_log)
+        p2, e2 = gj.gen_probes(k2, m, _lib.PROBES_UNIFORM, 1.0, 0.0, 63, 0x23 + s_log)
+        dk2 = gj.DeviceArray.from_numpy(k2, stream)
+        dp.upload(p2)
+        for kind in ("btree", "learned_plr"):
+            idx = gj.Index(kind, dk2, host_column=k2, stream=stream)
+            for appr in ("plain", "partitioning"):
+                ms = _time(gj, lambda: idx.lookup(dp, out, params=gj.lookup_params(appr), stream=stream), stream,
+                           steps=3, warmup=1)
+                emit({"config": f"index-size sweep: 2^{s_log} u64 keys ({(8 << s_log) / 2**30:.2f} GiB), "
+                                f"2^{m_log} probes", "index": kind, "approach": appr, "lookups_per_s": m / ms * 1e3,
+                      "ms": ms, "rowids_match": bool(np.array_equal(out.numpy(), e2))})
+            idx.close()
+        del dk2, k2, p2, e2
+    log("partitioned sweeps done")
+
+    # ---- learned indexes with misses ------------------------------------------------
+    for dist, dname in ((_lib.KEYS_UNIFORM, "uniform"), (_lib.KEYS_LOGNORMAL, "lognormal"),
+                        (_lib.KEYS_CLUSTERED, "clustered")):
+        k4 = gj.gen_build_keys(n, 8, dist, 63, 0x31 + dist)
+        p4, e4 = gj.gen_probes(k4, m, _lib.PROBES_UNIFORM, 1.0, 0.1, 63, 0x32 + dist)
+        dk4 = gj.DeviceArray.from_numpy(k4, stream)
+        dp.upload(p4)
+        variants = [("btree", {}), ("radix_spline", {"rs_max_error": 8}), ("radix_spline", {"rs_max_error": 32}),
+                    ("radix_spline", {"rs_max_error": 64}), ("learned_plr", {}), ("learned_mlp", {})]
+        for kind, prm in variants:
+            t = time.time()
+            idx = gj.Index(kind, dk4, host_column=k4, params=gj.index_params(**prm), stream=stream)
+            build_s = time.time() - t
+            st = idx.stats()
+            ms = _time(gj, lambda: idx.lookup(dp, out, params=gj.lookup_params(), stream=stream), stream, steps=3)
+            emit({"config": f"learned, 10% misses: 2^{n_log} {dname} u64 keys, 2^{m_log} probes, 10% misses", "index": kind,
+                  "params": prm, "lookups_per_s": m / ms * 1e3, "gather_roofline_frac": m / ms * 1e3 / G, "ms": ms,
+                  "rowids_match": bool(np.array_equal(out.numpy(), e4)), "build_s": build_s,
+                  "fallback_keys": st["fallback_keys"], "fallback_leaves": st["fallback_leaves"],
+                  "avg_window": st["avg_window"], "max_window": st["max_window"],
+                  "spline_points": st["spline_points"], "train_loss": st["train_loss"]})
+            idx.close()
+        del dk4, k4, p4, e4
+    log("learned sweeps done")
+    del dp, out
+
+    # ---- join-aggregate -------------------------------------------------------
+    dim_rows = 2_000_000
+    fact_rows = 6_000_000 if quick else 60_000_000
+    key, flag = gj.gen_dimension(dim_rows, 0x41)
+    skey, sflag = gj.sort_dimension(key, flag)
+    fkey, date, price, disc = gj.gen_fact(fact_rows, dim_rows, 2557, 90000, 10500000, 10, 0x42)
+    lo, hi = 1200, 1230
+    onum, oden, omt = oracle.join_aggregate(skey, sflag, fkey, date, price, disc, lo, hi, threads=cores)
+    dcols = [gj.DeviceArray.from_numpy(a, stream) for a in (fkey, date, price, disc)]
+    dflag = gj.DeviceArray.from_numpy(sflag, stream)
+    dkey = gj.DeviceArray.from_numpy(skey, stream)
+    rng = np.random.default_rng(0x43)
+    sizes, total = [], 0
+    while total < fact_rows:
+        s = int(np.exp(rng.uniform(np.log(1e3), np.log(1e7))))
+        s = min(s, fact_rows - total)
+        sizes.append(s)
+        total += s
+    idx = gj.Index("learned_plr", dkey, host_column=skey, stream=stream)
+    res1 = gj.DeviceArray(4, np.int64)
+
+    def single():
+        res1.fill(0, stream)
+        gj.join_aggregate(idx, *dcols, dflag, lo, hi, res1, stream=stream)
+
+    ms = _time(gj, single, stream)
+    got = tuple(int(v) for v in res1.numpy()[:3])
+    emit({"config": f"join-aggregate, one batch of {fact_rows} fact rows", "index": "learned_plr",
+          "rows_per_s": fact_rows / ms * 1e3, "ms": ms, "sums_match": got == (onum, oden, omt),
+          "promo_revenue": gj.promo_revenue(got[0], got[1]),
+          "algorithmic_GBps": (fact_rows * 4 + omt * 26) / (ms * 1e-3) / 1e9})
+    batches, off = [], 0
+    for s in sizes:
+        batches.append(tuple(c.offset(off) for c in dcols) + (s,))
+        off += s
+    res = gj.DeviceArray(4 * len(batches), np.int64)
+    for mode, mname in ((_lib.BATCH_LAUNCH, "launch"), (_lib.BATCH_GRAPH, "hipgraph"),
+                        (_lib.BATCH_PERSISTENT, "persistent")):
+        plan = gj.AggregatePlan(idx, batches, dflag, lo, hi, res, mode, stream=stream)
+        ms = _time(gj, lambda: plan.run(stream), stream)
+        sums = res.numpy().reshape(-1, 4)[:, :3].sum(axis=0)
+        emit({"config": f"join-aggregate, {len(sizes)} log-uniform batches (1e3..1e7 rows)", "mode": mname,
+              "rows_per_s": fact_rows / ms * 1e3, "ms": ms,
+              "sums_match": tuple(int(v) for v in sums) == (onum, oden, omt)})
+        plan.close()
+    if cpu:
+        t = time.perf_counter()
+        oracle.join_aggregate(skey, sflag, fkey, date, price, disc, lo, hi, threads=cores)
+        cpu_s = time.perf_counter() - t
+        emit({"config": "join-aggregate CPU baseline (oracle restatement of the Q14 loop, OpenMP)", "cores": cores,
+              "rows_per_s": fact_rows / cpu_s})
+    log("join-aggregate done")
