@@ -13,6 +13,8 @@ Reported:
                    (8 B probe key + 8 B indexed key + 4 B row id) / average kernel time, vs 8 TB/s
   gather_roofline  G = independent random 8 B gathers per second measured on the same GPU in the same
                    process (gj_gather), and lookups/s / G per index
+  hbm_traffic      measured HBM bytes per lookup launch (rocprofv3 counters, committed under profiles/)
+                   over the live average kernel time: how close the kernel runs to the 8 TB/s peak
   cpu_baseline     the oracle's OpenMP equality-checked lower_bound (port of binary_search_index) on a
                    bounded probe sample, on this host's cores
 
@@ -260,6 +262,10 @@ def main():
         "indexes": results,
         "learned_vs_btree": (results[head]["lookups_per_s"] / results["btree"]["lookups_per_s"])
         if head in ("learned_mlp", "learned_plr", "radix_spline") and "btree" in results else None,
+        "hbm_traffic": {"bytes_per_launch": traffic, "bytes_per_lookup": traffic / m if traffic else None,
+                        "GBps": traffic / avg_s / 1e9 if (traffic and r) else None,
+                        "frac_of_peak": traffic / avg_s / 1e9 / PEAK_HBM_GBPS if (traffic and r) else None,
+                        "source": "rocprofv3 FETCH_SIZE x2 + WRITE_SIZE (copy-calibrated), profiles/traffic.txt"},
         "stream_copy_GBps": copy_gbps,
         "pointer_chase_ns": chase_ns,
     }
