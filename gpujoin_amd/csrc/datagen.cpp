@@ -111,8 +111,11 @@ void distinct_sorted(K* out, uint64_t n, uint64_t seed, int threads, Sample&& sa
     v.erase(std::unique(v.begin(), v.end()), v.end());
     if (v.size() >= n) break;
     if (round > 64) throw std::invalid_argument("key distribution too narrow for the requested number of distinct keys");
+    // a narrow distribution yields fewer new values per draw as it fills up:
+    // the draw grows geometrically with the round (bounded by 4n)
     const uint64_t need = n - v.size();
-    draw(need + need / 4 + 64);
+    const uint64_t count = (need + need / 4 + 64) << std::min(round, 8);
+    draw(std::min<uint64_t>(count, 4 * n + 64));
   }
   if (v.size() > n) {
     const uint64_t excess = v.size() - n;
@@ -169,18 +172,24 @@ void gen_build_keys(K* out, uint64_t n, gj_key_dist dist, unsigned upper_bits, u
       return;
     }
     case GJ_KEYS_CLUSTERED: {
-      // 64 centres uniform in the range, Gaussian spread 2^20 around each
+      // 64 centres uniform in the range, Gaussian spread 2^20 around each.  Centre and
+      // offset are added as integers: a double near 2^63 resolves only multiples of 2^10.
       const uint64_t sc = sub_seed(seed, 3), s = sub_seed(seed, 4);
-      std::vector<double> centre(64);
-      for (int c = 0; c < 64; ++c) centre[c] = unit(rng(sc, c)) * upper;
+      std::vector<uint64_t> centre(64);
+      for (int c = 0; c < 64; ++c) centre[c] = maxv == ~0ull ? rng(sc, c) : below(rng(sc, c), maxv + 1);
       distinct_sorted(out, n, seed, threads, [&](uint64_t i) -> K {
         const uint64_t c = below(rng(s, 3 * i), 64);
         const double u1 = 1.0 - unit(rng(s, 3 * i + 1));
         const double u2 = unit(rng(s, 3 * i + 2));
         const double z = std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
-        double v = std::floor(centre[c] + z * 1048576.0);
-        if (v < 0) v = 0;
-        return static_cast<K>(v >= static_cast<double>(maxv) ? maxv : static_cast<uint64_t>(v));
+        const int64_t off = static_cast<int64_t>(std::floor(z * 1048576.0));  // |off| < 2^24
+        const uint64_t ctr = centre[c];
+        uint64_t v;
+        if (off < 0)
+          v = ctr < static_cast<uint64_t>(-off) ? 0 : ctr - static_cast<uint64_t>(-off);
+        else
+          v = maxv - ctr < static_cast<uint64_t>(off) ? maxv : ctr + static_cast<uint64_t>(off);
+        return static_cast<K>(v);
       });
       return;
     }

@@ -140,7 +140,7 @@ def test_radix_spline_error_invariant(gj, orc, err, dist):
     assert rs.max_abs_error() <= err + 1e-6
     px, py, radix = rs.arrays()
     assert px[0] == keys[0] and px[-1] == keys[-1]
-    assert np.all(np.diff(px.astype(np.float64)) > 0)
+    assert np.all(px[1:] > px[:-1])
     pos = np.searchsorted(keys, px)
     assert np.array_equal(keys[pos], px) and np.array_equal(py, pos.astype(np.float64))
     assert np.all(np.diff(radix.astype(np.int64)) >= 0) and radix[-1] == px.size
