@@ -12,10 +12,11 @@
 //
 // Kernels (no cooperative launch, no device-wide barrier):
 //   hist     one block per contiguous chunk, LDS histogram of 2^radix_bits u32 bins
-//   scan_p   per partition: column prefix over the blocks and the partition total
-//   scan_s   one block: aligned exclusive prefix over partitions -> partition starts
-//   scatter  one block per chunk; waves take turns in order, each assigning stable
-//            positions from LDS running offsets (wave-level peer masks by ballots)
+//   colscan  one wave per partition: column prefix over the blocks and the partition total
+//   starts   one block: aligned exclusive prefix over partitions -> partition starts
+//   scatter  one block per chunk, in register tiles of 4096 items; per tile each wave
+//            counts its items per partition, wave cursors follow from the counts, and
+//            each wave writes its items in order (ranks from ballots)
 #include <algorithm>
 
 #include "common.hpp"
@@ -24,6 +25,9 @@
 namespace gj {
 
 constexpr int kPartBlock = 256;
+constexpr int kPartWaves = kPartBlock / kWave;
+constexpr int kPartItems = 16;                        // items per lane per tile
+constexpr int kPartTile = kPartBlock * kPartItems;    // 4096 items
 constexpr uint32_t kMaxRadixBits = 12;
 
 __host__ __device__ __forceinline__ uint32_t part_align(int key_bytes) { return 128u / static_cast<uint32_t>(key_bytes); }
@@ -48,18 +52,27 @@ __global__ __launch_bounds__(kPartBlock) void part_hist_kernel(const K* __restri
   for (uint32_t p = threadIdx.x; p < fan; p += blockDim.x) hist[static_cast<uint64_t>(blockIdx.x) * fan + p] = h[p];
 }
 
-// hist[b][p] <- sum_{b' < b} hist[b'][p]; count[p] <- total
-__global__ void part_scan_blocks_kernel(uint32_t* __restrict__ hist, uint32_t blocks, uint32_t fan,
-                                        uint64_t* __restrict__ count) {
-  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= fan) return;
-  uint64_t run = 0;
-  for (uint32_t b = 0; b < blocks; ++b) {
-    const uint32_t v = hist[static_cast<uint64_t>(b) * fan + p];
-    hist[static_cast<uint64_t>(b) * fan + p] = static_cast<uint32_t>(run);
-    run += v;
+// hist[b][p] <- sum_{b' < b} hist[b'][p]; count[p] <- total.  One wave per
+// partition scans its column 64 rows at a time (wave prefix sum by shuffles).
+__global__ __launch_bounds__(256) void part_colscan_kernel(uint32_t* __restrict__ hist, uint32_t blocks, uint32_t fan,
+                                                           uint64_t* __restrict__ count) {
+  const uint32_t p = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  if (p >= fan) return;  // wave-uniform
+  const int ln = lane();
+  uint64_t carry = 0;
+  for (uint32_t b0 = 0; b0 < blocks; b0 += kWave) {
+    const uint32_t b = b0 + static_cast<uint32_t>(ln);
+    const uint32_t v = b < blocks ? hist[static_cast<uint64_t>(b) * fan + p] : 0u;
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      const uint32_t y = __shfl_up(x, off, kWave);
+      if (ln >= off) x += y;
+    }
+    if (b < blocks) hist[static_cast<uint64_t>(b) * fan + p] = static_cast<uint32_t>(carry) + x - v;
+    carry += __shfl(x, kWave - 1, kWave);
   }
-  count[p] = run;
+  if (ln == 0) count[p] = carry;
 }
 
 // start[p] = align_up(start[p-1] + count[p-1], align); one block of 1024 threads.
@@ -107,50 +120,102 @@ __global__ void part_pad_kernel(const uint64_t* __restrict__ start, const uint64
 }
 
 // Stable scatter.  in_payload == nullptr: payload = payload_base + input position.
+// A block walks its chunk in tiles of kPartTile items held in registers (wave w
+// owns items [w * 1024, (w + 1) * 1024) of the tile, lane-minor).  Per tile:
+//   count   each wave counts its items per partition (LDS row of the wave; the
+//           lowest lane of each group of equal digits adds the group size)
+//   cursor  per partition: wave cursors = block cursor + counts of earlier waves
+//   scatter each wave writes its items at its cursors (rank among equal digits
+//           from ballots) and advances them; no ordering between waves is needed
+// so a tile costs two block barriers, and the output order is the input order
+// inside every partition.
 template <class K>
 __global__ __launch_bounds__(kPartBlock) void part_scatter_kernel(
     const K* __restrict__ keys, const uint32_t* __restrict__ in_payload, uint32_t payload_base, uint64_t m,
     uint64_t chunk, uint32_t shift, uint32_t bits, const uint32_t* __restrict__ block_off,
-    const uint64_t* __restrict__ start, K* __restrict__ out_keys, uint32_t* __restrict__ out_payload) {
-  extern __shared__ uint32_t run[];  // running output position per partition (relative to 0)
+    const uint64_t* __restrict__ start, K* __restrict__ out_keys, uint32_t* __restrict__ out_payload, uint64_t cap) {
+  extern __shared__ uint32_t sm[];
   const uint32_t fan = 1u << bits;
-  for (uint32_t p = threadIdx.x; p < fan; p += blockDim.x)
-    run[p] = static_cast<uint32_t>(start[p]) + block_off[static_cast<uint64_t>(blockIdx.x) * fan + p];
+  uint32_t* cnt = sm;                        // [kPartWaves][fan]: counts, then cursors
+  uint32_t* brun = sm + kPartWaves * fan;    // [fan]: block cursor (absolute position)
+  for (uint32_t p = threadIdx.x; p < fan; p += blockDim.x) {
+    brun[p] = static_cast<uint32_t>(start[p]) + block_off[static_cast<uint64_t>(blockIdx.x) * fan + p];
+#pragma unroll
+    for (int w = 0; w < kPartWaves; ++w) cnt[w * fan + p] = 0;
+  }
   __syncthreads();
   const uint64_t lo = static_cast<uint64_t>(blockIdx.x) * chunk;
   const uint64_t hi = std::min<uint64_t>(m, lo + chunk);
   const int w = threadIdx.x / kWave;
-  const int nw = blockDim.x / kWave;
   const int ln = lane();
   const uint64_t lt = (ln == 0) ? 0ull : (~0ull >> (64 - ln));  // lanes below this one
-  for (uint64_t tile = lo; tile < hi; tile += static_cast<uint64_t>(blockDim.x)) {
-    const uint64_t i = tile + static_cast<uint64_t>(threadIdx.x);
-    const bool valid = i < hi;
-    K k = K(0);
-    uint32_t pay = 0;
-    if (valid) {
-      k = load_stream(keys + i);
-      pay = in_payload ? load_stream(in_payload + i) : payload_base + static_cast<uint32_t>(i);
+  uint32_t* mine = cnt + w * fan;
+  for (uint64_t tile = lo; tile < hi; tile += kPartTile) {
+    const uint64_t wbase = tile + static_cast<uint64_t>(w) * (kPartItems * kWave);
+    K k[kPartItems];
+    uint32_t pay[kPartItems];
+#pragma unroll
+    for (int j = 0; j < kPartItems; ++j) {
+      const uint64_t i = wbase + static_cast<uint64_t>(j * kWave + ln);
+      const bool valid = i < hi;
+      k[j] = valid ? load_stream(keys + i) : K(0);
+      pay[j] = valid ? (in_payload ? load_stream(in_payload + i) : payload_base + static_cast<uint32_t>(i)) : 0u;
     }
-    const uint32_t d = digit_of(k, shift, fan - 1);
-    // lanes of this wave holding the same digit
-    uint64_t peers = ballot(valid);
-    for (uint32_t b = 0; b < bits; ++b) {
-      const uint64_t set = ballot((d >> b) & 1u);
-      peers &= ((d >> b) & 1u) ? set : ~set;
-    }
-    const uint32_t rank = static_cast<uint32_t>(__popcll(peers & lt));
-    const bool leader = valid && (peers & lt) == 0;
-    for (int turn = 0; turn < nw; ++turn) {
-      if (turn == w && valid) {
-        const uint32_t pos = run[d] + rank;
-        out_keys[pos] = k;
-        out_payload[pos] = pay;
+    // lanes of this wave whose item j has the same digit (invalid items match nothing)
+    auto peers_of = [&](int j, uint32_t d) {
+      const uint64_t i = wbase + static_cast<uint64_t>(j * kWave + ln);
+      uint64_t peers = ballot(i < hi);
+      for (uint32_t b = 0; b < bits; ++b) {
+        const uint64_t set = ballot((d >> b) & 1u);
+        peers &= ((d >> b) & 1u) ? set : ~set;
       }
-      __syncthreads();
-      if (turn == w && leader) run[d] += static_cast<uint32_t>(__popcll(peers));
-      __syncthreads();
+      return (i < hi) ? peers : 0ull;
+    };
+#pragma unroll
+    for (int j = 0; j < kPartItems; ++j) {
+      const uint32_t d = digit_of(k[j], shift, fan - 1);
+      const uint64_t peers = peers_of(j, d);
+      if (peers && (peers & lt) == 0) mine[d] += static_cast<uint32_t>(__popcll(peers));
+      __builtin_amdgcn_wave_barrier();  // keeps the wave's LDS accesses in program order
     }
+    __syncthreads();
+    // a cursor is written only where the wave has items of the partition: those
+    // entries are the ones the wave zeroes after its scatter, so every entry is a
+    // zero count again when the next tile starts
+    for (uint32_t p = threadIdx.x; p < fan; p += blockDim.x) {
+      uint32_t r = brun[p];
+#pragma unroll
+      for (int v = 0; v < kPartWaves; ++v) {
+        const uint32_t c = cnt[v * fan + p];
+        if (c != 0) cnt[v * fan + p] = r;
+        r += c;
+      }
+      brun[p] = r;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kPartItems; ++j) {
+      const uint32_t d = digit_of(k[j], shift, fan - 1);
+      const uint64_t peers = peers_of(j, d);
+      if (peers) {
+        const uint32_t at = mine[d];  // every lane of the group reads the cursor before the leader moves it
+        const uint32_t pos = at + static_cast<uint32_t>(__popcll(peers & lt));
+        if (pos < cap) {  // always true for a consistent histogram; guards the buffers regardless
+          out_keys[pos] = k[j];
+          out_payload[pos] = pay[j];
+        }
+        if ((peers & lt) == 0) mine[d] = at + static_cast<uint32_t>(__popcll(peers));
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+    // cursors back to zero counts for the next tile (this wave's row only)
+#pragma unroll
+    for (int j = 0; j < kPartItems; ++j) {
+      const uint32_t d = digit_of(k[j], shift, fan - 1);
+      const uint64_t i = wbase + static_cast<uint64_t>(j * kWave + ln);
+      if (i < hi) mine[d] = 0;
+    }
+    __syncthreads();  // the next tile's cursor step reads every wave's row
   }
 }
 
@@ -164,11 +229,11 @@ struct PartPlan {
 PartPlan plan(uint64_t m, uint32_t bits, int cu) {
   PartPlan p;
   p.fan = 1u << bits;
-  // a few blocks per CU; each block's histogram row lives in scratch
-  uint64_t want = static_cast<uint64_t>(cu) * 4;
-  const uint64_t min_chunk = 4096;
-  uint64_t chunk = std::max<uint64_t>(min_chunk, ceil_div(m, want));
-  chunk = ceil_div(chunk, kPartBlock) * kPartBlock;
+  // whole tiles per block; the scatter's LDS (5 x 4 B per partition) allows two
+  // blocks per CU at fan-out 4096, more below; each block's histogram row lives in scratch
+  const uint64_t want = static_cast<uint64_t>(cu) * (bits > 10 ? 2 : 4);
+  uint64_t chunk = std::max<uint64_t>(kPartTile, ceil_div(m, want));
+  chunk = ceil_div(chunk, kPartTile) * kPartTile;
   p.chunk = chunk;
   p.blocks = static_cast<uint32_t>(std::max<uint64_t>(1, ceil_div(m, chunk)));
   return p;
@@ -196,7 +261,7 @@ void partition_pass(const K* keys, const uint32_t* in_payload, uint32_t payload_
   hipLaunchKernelGGL(part_hist_kernel<K>, dim3(p.blocks), dim3(kPartBlock), lds, s, keys, m, p.chunk, shift, bits,
                      hist);
   GJ_HIP_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(part_scan_blocks_kernel, dim3((p.fan + 255) / 256), dim3(256), 0, s, hist, p.blocks, p.fan, count);
+  hipLaunchKernelGGL(part_colscan_kernel, dim3((p.fan + 3) / 4), dim3(256), 0, s, hist, p.blocks, p.fan, count);
   GJ_HIP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(part_scan_starts_kernel, dim3(1), dim3(1024), 0, s, count, p.fan, align, start);
   GJ_HIP_CHECK(hipGetLastError());
@@ -205,8 +270,13 @@ void partition_pass(const K* keys, const uint32_t* in_payload, uint32_t payload_
                        out_payload);
     GJ_HIP_CHECK(hipGetLastError());
   }
-  hipLaunchKernelGGL(part_scatter_kernel<K>, dim3(p.blocks), dim3(kPartBlock), lds, s, keys, in_payload, payload_base,
-                     m, p.chunk, shift, bits, hist, start, out_keys, out_payload);
+  const size_t slds = static_cast<size_t>(kPartWaves + 1) * p.fan * sizeof(uint32_t);  // <= 80 KiB of the 160
+  if (slds > 65536)
+    GJ_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&part_scatter_kernel<K>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(slds)));
+  hipLaunchKernelGGL(part_scatter_kernel<K>, dim3(p.blocks), dim3(kPartBlock), slds, s, keys, in_payload, payload_base,
+                     m, p.chunk, shift, bits, hist, start, out_keys, out_payload,
+                     m + (align > 1 ? static_cast<uint64_t>(p.fan) * align : 0ull));
   GJ_HIP_CHECK(hipGetLastError());
 }
 

@@ -47,6 +47,28 @@ def test_partition_matches_oracle(gj, orc, gpu, kb, bits, passes, m):
         assert np.all(gp[o + c:pad_end] == 0xFFFFFFFF)
 
 
+@pytest.mark.parametrize("bits,passes", [(10, 1), (12, 1), (12, 2)])
+def test_partition_many_tiles_per_block(gj, orc, gpu, bits, passes):
+    # 5e6 keys: every block of the scatter walks several 4096-item register tiles,
+    # and the last tile of the last block is partial
+    m = 5_000_003
+    print(f"[case] partition many tiles bits={bits} passes={passes} m={m}", flush=True)
+    rng = np.random.default_rng(bits + 17 * passes)
+    keys = rng.integers(0, 2 ** 64 - 1, size=m, dtype=np.uint64)
+    ignore = 64 - bits
+    dk, dp, off, cnt = gj.partition(gj.DeviceArray.from_numpy(keys), m, bits, ignore, passes)
+    gj.synchronize()
+    counts, ok, op = orc.partition(keys, bits, ignore)
+    assert np.array_equal(cnt.numpy(), counts)
+    offs = off.numpy().astype(np.int64)
+    starts = np.concatenate([[0], np.cumsum(counts)[:-1]]).astype(np.int64)
+    # gather every partition's run from the padded layout into the unpadded order
+    idx = np.repeat(offs - starts, counts.astype(np.int64)) + np.arange(m, dtype=np.int64)
+    gk, gp = dk.numpy(), dp.numpy()
+    assert np.array_equal(gk[idx], ok)
+    assert np.array_equal(gp[idx], op)
+
+
 def test_partition_rejects_bad_args(gj, gpu):
     keys = gj.DeviceArray(16, np.uint64)
     with pytest.raises(gj.GjError):
