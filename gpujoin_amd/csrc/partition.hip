@@ -45,11 +45,12 @@ __global__ __launch_bounds__(kPartBlock) void part_hist_kernel(const K* __restri
   const uint32_t fan = 1u << bits;
   for (uint32_t p = threadIdx.x; p < fan; p += blockDim.x) h[p] = 0;
   __syncthreads();
-  const uint64_t lo = static_cast<uint64_t>(blockIdx.x) * chunk;
+  const uint64_t c = xcd_logical_block();  // chunk index; the scatter maps blocks the same way
+  const uint64_t lo = c * chunk;
   const uint64_t hi = std::min<uint64_t>(m, lo + chunk);
   for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) atomicAdd(&h[digit_of(load_stream(keys + i), shift, fan - 1)], 1u);
   __syncthreads();
-  for (uint32_t p = threadIdx.x; p < fan; p += blockDim.x) hist[static_cast<uint64_t>(blockIdx.x) * fan + p] = h[p];
+  for (uint32_t p = threadIdx.x; p < fan; p += blockDim.x) hist[c * fan + p] = h[p];
 }
 
 // hist[b][p] <- sum_{b' < b} hist[b'][p]; count[p] <- total.  One wave per
@@ -138,13 +139,16 @@ __global__ __launch_bounds__(kPartBlock) void part_scatter_kernel(
   const uint32_t fan = 1u << bits;
   uint32_t* cnt = sm;                        // [kPartWaves][fan]: counts, then cursors
   uint32_t* brun = sm + kPartWaves * fan;    // [fan]: block cursor (absolute position)
+  // neighbouring chunks run on one XCD, so the partial lines that neighbouring
+  // chunks write into a partition's run meet in the same L2
+  const uint64_t c = xcd_logical_block();
   for (uint32_t p = threadIdx.x; p < fan; p += blockDim.x) {
-    brun[p] = static_cast<uint32_t>(start[p]) + block_off[static_cast<uint64_t>(blockIdx.x) * fan + p];
+    brun[p] = static_cast<uint32_t>(start[p]) + block_off[c * fan + p];
 #pragma unroll
     for (int w = 0; w < kPartWaves; ++w) cnt[w * fan + p] = 0;
   }
   __syncthreads();
-  const uint64_t lo = static_cast<uint64_t>(blockIdx.x) * chunk;
+  const uint64_t lo = c * chunk;
   const uint64_t hi = std::min<uint64_t>(m, lo + chunk);
   const int w = threadIdx.x / kWave;
   const int ln = lane();
