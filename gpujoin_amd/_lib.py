@@ -43,7 +43,7 @@ class IndexParams(C.Structure):
         ("rs_radix_bits", C.c_uint32), ("rs_max_error", C.c_uint32), ("harmonia_group_lanes", C.c_uint32),
         ("btree_lds_bytes", C.c_uint32), ("learned_leaf_bits", C.c_uint32), ("learned_max_error", C.c_uint32),
         ("mlp_hidden", C.c_uint32), ("mlp_steps", C.c_uint32), ("mlp_batch", C.c_uint32), ("mlp_lr", C.c_float),
-        ("seed", C.c_uint64),
+        ("seed", C.c_uint64), ("learned_root_segments", C.c_uint32),
     ]
 
 
@@ -59,7 +59,7 @@ class IndexStats(C.Structure):
         ("kind", C.c_int), ("key_bytes", C.c_int), ("n", C.c_uint64), ("memory_bytes", C.c_uint64),
         ("depth", C.c_uint32), ("lds_levels", C.c_uint32), ("spline_points", C.c_uint64), ("leaves", C.c_uint64),
         ("fallback_leaves", C.c_uint64), ("fallback_keys", C.c_uint64), ("max_window", C.c_uint32),
-        ("avg_window", C.c_double), ("train_loss", C.c_float),
+        ("avg_window", C.c_double), ("train_loss", C.c_float), ("root_segments", C.c_uint32),
     ]
 
 

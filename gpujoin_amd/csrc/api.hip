@@ -14,7 +14,7 @@ namespace {
 void fill_defaults(gj_index_params& p) {
   if (!p.rs_radix_bits) p.rs_radix_bits = 18;
   if (!p.rs_max_error) p.rs_max_error = 32;
-  if (!p.harmonia_group_lanes) p.harmonia_group_lanes = 16;
+  if (!p.harmonia_group_lanes) p.harmonia_group_lanes = 8;  // one 128-byte line per load
   if (!p.btree_lds_bytes) p.btree_lds_bytes = 16384;
   if (!p.learned_leaf_bits) p.learned_leaf_bits = 16;
   if (!p.learned_max_error) p.learned_max_error = 64;

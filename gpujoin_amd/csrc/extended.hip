@@ -147,7 +147,7 @@ void build_harmonia(gj_index& ix, hipStream_t s) {
 // until it does (the FP64 interpolation can round across the exact bound).
 // ---------------------------------------------------------------------------
 template <class K>
-__global__ void rs_check_kernel(RadixSplineDev<K, 8> d, unsigned long long* bad) {
+__global__ void rs_check_kernel(RadixSplineDev<K> d, unsigned long long* bad) {
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
   unsigned long long cnt = 0;
   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < d.n; i += stride) {
@@ -184,17 +184,15 @@ void build_rs(gj_index& ix, hipStream_t s) {
   ix.rs_points = rs.knot_x.size();
   ix.rs_radix.alloc(rs.radix.size() * sizeof(uint32_t));
   GJ_HIP_CHECK(hipMemcpyAsync(ix.rs_radix.p, rs.radix.data(), ix.rs_radix.bytes, hipMemcpyHostToDevice, s));
-  std::vector<K> kx(rs.knot_x.size());
-  for (size_t i = 0; i < kx.size(); ++i) kx[i] = static_cast<K>(rs.knot_x[i]);
-  ix.rs_knot_x.alloc(kx.size() * sizeof(K));
-  GJ_HIP_CHECK(hipMemcpyAsync(ix.rs_knot_x.p, kx.data(), ix.rs_knot_x.bytes, hipMemcpyHostToDevice, s));
-  ix.rs_knot_y.alloc(rs.knot_y.size() * sizeof(double));
-  GJ_HIP_CHECK(hipMemcpyAsync(ix.rs_knot_y.p, rs.knot_y.data(), ix.rs_knot_y.bytes, hipMemcpyHostToDevice, s));
+  std::vector<RsKnot> kn(rs.knot_x.size());
+  for (size_t i = 0; i < kn.size(); ++i) kn[i] = RsKnot{static_cast<uint64_t>(rs.knot_x[i]), rs.knot_y[i]};
+  ix.rs_knots.alloc(kn.size() * sizeof(RsKnot));
+  GJ_HIP_CHECK(hipMemcpyAsync(ix.rs_knots.p, kn.data(), ix.rs_knots.bytes, hipMemcpyHostToDevice, s));
   ix.rs_err = ix.params.rs_max_error;
   dbuf bad(sizeof(unsigned long long));
   for (int attempt = 0; attempt < 40; ++attempt) {
     fill_bytes(bad.p, 0, bad.bytes, s);
-    hipLaunchKernelGGL(rs_check_kernel<K>, dim3(grid_for(ix.n, 256, 8192)), dim3(256), 0, s, ix.rs_dev<K, 8>(),
+    hipLaunchKernelGGL(rs_check_kernel<K>, dim3(grid_for(ix.n, 256, 8192)), dim3(256), 0, s, ix.rs_dev<K>(),
                        bad.as<unsigned long long>());
     GJ_HIP_CHECK(hipGetLastError());
     unsigned long long h = 0;
@@ -252,8 +250,8 @@ template void build_hash_table<uint32_t>(const gj_index&, dbuf&, uint64_t&, hipS
 template void build_hash_table<uint64_t>(const gj_index&, dbuf&, uint64_t&, hipStream_t);
 
 uint64_t extended_memory(const gj_index& ix) {
-  return ix.hm_keys.bytes + ix.hm_children.bytes + ix.rs_radix.bytes + ix.rs_knot_x.bytes + ix.rs_knot_y.bytes +
-         ix.lm_leaves.bytes + ix.lm_mlp.bytes + ix.ht.bytes;
+  return ix.hm_keys.bytes + ix.hm_children.bytes + ix.rs_radix.bytes + ix.rs_knots.bytes +
+         ix.lm_leaves.bytes + ix.lm_mlp.bytes + ix.lm_knots.bytes + ix.ht.bytes;
 }
 
 template <class K>

@@ -49,12 +49,12 @@ struct gj_index {
   uint32_t hm_depth = 0, hm_lds_levels = 0;
 
   // RadixSpline
-  gj::dbuf rs_radix, rs_knot_x, rs_knot_y;
+  gj::dbuf rs_radix, rs_knots;
   uint32_t rs_shift = 0, rs_err = 0;
   uint64_t rs_points = 0;
 
   // learned two-stage models (B-tree above is their fallback)
-  gj::dbuf lm_leaves, lm_mlp;
+  gj::dbuf lm_leaves, lm_mlp, lm_knots;
   gj::RootModel lm_root{};
   uint32_t lm_num_leaves = 0, lm_fallback_any = 0;
   int32_t lm_err_lo = 0, lm_err_hi = 0;  // learned window bounds over all leaves
@@ -94,14 +94,13 @@ struct gj_index {
     return d;
   }
 
-  template <class K, int G>
-  gj::RadixSplineDev<K, G> rs_dev() const {
-    gj::RadixSplineDev<K, G> d{};
+  template <class K>
+  gj::RadixSplineDev<K> rs_dev() const {
+    gj::RadixSplineDev<K> d{};
     d.col = static_cast<const K*>(d_col);
     d.n = n;
     d.radix = rs_radix.as<const uint32_t>();
-    d.knot_x = rs_knot_x.as<const K>();
-    d.knot_y = rs_knot_y.as<const double>();
+    d.knots = rs_knots.as<const gj::RsKnot>();
     d.min_key = static_cast<K>(min_key);
     d.max_key = static_cast<K>(max_key);
     d.shift_bits = rs_shift;
@@ -118,6 +117,7 @@ struct gj_index {
     d.max_key = static_cast<K>(max_key);
     d.root = lm_root;
     d.root.mlp = lm_mlp.as<const float>();
+    d.root.knots = lm_knots.as<const uint64_t>();
     d.leaves = lm_leaves.as<const gj::LeafModel>();
     d.num_leaves = lm_num_leaves;
     d.err_lo = lm_err_lo;

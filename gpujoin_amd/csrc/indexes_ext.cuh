@@ -282,18 +282,24 @@ __device__ __forceinline__ uint32_t group_window_search(const K* col, uint64_t n
 
 // ---------------------------------------------------------------------------
 // RadixSpline (radix_spline_index, src/indexes.cuh:176-248; estimate
-// src/rs.cuh:118-154).  Knots are stored structure-of-arrays (keys, FP64
-// positions) so the knot search reads 8-byte keys; the estimate is the
-// reference's FP64 interpolation with an explicit fma, and the last mile is
+// src/rs.cuh:118-154).  Knots are {u64 key, f64 position} pairs, 16 bytes, like
+// the reference's Coord (src/rs.cuh:38-52): the two knots around a probe are 32
+// contiguous bytes, read by two 16-byte loads issued together.  The estimate is
+// the reference's FP64 interpolation with an explicit fma, and the last mile is
 // the line-walking window search over [est - err, est + err + 2).
 // ---------------------------------------------------------------------------
-template <class K, int G>
+struct RsKnot {
+  uint64_t x;
+  double y;
+};
+static_assert(sizeof(RsKnot) == 16, "16-byte knot");
+
+template <class K>
 struct RadixSplineDev {
   const K* col;
   uint64_t n;
   const uint32_t* radix;  // radix table
-  const K* knot_x;
-  const double* knot_y;
+  const RsKnot* knots;
   K min_key, max_key;
   uint32_t shift_bits;
   uint32_t err;  // search bound (>= the builder's bound; widened at build until every key is inside)
@@ -309,17 +315,16 @@ struct RadixSplineDev {
     uint64_t count = static_cast<uint64_t>(radix[prefix + 1]) - lower;
     while (count > 0) {
       const uint64_t step = count >> 1, mid = lower + step;
-      if (knot_x[mid] < x) {
+      if (knots[mid].x < static_cast<uint64_t>(x)) {
         lower = mid + 1;
         count -= step + 1;
       } else {
         count = step;
       }
     }
-    const K x0 = knot_x[lower - 1], x1 = knot_x[lower];
-    const double y0 = knot_y[lower - 1], y1 = knot_y[lower];
-    const double slope = (y1 - y0) / static_cast<double>(x1 - x0);
-    return __builtin_fma(static_cast<double>(x - x0), slope, y0);
+    const RsKnot a = knots[lower - 1], b = knots[lower];
+    const double slope = (b.y - a.y) / static_cast<double>(b.x - a.x);
+    return __builtin_fma(static_cast<double>(static_cast<uint64_t>(x) - a.x), slope, a.y);
   }
 
   __device__ __forceinline__ void window(double est, uint64_t& lo, uint64_t& hi) const {
@@ -344,8 +349,9 @@ struct RadixSplineDev {
 // ---------------------------------------------------------------------------
 // Learned two-stage position model (no reference counterpart).
 //   root:  cdf = root(x), x = (key - min_key) / (max_key - min_key) in FP32;
-//          linear (a*x + b) or a monotone 1xH MLP with ramp units
-//          clamp(w1*x + b1, 0, 1) (w1, w2 >= 0); leaf = floor(cdf * L)
+//          linear (a*x + b), a monotone 1xH MLP with ramp units
+//          clamp(w1*x + b1, 0, 1) (w1, w2 >= 0), or an equi-depth spline over
+//          the column's quantile keys (any key distribution); leaf = floor(cdf * L)
 //   leaf:  pred = start + floor(slope * dkey) in FP32, dkey = key - leaf base key
 //          (the fit's intercept is folded into start); one 16-byte load.
 //   window: [pred + err_lo, pred + err_hi], with err_lo/err_hi the extremes of
@@ -368,9 +374,32 @@ constexpr int kMaxHidden = 64;
 struct RootModel {
   float a, b;                 // linear root
   float inv_range;            // 1 / (max_key - min_key) in FP32
-  uint32_t hidden;            // 0: linear root
+  uint32_t hidden;            // > 0: MLP root
   const float* mlp;           // [w1(h), b1(h), w2(h), b2] in global memory (staged in LDS)
+  uint32_t segments;          // > 0: equi-depth spline root (a power of two)
+  const uint64_t* knots;      // segments + 1 keys: knot s = key at position s * n / segments, last = max key;
+                              // then `segments` FP32 inverse knot gaps (staged in LDS)
+
+  __host__ __device__ size_t knot_bytes() const { return (static_cast<size_t>(segments) + 1) * 8 + 8; }
+  __host__ __device__ size_t bytes() const {
+    if (hidden) return ((3 * hidden + 1) * 4 + 15) / 16 * 16;
+    if (segments) return (knot_bytes() + static_cast<size_t>(segments) * 4 + 15) / 16 * 16;
+    return 0;
+  }
+  __host__ __device__ const void* params() const { return hidden ? static_cast<const void*>(mlp) : knots; }
 };
+
+// Equi-depth piecewise-linear CDF: the segment s with knot[s] <= key < knot[s + 1]
+// by a branch-free binary search, then linear inside it.  Monotone in the key,
+// so every leaf still owns a contiguous key range.
+__device__ __forceinline__ float spline_cdf(uint64_t key, const uint64_t* knot, const float* inv_gap,
+                                            uint32_t segments) {
+  uint32_t s = 0;
+  for (uint32_t step = segments >> 1; step > 0; step >>= 1) s = (knot[s + step] <= key) ? s + step : s;
+  float f = key >= knot[s] ? static_cast<float>(key - knot[s]) * inv_gap[s] : 0.0f;
+  f = fminf(f, 1.0f);
+  return (static_cast<float>(s) + f) * (1.0f / static_cast<float>(segments));
+}
 
 __device__ __forceinline__ float mlp_eval(const float* p, uint32_t h, float x) {
   float f = p[3 * h];
@@ -419,24 +448,23 @@ struct LearnedDev {
   int32_t err_lo, err_hi;  // window [pred + err_lo, pred + err_hi]
   BTreeDev<K> btree;       // fallback (used when FB)
 
-  __host__ __device__ size_t mlp_offset() const { return FB ? btree.lds_bytes() : 0; }
-  __host__ __device__ size_t lds_bytes() const {
-    size_t b = mlp_offset();
-    if (root.hidden) b += ((3 * root.hidden + 1) * 4 + 15) / 16 * 16;
-    return b;
-  }
+  __host__ __device__ size_t root_offset() const { return FB ? btree.lds_bytes() : 0; }
+  __host__ __device__ size_t lds_bytes() const { return root_offset() + root.bytes(); }
   __device__ void stage(uint8_t* lds) const {
     if constexpr (FB) btree.stage(lds);
-    if (root.hidden) stage_to_lds(lds + mlp_offset(), root.mlp, ((3 * root.hidden + 1) * 4 + 15) / 16 * 16);
+    if (root.bytes()) stage_to_lds(lds + root_offset(), root.params(), root.bytes());
   }
 
   __device__ __forceinline__ uint32_t route(K x, const uint8_t* lds) const {
-    const float u = key_unit(x, min_key, root.inv_range);
+    const uint8_t* rp = lds + root_offset();
     float cdf;
-    if (root.hidden) {
-      cdf = mlp_eval(reinterpret_cast<const float*>(lds + mlp_offset()), root.hidden, u);
+    if (root.segments) {
+      cdf = spline_cdf(static_cast<uint64_t>(x), reinterpret_cast<const uint64_t*>(rp),
+                       reinterpret_cast<const float*>(rp + root.knot_bytes()), root.segments);
+    } else if (root.hidden) {
+      cdf = mlp_eval(reinterpret_cast<const float*>(rp), root.hidden, key_unit(x, min_key, root.inv_range));
     } else {
-      cdf = __builtin_fmaf(root.a, u, root.b);
+      cdf = __builtin_fmaf(root.a, key_unit(x, min_key, root.inv_range), root.b);
     }
     return leaf_of(cdf, num_leaves);
   }

@@ -220,11 +220,35 @@ struct Router {
   RootModel root;
   uint32_t leaves;
   __device__ __forceinline__ uint32_t operator()(K key) const {
-    const float u = key_unit(key, min_key, root.inv_range);
-    const float cdf = root.hidden ? mlp_eval(root.mlp, root.hidden, u) : __builtin_fmaf(root.a, u, root.b);
+    float cdf;
+    if (root.segments) {
+      cdf = spline_cdf(static_cast<uint64_t>(key), root.knots,
+                       reinterpret_cast<const float*>(reinterpret_cast<const uint8_t*>(root.knots) + root.knot_bytes()),
+                       root.segments);
+    } else {
+      const float u = key_unit(key, min_key, root.inv_range);
+      cdf = root.hidden ? mlp_eval(root.mlp, root.hidden, u) : __builtin_fmaf(root.a, u, root.b);
+    }
     return leaf_of(cdf, leaves);
   }
 };
+
+// Spline root: knot s = key at position floor(s * n / S), knot S = the max key;
+// inverse gaps in FP32 (0 for an empty gap).
+template <class K>
+__global__ void spline_knots_kernel(const K* __restrict__ col, uint64_t n, uint32_t segs, uint64_t* __restrict__ knot,
+                                    float* __restrict__ inv_gap) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s > segs) return;
+  const uint64_t i = (s == segs) ? n - 1 : static_cast<uint64_t>((static_cast<unsigned __int128>(s) * n) / segs);
+  const uint64_t k = static_cast<uint64_t>(col[i]);
+  knot[s] = k;
+  if (s < segs) {
+    const uint64_t j = (s + 1 == segs) ? n - 1 : static_cast<uint64_t>((static_cast<unsigned __int128>(s + 1) * n) / segs);
+    const uint64_t k1 = static_cast<uint64_t>(col[j]);
+    inv_gap[s] = k1 > k ? static_cast<float>(1.0 / static_cast<double>(k1 - k)) : 0.0f;
+  }
+}
 
 // start[l] = first position routed to a leaf >= l (start[] pre-filled with n); counts
 // routing inversions (which must not occur: the root is monotone).
@@ -429,69 +453,106 @@ void build_learned(gj_index& ix, hipStream_t s) {
     root.mlp = ix.lm_mlp.as<const float>();
   }
 
-  // routing and leaf boundaries
-  Router<K> route{kmin, root, L};
-  dbuf start((static_cast<uint64_t>(L) + 1 + 3) * sizeof(uint32_t));
-  {
-    // fill start[] with n
-    std::vector<uint32_t> init(L + 1, static_cast<uint32_t>(n));
-    GJ_HIP_CHECK(hipMemcpyAsync(start.p, init.data(), init.size() * 4, hipMemcpyHostToDevice, s));
-    dbuf inv(sizeof(unsigned long long));
-    fill_bytes(inv.p, 0, inv.bytes, s);
-    hipLaunchKernelGGL(leaf_bounds_kernel<K>, dim3(grid_for(n, 256)), dim3(256), 0, s, col, n, route,
-                       start.as<uint32_t>(), inv.as<unsigned long long>());
+  // spline root over `segs` equi-depth segments (stored in ix.lm_knots)
+  auto spline_root = [&](uint32_t segs) {
+    RootModel r{};
+    r.inv_range = root.inv_range;
+    r.segments = segs;
+    ix.lm_knots.alloc(r.bytes());
+    fill_bytes(ix.lm_knots.p, 0, ix.lm_knots.bytes, s);
+    uint8_t* base = ix.lm_knots.as<uint8_t>();
+    hipLaunchKernelGGL(spline_knots_kernel<K>, dim3(grid_for(segs + 1, 256)), dim3(256), 0, s, col, n, segs,
+                       reinterpret_cast<uint64_t*>(base), reinterpret_cast<float*>(base + r.knot_bytes()));
     GJ_HIP_CHECK(hipGetLastError());
-    unsigned long long hinv = 0;
-    GJ_HIP_CHECK(hipMemcpyAsync(&hinv, inv.p, sizeof(hinv), hipMemcpyDeviceToHost, s));
-    GJ_HIP_CHECK(hipStreamSynchronize(s));
-    GJ_REQUIRE(hinv == 0, GJ_ERR_INTERNAL, "learned index: root model is not monotone");
+    r.knots = reinterpret_cast<const uint64_t*>(base);
+    return r;
+  };
+  // segments of a requested spline root, capped by the keys available
+  auto spline_segments = [&](uint32_t want) {
+    uint32_t segs = 2;
+    while (segs * 2 <= want && static_cast<uint64_t>(segs) * 2 <= n) segs *= 2;
+    return segs;
+  };
+  if (ix.kind == GJ_LEARNED_PLR && ix.params.learned_root_segments >= 2) {
+    const uint32_t want = ix.params.learned_root_segments;
+    GJ_REQUIRE((want & (want - 1)) == 0 && want <= 4096, GJ_ERR_INVALID,
+               "learned_root_segments must be 0, 1 or a power of two <= 4096");
+    root = spline_root(spline_segments(want));
   }
-  ix.lm_leaves.alloc(sizeof(LeafModel) * L);
-  dbuf fit(sizeof(LeafFit) * L);
-  hipLaunchKernelGGL(leaf_fit_kernel<K>, dim3(L), dim3(256), 0, s, col, n, start.as<const uint32_t>(), L,
-                     ix.lm_leaves.as<LeafModel>(), fit.as<LeafFit>());
-  GJ_HIP_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(leaf_error_kernel<K>, dim3(grid_for(n, 256)), dim3(256), 0, s, col, n, route,
-                     ix.lm_leaves.as<const LeafModel>(), fit.as<LeafFit>());
-  GJ_HIP_CHECK(hipGetLastError());
-  const int64_t max_width = 2 * static_cast<int64_t>(ix.params.learned_max_error) + 2;
-  hipLaunchKernelGGL(leaf_finalize_kernel, dim3(grid_for(L, 256)), dim3(256), 0, s, ix.lm_leaves.as<LeafModel>(),
-                     fit.as<LeafFit>(), L, max_width);
-  GJ_HIP_CHECK(hipGetLastError());
 
-  // window bounds over the leaves that are not routed to the B-tree, and statistics
-  std::vector<LeafFit> hf(L);
-  GJ_HIP_CHECK(hipMemcpyAsync(hf.data(), fit.p, fit.bytes, hipMemcpyDeviceToHost, s));
-  GJ_HIP_CHECK(hipStreamSynchronize(s));
-  uint64_t fb_leaves = 0, fb_keys = 0, wsum = 0, wkeys = 0;
-  uint32_t wmax = 0;
-  int32_t elo = 0, ehi = 0;
-  for (const auto& f : hf) {
-    if (f.count == 0) continue;
-    if (f.err_hi < f.err_lo) {
-      ++fb_leaves;
-      fb_keys += f.count;
-    } else {
-      const uint32_t w = static_cast<uint32_t>(f.err_hi - f.err_lo + 1);
-      wmax = std::max(wmax, w);
-      wsum += static_cast<uint64_t>(w) * f.count;
-      wkeys += f.count;
-      elo = std::min(elo, f.err_lo);
-      ehi = std::max(ehi, f.err_hi);
+  // routing, leaf fits and calibration under a root; returns the build keys routed to the B-tree
+  auto build_leaves = [&](const RootModel& root) -> uint64_t {
+    Router<K> route{kmin, root, L};
+    dbuf start((static_cast<uint64_t>(L) + 1 + 3) * sizeof(uint32_t));
+    {
+      // fill start[] with n
+      std::vector<uint32_t> init(L + 1, static_cast<uint32_t>(n));
+      GJ_HIP_CHECK(hipMemcpyAsync(start.p, init.data(), init.size() * 4, hipMemcpyHostToDevice, s));
+      dbuf inv(sizeof(unsigned long long));
+      fill_bytes(inv.p, 0, inv.bytes, s);
+      hipLaunchKernelGGL(leaf_bounds_kernel<K>, dim3(grid_for(n, 256)), dim3(256), 0, s, col, n, route,
+                         start.as<uint32_t>(), inv.as<unsigned long long>());
+      GJ_HIP_CHECK(hipGetLastError());
+      unsigned long long hinv = 0;
+      GJ_HIP_CHECK(hipMemcpyAsync(&hinv, inv.p, sizeof(hinv), hipMemcpyDeviceToHost, s));
+      GJ_HIP_CHECK(hipStreamSynchronize(s));
+      GJ_REQUIRE(hinv == 0, GJ_ERR_INTERNAL, "learned index: root model is not monotone");
     }
+    ix.lm_leaves.alloc(sizeof(LeafModel) * L);
+    dbuf fit(sizeof(LeafFit) * L);
+    hipLaunchKernelGGL(leaf_fit_kernel<K>, dim3(L), dim3(256), 0, s, col, n, start.as<const uint32_t>(), L,
+                       ix.lm_leaves.as<LeafModel>(), fit.as<LeafFit>());
+    GJ_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(leaf_error_kernel<K>, dim3(grid_for(n, 256)), dim3(256), 0, s, col, n, route,
+                       ix.lm_leaves.as<const LeafModel>(), fit.as<LeafFit>());
+    GJ_HIP_CHECK(hipGetLastError());
+    const int64_t max_width = 2 * static_cast<int64_t>(ix.params.learned_max_error) + 2;
+    hipLaunchKernelGGL(leaf_finalize_kernel, dim3(grid_for(L, 256)), dim3(256), 0, s, ix.lm_leaves.as<LeafModel>(),
+                       fit.as<LeafFit>(), L, max_width);
+    GJ_HIP_CHECK(hipGetLastError());
+
+    // window bounds over the leaves that are not routed to the B-tree, and statistics
+    std::vector<LeafFit> hf(L);
+    GJ_HIP_CHECK(hipMemcpyAsync(hf.data(), fit.p, fit.bytes, hipMemcpyDeviceToHost, s));
+    GJ_HIP_CHECK(hipStreamSynchronize(s));
+    uint64_t fb_leaves = 0, fb_keys = 0, wsum = 0, wkeys = 0;
+    uint32_t wmax = 0;
+    int32_t elo = 0, ehi = 0;
+    for (const auto& f : hf) {
+      if (f.count == 0) continue;
+      if (f.err_hi < f.err_lo) {
+        ++fb_leaves;
+        fb_keys += f.count;
+      } else {
+        const uint32_t w = static_cast<uint32_t>(f.err_hi - f.err_lo + 1);
+        wmax = std::max(wmax, w);
+        wsum += static_cast<uint64_t>(w) * f.count;
+        wkeys += f.count;
+        elo = std::min(elo, f.err_lo);
+        ehi = std::max(ehi, f.err_hi);
+      }
+    }
+    ix.lm_err_lo = elo;
+    ix.lm_err_hi = ehi;
+    ix.lm_root = root;
+    ix.stats.root_segments = root.segments;
+    ix.lm_num_leaves = L;
+    ix.lm_fallback_any = fb_leaves ? 1u : 0u;
+    ix.stats.leaves = L;
+    ix.stats.fallback_leaves = fb_leaves;
+    ix.stats.fallback_keys = fb_keys;
+    ix.stats.max_window = wmax;
+    ix.stats.avg_window = wkeys ? static_cast<double>(wsum) / static_cast<double>(wkeys) : 0.0;
+    ix.stats.depth = 2;
+    ix.stats.lds_levels = ix.bt.lds_levels;
+    return fb_keys;
+  };
+  const uint64_t fb = build_leaves(root);
+  if (ix.kind == GJ_LEARNED_PLR && ix.params.learned_root_segments == 0 && fb > 0 && n >= 4) {
+    // the linear root cannot spread these keys evenly over the leaves: route by the
+    // column's own quantiles instead
+    build_leaves(spline_root(spline_segments(2048)));
   }
-  ix.lm_err_lo = elo;
-  ix.lm_err_hi = ehi;
-  ix.lm_root = root;
-  ix.lm_num_leaves = L;
-  ix.lm_fallback_any = fb_leaves ? 1u : 0u;
-  ix.stats.leaves = L;
-  ix.stats.fallback_leaves = fb_leaves;
-  ix.stats.fallback_keys = fb_keys;
-  ix.stats.max_window = wmax;
-  ix.stats.avg_window = wkeys ? static_cast<double>(wsum) / static_cast<double>(wkeys) : 0.0;
-  ix.stats.depth = 2;
-  ix.stats.lds_levels = ix.bt.lds_levels;
 }
 template void build_learned<uint32_t>(gj_index&, hipStream_t);
 template void build_learned<uint64_t>(gj_index&, hipStream_t);

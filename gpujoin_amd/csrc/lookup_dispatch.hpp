@@ -92,7 +92,7 @@ void with_index_dev(const gj_index& ix, F&& f) {
           break;
       }
       throw error(GJ_ERR_INVALID, "unsupported harmonia_group_lanes");
-    case GJ_RADIX_SPLINE: f(ix.rs_dev<K, 8>()); return;
+    case GJ_RADIX_SPLINE: f(ix.rs_dev<K>()); return;
     case GJ_LEARNED_PLR:
     case GJ_LEARNED_MLP:
       if (ix.lm_fallback_any)

@@ -80,7 +80,7 @@ typedef enum gj_approach {
 typedef struct gj_index_params {
   uint32_t rs_radix_bits;        /* RadixSpline radix bits, default 18 */
   uint32_t rs_max_error;         /* RadixSpline error bound, default 32 */
-  uint32_t harmonia_group_lanes; /* lanes per Harmonia node search (8, 16, 32, 64), default 16 */
+  uint32_t harmonia_group_lanes; /* lanes per Harmonia node search (8, 16, or 32 for u32 keys), default 8 */
   uint32_t btree_lds_bytes;      /* LDS budget for resident upper B-tree levels, default 16384 */
   uint32_t learned_leaf_bits;    /* log2 of the number of linear leaves, default 16 */
   uint32_t learned_max_error;    /* last-mile bound; leaves with a wider measured window use the B-tree, default 64 */
@@ -89,6 +89,9 @@ typedef struct gj_index_params {
   uint32_t mlp_batch;            /* training minibatch (keys per step), default 65536 */
   float mlp_lr;                  /* Adam learning rate, default 3e-3 */
   uint64_t seed;                 /* training sample / initialisation seed, default 0x5eed */
+  uint32_t learned_root_segments; /* learned_plr root: 0 = automatic (linear; an equi-depth spline of 2048
+                                     segments when the linear root routes keys to the B-tree), 1 = linear,
+                                     otherwise the spline's segment count (a power of two, <= 4096) */
 } gj_index_params;
 
 void gj_index_params_default(gj_index_params* p);
@@ -153,6 +156,7 @@ typedef struct gj_index_stats {
   uint32_t max_window;        /* widest last-mile window over non-fallback leaves (learned), 2*err+2 (RadixSpline) */
   double avg_window;          /* mean last-mile window over build keys (learned) */
   float train_loss;           /* final MLP training loss (learned_mlp) */
+  uint32_t root_segments;     /* learned_plr: segments of the spline root (0: linear root) */
 } gj_index_stats;
 
 int gj_index_get_stats(const gj_index* index, gj_index_stats* out);

@@ -129,12 +129,57 @@ def test_mlp_root_beats_linear_root_on_skewed_keys(gj, gpu):
     keys = gj.gen_build_keys(1 << 20, 8, _lib.KEYS_LOGNORMAL, 63, 0x31)
     params = gj.index_params(learned_leaf_bits=12, learned_max_error=32)
     col = gj.DeviceArray.from_numpy(keys)
-    plr = gj.Index("learned_plr", col, host_column=keys, params=params).stats()
+    linear = gj.index_params(learned_leaf_bits=12, learned_max_error=32, learned_root_segments=1)
+    plr = gj.Index("learned_plr", col, host_column=keys, params=linear).stats()
+    assert plr["root_segments"] == 0
     mlp = gj.Index("learned_mlp", col, host_column=keys, params=params).stats()
     print(f"[gain] plr fallback_keys={plr['fallback_keys']} avg_window={plr['avg_window']:.1f}; "
           f"mlp fallback_keys={mlp['fallback_keys']} avg_window={mlp['avg_window']:.1f} loss={mlp['train_loss']:.2e}",
           flush=True)
     assert mlp["fallback_keys"] < plr["fallback_keys"]
+
+
+@pytest.mark.parametrize("segments", [0, 2, 64, 2048, 4096])
+@pytest.mark.parametrize("dist", [_lib.KEYS_UNIFORM, _lib.KEYS_LOGNORMAL, _lib.KEYS_CLUSTERED])
+@pytest.mark.parametrize("kb", [4, 8])
+def test_learned_spline_root_matches_oracle(gj, orc, gpu, segments, dist, kb):
+    n = 1 << 18
+    print(f"[case] spline root segments={segments} dist={dist} kb={kb}", flush=True)
+    keys = gj.gen_build_keys(n, kb, dist, 0, 700 + dist)
+    probes = _probes(gj, keys, 40000, 800 + dist)
+    params = gj.index_params(learned_leaf_bits=10, learned_max_error=16, learned_root_segments=segments)
+    got, st = _lookup(gj, "learned_plr", keys, probes, params=params)
+    assert np.array_equal(got, orc.lookup_lower_bound(keys, probes))
+    if segments >= 2:
+        assert st["root_segments"] == segments
+
+
+def test_learned_spline_root_tiny_columns(gj, orc, gpu):
+    # fewer keys than requested segments: the root keeps at most n segments
+    for n in (1, 2, 3, 5, 64, 100):
+        keys = np.unique(np.random.default_rng(n).integers(1, 2 ** 40, size=n, dtype=np.uint64))
+        probes = np.concatenate([keys, keys + 1, keys - 1])
+        params = gj.index_params(learned_leaf_bits=4, learned_root_segments=4096)
+        got, st = _lookup(gj, "learned_plr", keys, probes, params=params)
+        assert np.array_equal(got, orc.lookup_lower_bound(keys, probes)), n
+        assert st["root_segments"] <= max(2, keys.size)
+
+
+def test_spline_root_spreads_clustered_keys(gj, gpu):
+    # 64 narrow clusters: a linear root sends each cluster to one leaf, so every key
+    # falls back to the B-tree; the automatic choice switches to the quantile
+    # spline root, which spreads the keys evenly over the leaves
+    keys = gj.gen_build_keys(1 << 20, 8, _lib.KEYS_CLUSTERED, 63, 0x33)
+    col = gj.DeviceArray.from_numpy(keys)
+    lin = gj.Index("learned_plr", col, host_column=keys,
+                   params=gj.index_params(learned_leaf_bits=12, learned_max_error=32, learned_root_segments=1)).stats()
+    auto = gj.Index("learned_plr", col, host_column=keys,
+                    params=gj.index_params(learned_leaf_bits=12, learned_max_error=32)).stats()
+    print(f"[gain] linear root fallback_keys={lin['fallback_keys']}; auto root segments={auto['root_segments']} "
+          f"fallback_keys={auto['fallback_keys']} avg_window={auto['avg_window']:.1f}", flush=True)
+    assert lin["root_segments"] == 0 and lin["fallback_keys"] > keys.size // 2
+    assert auto["root_segments"] == 2048
+    assert auto["fallback_keys"] * 10 < lin["fallback_keys"]
 
 
 def test_mlp_gradients_match_oracle(gj, orc, gpu):
