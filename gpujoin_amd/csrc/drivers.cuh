@@ -3,6 +3,7 @@
 
 #include "device.cuh"
 #include "indexes.cuh"
+#include "indexes_ext.cuh"
 
 namespace gj {
 
@@ -25,6 +26,44 @@ __global__ __launch_bounds__(256) void lookup_plain_kernel(Dev d, const K* __res
     const K x = active ? load_stream(probes + i) : K(0);
     const uint32_t r = d.lookup(x, active, lds);
     if (active) store_stream(out + i, r);
+  }
+}
+
+// Second pass of the plain driver for learned models with a few leaves routed
+// to the B-tree: every probe whose result holds kFallbackMark is looked up again
+// with index d.  A wave scans kMarkSlots x 64 results per step (independent,
+// coalesced loads) and runs a cooperative lookup only for the slots holding a mark.
+constexpr int kMarkSlots = 16;
+
+template <class Dev, class K>
+__global__ __launch_bounds__(256) void lookup_marked_kernel(Dev d, const K* __restrict__ probes, uint64_t m,
+                                                            uint32_t* __restrict__ out) {
+  extern __shared__ __attribute__((aligned(16))) uint8_t lds[];
+  d.stage(lds);
+  __syncthreads();
+  const uint64_t waves = (static_cast<uint64_t>(gridDim.x) * blockDim.x) / kWave;
+  const uint64_t wave = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+  const uint64_t span = static_cast<uint64_t>(kMarkSlots) * kWave;
+  for (uint64_t base = wave * span; base < m; base += waves * span) {
+    uint32_t v[kMarkSlots];
+#pragma unroll
+    for (int k = 0; k < kMarkSlots; ++k) {
+      const uint64_t i = base + static_cast<uint64_t>(k * kWave + lane());
+      v[k] = i < m ? out[i] : 0u;
+    }
+    uint32_t mk = 0;  // bit k: slot k holds a mark
+#pragma unroll
+    for (int k = 0; k < kMarkSlots; ++k) mk |= (v[k] == kFallbackMark ? 1u : 0u) << k;
+    if (ballot(mk != 0) == 0) continue;  // wave-uniform
+#pragma unroll 1
+    for (int k = 0; k < kMarkSlots; ++k) {
+      const bool marked = (mk >> k) & 1u;
+      if (ballot(marked) == 0) continue;  // wave-uniform
+      const uint64_t i = base + static_cast<uint64_t>(k * kWave + lane());
+      const K x = marked ? probes[i] : K(0);
+      const uint32_t r = d.lookup(x, marked, lds);
+      if (marked) out[i] = r;
+    }
   }
 }
 

@@ -14,6 +14,12 @@ namespace {
 // ignore_bits = floor(log2(max key)) - radix_bits (src/index_lookup_new.cu:63-67),
 // which leaves the top significant bit out of the digit; using the bit width
 // keeps the digit on the top radix_bits significant bits.
+// Below one fallback key in this many build keys, the plain driver marks and
+// re-looks-up fallback probes; above, it runs the B-tree inline.  A wave with a
+// mark costs one B-tree lookup, so marking wins while fewer than about one wave
+// in six (64 / 400) holds a fallback probe.
+constexpr uint64_t kMarkFallbackRatio = 400;
+
 uint32_t key_bit_width(uint64_t max_key) { return max_key ? 64u - static_cast<uint32_t>(__builtin_clzll(max_key)) : 0u; }
 
 // partitioning_approach (src/index_lookup_partitioning.cu:377-513): the probes
@@ -85,7 +91,18 @@ void lookup_dispatch(const gj_index& ix, const void* probes, uint64_t m, uint32_
   dispatch_key(ix.key_bytes, [&]<class K>() {
     const K* pk = static_cast<const K*>(probes);
     switch (p.approach) {
-      case GJ_PLAIN: with_index_dev<K>(ix, [&](auto d) { launch_plain(d, pk, m, out, p, s, ix.cu); }); return;
+      case GJ_PLAIN:
+        if ((ix.kind == GJ_LEARNED_PLR || ix.kind == GJ_LEARNED_MLP) && ix.lm_fallback_any &&
+            ix.stats.fallback_keys * kMarkFallbackRatio <= ix.n) {
+          // few keys in leaves routed to the B-tree: the learned kernel marks their
+          // probes, then one B-tree pass over the marks (the hot kernel stays free of
+          // the B-tree's registers); with more, the B-tree runs inline
+          launch_plain(ix.learned_dev<K, kLeafMark>(), pk, m, out, p, s, ix.cu);
+          launch_marked(ix.btree_dev<K>(), pk, m, out, p, s, ix.cu);
+          return;
+        }
+        with_index_dev<K>(ix, [&](auto d) { launch_plain(d, pk, m, out, p, s, ix.cu); });
+        return;
       case GJ_BWS: with_index_dev<K>(ix, [&](auto d) { launch_bws(d, pk, m, out, p, s, ix.cu); }); return;
       case GJ_PARTITIONING: lookup_partitioned<K>(ix, pk, m, out, p, s); return;
       case GJ_HJ: lookup_hash_join<K>(ix, pk, m, out, p, s); return;

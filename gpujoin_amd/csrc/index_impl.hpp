@@ -108,9 +108,9 @@ struct gj_index {
     return d;
   }
 
-  template <class K, bool FB>
-  gj::LearnedDev<K, FB> learned_dev() const {
-    gj::LearnedDev<K, FB> d{};
+  template <class K, int Mode>
+  gj::LearnedDev<K, Mode> learned_dev() const {
+    gj::LearnedDev<K, Mode> d{};
     d.col = static_cast<const K*>(d_col);
     d.n = n;
     d.min_key = static_cast<K>(min_key);

@@ -350,8 +350,8 @@ struct RadixSplineDev {
 // Learned two-stage position model (no reference counterpart).
 //   root:  cdf = root(x), x = (key - min_key) / (max_key - min_key) in FP32;
 //          linear (a*x + b), a monotone 1xH MLP with ramp units
-//          clamp(w1*x + b1, 0, 1) (w1, w2 >= 0), or an equi-depth spline over
-//          the column's quantile keys (any key distribution); leaf = floor(cdf * L)
+//          clamp(w1*x + b1, 0, 1) (w1, w2 >= 0), or an error-bounded spline of
+//          the column's CDF (any key distribution); leaf = floor(cdf * L)
 //   leaf:  pred = start + floor(slope * dkey) in FP32, dkey = key - leaf base key
 //          (the fit's intercept is folded into start); one 16-byte load.
 //   window: [pred + err_lo, pred + err_hi], with err_lo/err_hi the extremes of
@@ -376,29 +376,35 @@ struct RootModel {
   float inv_range;            // 1 / (max_key - min_key) in FP32
   uint32_t hidden;            // > 0: MLP root
   const float* mlp;           // [w1(h), b1(h), w2(h), b2] in global memory (staged in LDS)
-  uint32_t segments;          // > 0: equi-depth spline root (a power of two)
-  const uint64_t* knots;      // segments + 1 keys: knot s = key at position s * n / segments, last = max key;
-                              // then `segments` FP32 inverse knot gaps (staged in LDS)
+  uint32_t segments;          // > 0: spline root with this many segments (a power of two; the
+                              // knots past the last real one repeat it)
+  const uint64_t* knots;      // spline arrays in global memory (staged in LDS):
+                              // key[segments + 1] u64 | cdf[segments + 1] f32 | slope[segments] f32
 
-  __host__ __device__ size_t knot_bytes() const { return (static_cast<size_t>(segments) + 1) * 8 + 8; }
+  __host__ __device__ size_t key_bytes() const { return ((static_cast<size_t>(segments) + 1) * 8 + 15) / 16 * 16; }
+  __host__ __device__ size_t cdf_bytes() const { return ((static_cast<size_t>(segments) + 1) * 4 + 15) / 16 * 16; }
   __host__ __device__ size_t bytes() const {
     if (hidden) return ((3 * hidden + 1) * 4 + 15) / 16 * 16;
-    if (segments) return (knot_bytes() + static_cast<size_t>(segments) * 4 + 15) / 16 * 16;
+    if (segments) return key_bytes() + cdf_bytes() + (static_cast<size_t>(segments) * 4 + 15) / 16 * 16;
     return 0;
   }
   __host__ __device__ const void* params() const { return hidden ? static_cast<const void*>(mlp) : knots; }
 };
 
-// Equi-depth piecewise-linear CDF: the segment s with knot[s] <= key < knot[s + 1]
-// by a branch-free binary search, then linear inside it.  Monotone in the key,
-// so every leaf still owns a contiguous key range.
-__device__ __forceinline__ float spline_cdf(uint64_t key, const uint64_t* knot, const float* inv_gap,
-                                            uint32_t segments) {
+// Spline root: the segment s with key[s] <= x < key[s + 1] by a branch-free
+// binary search, then cdf = cdf[s] + (x - key[s]) * slope[s], kept inside
+// [cdf[s], cdf[s + 1]].  Monotone in the key, so every leaf still owns a
+// contiguous key range.  The knots come from an error-bounded greedy spline over
+// (key, position / n), so a segment never spans a gap that the keys jump across.
+__device__ __forceinline__ float spline_cdf(uint64_t x, const uint8_t* base, const RootModel& r) {
+  const uint64_t* key = reinterpret_cast<const uint64_t*>(base);
+  const float* cdf = reinterpret_cast<const float*>(base + r.key_bytes());
+  const float* slope = reinterpret_cast<const float*>(base + r.key_bytes() + r.cdf_bytes());
   uint32_t s = 0;
-  for (uint32_t step = segments >> 1; step > 0; step >>= 1) s = (knot[s + step] <= key) ? s + step : s;
-  float f = key >= knot[s] ? static_cast<float>(key - knot[s]) * inv_gap[s] : 0.0f;
-  f = fminf(f, 1.0f);
-  return (static_cast<float>(s) + f) * (1.0f / static_cast<float>(segments));
+  for (uint32_t step = r.segments >> 1; step > 0; step >>= 1) s = (key[s + step] <= x) ? s + step : s;
+  const float d = x >= key[s] ? static_cast<float>(x - key[s]) : 0.0f;
+  const float c = __builtin_fmaf(d, slope[s], cdf[s]);
+  return fminf(fmaxf(c, cdf[s]), cdf[s + 1]);
 }
 
 __device__ __forceinline__ float mlp_eval(const float* p, uint32_t h, float x) {
@@ -435,10 +441,20 @@ __device__ __forceinline__ int64_t leaf_pred(const LeafModel& m, float dkey) {
   return static_cast<int64_t>(m.start) + static_cast<int64_t>(floorf(p));
 }
 
-// FB: some leaf is routed to the B-tree.  The common case (no such leaf)
-// compiles without the B-tree code, so the kernel keeps fewer registers.
-template <class K, bool FB>
+// How probes of leaves routed to the B-tree are answered:
+//   kLeafNoFallback  no such leaf exists; the kernel carries no B-tree code
+//   kLeafInline      the B-tree lookup runs inside the same kernel
+//   kLeafMark        the probe's result is kFallbackMark; the plain driver defers
+//                    such probes into a list that a second kernel answers with the
+//                    B-tree, so the hot kernel keeps the no-fallback register count
+constexpr int kLeafNoFallback = 0;
+constexpr int kLeafInline = 1;
+constexpr int kLeafMark = 2;
+constexpr uint32_t kFallbackMark = 0xFFFFFFFEu;  // never a row id: n < 2^32 - 1
+
+template <class K, int Mode>
 struct LearnedDev {
+  static constexpr bool FB = Mode == kLeafInline;
   const K* col;
   uint64_t n;
   K min_key, max_key;
@@ -459,8 +475,7 @@ struct LearnedDev {
     const uint8_t* rp = lds + root_offset();
     float cdf;
     if (root.segments) {
-      cdf = spline_cdf(static_cast<uint64_t>(x), reinterpret_cast<const uint64_t*>(rp),
-                       reinterpret_cast<const float*>(rp + root.knot_bytes()), root.segments);
+      cdf = spline_cdf(static_cast<uint64_t>(x), rp, root);
     } else if (root.hidden) {
       cdf = mlp_eval(reinterpret_cast<const float*>(rp), root.hidden, key_unit(x, min_key, root.inv_range));
     } else {
@@ -475,7 +490,7 @@ struct LearnedDev {
     bool live = active && x >= min_key && x <= max_key;
     if (live) {
       const LeafModel m = leaves[route(x, lds)];
-      if (FB && __builtin_isnan(m.slope)) {  // fallback leaf
+      if (Mode != kLeafNoFallback && __builtin_isnan(m.slope)) {  // fallback leaf
         use_tree = true;
       } else {
         const int64_t p = leaf_pred(m, leaf_dkey(x, m.base_key));
@@ -491,11 +506,13 @@ struct LearnedDev {
     }
     uint32_t r = group_window_search<K>(col, n, static_cast<uint32_t>(pred), static_cast<uint32_t>(lo),
                                         static_cast<uint32_t>(hi), x, live && !use_tree);
-    if constexpr (FB) {
+    if constexpr (Mode == kLeafInline) {
       if (ballot(use_tree) != 0) {  // wave-uniform
         const uint32_t t = btree.lookup(x, use_tree, lds);
         if (use_tree) r = t;
       }
+    } else if constexpr (Mode == kLeafMark) {
+      if (use_tree) r = kFallbackMark;
     }
     return r;
   }

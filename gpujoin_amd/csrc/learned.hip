@@ -3,7 +3,8 @@
 // search as the reference's RadixSpline, src/indexes.cuh:205-215).
 //
 // Pipeline, all on the GPU over the (key, position) pairs of the sorted column:
-//   1. root: linear least squares of cdf = i / n on x = unit(key)  (reduction kernel), or
+//   1. root: linear least squares of cdf = i / n on x = unit(key)  (reduction kernel),
+//            an error-bounded greedy spline of the CDF (learned_plr on skewed keys), or
 //            a 1xH MLP f(x) = b2 + sum_k w2_k clamp(w1_k x + b1_k, 0, 1) whose ramps start
 //            at the column's quantiles (the piecewise-linear CDF through h + 1 quantiles) and
 //            are then trained with minibatch Adam by hand-written forward/backward kernels;
@@ -22,6 +23,7 @@
 
 #include "common.hpp"
 #include "lookup_dispatch.hpp"
+#include "rs_builder.hpp"
 
 namespace gj {
 
@@ -222,9 +224,7 @@ struct Router {
   __device__ __forceinline__ uint32_t operator()(K key) const {
     float cdf;
     if (root.segments) {
-      cdf = spline_cdf(static_cast<uint64_t>(key), root.knots,
-                       reinterpret_cast<const float*>(reinterpret_cast<const uint8_t*>(root.knots) + root.knot_bytes()),
-                       root.segments);
+      cdf = spline_cdf(static_cast<uint64_t>(key), reinterpret_cast<const uint8_t*>(root.knots), root);
     } else {
       const float u = key_unit(key, min_key, root.inv_range);
       cdf = root.hidden ? mlp_eval(root.mlp, root.hidden, u) : __builtin_fmaf(root.a, u, root.b);
@@ -232,23 +232,6 @@ struct Router {
     return leaf_of(cdf, leaves);
   }
 };
-
-// Spline root: knot s = key at position floor(s * n / S), knot S = the max key;
-// inverse gaps in FP32 (0 for an empty gap).
-template <class K>
-__global__ void spline_knots_kernel(const K* __restrict__ col, uint64_t n, uint32_t segs, uint64_t* __restrict__ knot,
-                                    float* __restrict__ inv_gap) {
-  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s > segs) return;
-  const uint64_t i = (s == segs) ? n - 1 : static_cast<uint64_t>((static_cast<unsigned __int128>(s) * n) / segs);
-  const uint64_t k = static_cast<uint64_t>(col[i]);
-  knot[s] = k;
-  if (s < segs) {
-    const uint64_t j = (s + 1 == segs) ? n - 1 : static_cast<uint64_t>((static_cast<unsigned __int128>(s + 1) * n) / segs);
-    const uint64_t k1 = static_cast<uint64_t>(col[j]);
-    inv_gap[s] = k1 > k ? static_cast<float>(1.0 / static_cast<double>(k1 - k)) : 0.0f;
-  }
-}
 
 // start[l] = first position routed to a leaf >= l (start[] pre-filled with n); counts
 // routing inversions (which must not occur: the root is monotone).
@@ -453,31 +436,57 @@ void build_learned(gj_index& ix, hipStream_t s) {
     root.mlp = ix.lm_mlp.as<const float>();
   }
 
-  // spline root over `segs` equi-depth segments (stored in ix.lm_knots)
-  auto spline_root = [&](uint32_t segs) {
+  // Spline root: GreedySplineCorridor knots over (key, position) with the error bound
+  // doubled until at most `max_segs` segments remain; stored as (key, cdf = pos / n,
+  // slope) with the segment count padded to a power of two by repeating the last knot.
+  auto spline_root = [&](uint32_t max_segs) {
+    std::vector<uint8_t> hcopy;
+    const void* hk = ix.h_col;
+    if (!hk) {
+      hcopy.resize(n * sizeof(K));
+      GJ_HIP_CHECK(hipMemcpyAsync(hcopy.data(), col, hcopy.size(), hipMemcpyDeviceToHost, s));
+      GJ_HIP_CHECK(hipStreamSynchronize(s));
+      hk = hcopy.data();
+    }
+    uint64_t err = std::max<uint64_t>(16, n / (8ull * max_segs));
+    RadixSplineHost sp;
+    for (;;) {
+      sp = build_radix_spline(hk, n, sizeof(K), 1, static_cast<uint32_t>(std::min<uint64_t>(err, 0x7FFFFFFFull)));
+      if (sp.knot_x.size() <= static_cast<size_t>(max_segs) + 1 || err >= n) break;
+      err *= 2;
+    }
+    const uint32_t knots = static_cast<uint32_t>(sp.knot_x.size());
+    uint32_t segs = 1;
+    while (segs + 1 < knots) segs *= 2;
     RootModel r{};
     r.inv_range = root.inv_range;
     r.segments = segs;
-    ix.lm_knots.alloc(r.bytes());
-    fill_bytes(ix.lm_knots.p, 0, ix.lm_knots.bytes, s);
-    uint8_t* base = ix.lm_knots.as<uint8_t>();
-    hipLaunchKernelGGL(spline_knots_kernel<K>, dim3(grid_for(segs + 1, 256)), dim3(256), 0, s, col, n, segs,
-                       reinterpret_cast<uint64_t*>(base), reinterpret_cast<float*>(base + r.knot_bytes()));
-    GJ_HIP_CHECK(hipGetLastError());
-    r.knots = reinterpret_cast<const uint64_t*>(base);
+    std::vector<uint8_t> h(r.bytes(), 0);
+    auto* key = reinterpret_cast<uint64_t*>(h.data());
+    auto* cdf = reinterpret_cast<float*>(h.data() + r.key_bytes());
+    auto* slope = reinterpret_cast<float*>(h.data() + r.key_bytes() + r.cdf_bytes());
+    const double inv_n = 1.0 / static_cast<double>(n);
+    for (uint32_t i = 0; i <= segs; ++i) {
+      const uint32_t j = std::min(i, knots - 1);
+      key[i] = sp.knot_x[j];
+      cdf[i] = static_cast<float>(sp.knot_y[j] * inv_n);
+    }
+    for (uint32_t i = 0; i < segs; ++i) {
+      const uint64_t dx = key[i + 1] - key[i];
+      slope[i] = dx ? static_cast<float>((static_cast<double>(cdf[i + 1]) - cdf[i]) / static_cast<double>(dx)) : 0.0f;
+    }
+    ix.lm_knots.alloc(h.size());
+    GJ_HIP_CHECK(hipMemcpyAsync(ix.lm_knots.p, h.data(), h.size(), hipMemcpyHostToDevice, s));
+    GJ_HIP_CHECK(hipStreamSynchronize(s));
+    r.knots = ix.lm_knots.as<const uint64_t>();
+    ix.stats.spline_points = knots;
     return r;
-  };
-  // segments of a requested spline root, capped by the keys available
-  auto spline_segments = [&](uint32_t want) {
-    uint32_t segs = 2;
-    while (segs * 2 <= want && static_cast<uint64_t>(segs) * 2 <= n) segs *= 2;
-    return segs;
   };
   if (ix.kind == GJ_LEARNED_PLR && ix.params.learned_root_segments >= 2) {
     const uint32_t want = ix.params.learned_root_segments;
     GJ_REQUIRE((want & (want - 1)) == 0 && want <= 4096, GJ_ERR_INVALID,
                "learned_root_segments must be 0, 1 or a power of two <= 4096");
-    root = spline_root(spline_segments(want));
+    root = spline_root(want);
   }
 
   // routing, leaf fits and calibration under a root; returns the build keys routed to the B-tree
@@ -549,9 +558,9 @@ void build_learned(gj_index& ix, hipStream_t s) {
   };
   const uint64_t fb = build_leaves(root);
   if (ix.kind == GJ_LEARNED_PLR && ix.params.learned_root_segments == 0 && fb > 0 && n >= 4) {
-    // the linear root cannot spread these keys evenly over the leaves: route by the
-    // column's own quantiles instead
-    build_leaves(spline_root(spline_segments(2048)));
+    // the linear root cannot spread these keys evenly over the leaves: route by a
+    // spline of the column's own CDF instead
+    build_leaves(spline_root(2048));
   }
 }
 template void build_learned<uint32_t>(gj_index&, hipStream_t);

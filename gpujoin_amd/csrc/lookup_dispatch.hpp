@@ -35,6 +35,23 @@ void launch_plain(const Dev& d, const K* probes, uint64_t m, uint32_t* out, cons
   GJ_HIP_CHECK(hipGetLastError());
 }
 
+// Second pass of the plain driver for learned leaves routed to the B-tree: the
+// B-tree answers every probe whose result is kFallbackMark.
+template <class K>
+void launch_marked(const BTreeDev<K>& d, const K* probes, uint64_t m, uint32_t* out, const gj_lookup_params& p,
+                   hipStream_t s, int cu) {
+  if (m == 0) return;
+  const unsigned block = lookup_block(p);
+  const size_t lds = d.lds_bytes();
+  const void* fn = reinterpret_cast<const void*>(&lookup_marked_kernel<BTreeDev<K>, K>);
+  uint64_t blocks = static_cast<uint64_t>(blocks_per_cu(fn, block, lds)) * cu;
+  blocks = std::min<uint64_t>(blocks, ceil_div(m, static_cast<uint64_t>(block) * kMarkSlots));
+  blocks = std::max<uint64_t>(blocks, 1);
+  hipLaunchKernelGGL((lookup_marked_kernel<BTreeDev<K>, K>), dim3(static_cast<unsigned>(blocks)), dim3(block), lds, s,
+                     d, probes, m, out);
+  GJ_HIP_CHECK(hipGetLastError());
+}
+
 // total_max bounds the tuple count; d_total (optional) holds the exact count on the device.
 template <class Dev, class K>
 void launch_tuples(const Dev& d, const K* keys, const uint32_t* payload, const uint64_t* d_total, uint64_t total_max,
@@ -96,9 +113,9 @@ void with_index_dev(const gj_index& ix, F&& f) {
     case GJ_LEARNED_PLR:
     case GJ_LEARNED_MLP:
       if (ix.lm_fallback_any)
-        f(ix.learned_dev<K, true>());
+        f(ix.learned_dev<K, kLeafInline>());
       else
-        f(ix.learned_dev<K, false>());
+        f(ix.learned_dev<K, kLeafNoFallback>());
       return;
     case GJ_HASH: f(ix.hash_dev<K>()); return;
   }

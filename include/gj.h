@@ -89,9 +89,9 @@ typedef struct gj_index_params {
   uint32_t mlp_batch;            /* training minibatch (keys per step), default 65536 */
   float mlp_lr;                  /* Adam learning rate, default 3e-3 */
   uint64_t seed;                 /* training sample / initialisation seed, default 0x5eed */
-  uint32_t learned_root_segments; /* learned_plr root: 0 = automatic (linear; an equi-depth spline of 2048
+  uint32_t learned_root_segments; /* learned_plr root: 0 = automatic (linear; a CDF spline of <= 2048
                                      segments when the linear root routes keys to the B-tree), 1 = linear,
-                                     otherwise the spline's segment count (a power of two, <= 4096) */
+                                     otherwise the most spline segments (a power of two, <= 4096) */
 } gj_index_params;
 
 void gj_index_params_default(gj_index_params* p);
@@ -149,7 +149,7 @@ typedef struct gj_index_stats {
   uint64_t memory_bytes;
   uint32_t depth;             /* tree levels (B-tree inner levels, Harmonia levels) */
   uint32_t lds_levels;        /* upper levels resident in LDS during lookups */
-  uint64_t spline_points;     /* RadixSpline knots */
+  uint64_t spline_points;     /* RadixSpline knots; learned_plr spline-root knots */
   uint64_t leaves;            /* learned leaves */
   uint64_t fallback_leaves;   /* learned leaves routed to the B-tree */
   uint64_t fallback_keys;     /* build keys whose leaf is routed to the B-tree */

@@ -122,6 +122,23 @@ def test_learned_fallback_is_exact(gj, orc, gpu):
     assert np.array_equal(got, orc.lookup_lower_bound(keys, probes))
 
 
+@pytest.mark.parametrize("kind", ["learned_plr", "learned_mlp"])
+def test_learned_few_fallback_leaves_exact(gj, orc, gpu, kind):
+    # uniform keys plus one run of 600 consecutive integers: only the leaves over the
+    # run exceed the error bound, so fewer than 1 key in 400 falls back and the plain
+    # driver marks those probes and re-looks them up with the B-tree in a second pass
+    rng = np.random.default_rng(91)
+    base = np.unique(rng.integers(0, 2 ** 62, size=1 << 20, dtype=np.uint64))
+    run = np.arange(2 ** 61, 2 ** 61 + 600, dtype=np.uint64)
+    keys = np.unique(np.concatenate([base, run]))
+    probes = np.concatenate([_probes(gj, keys, 200000, 92), run, run + 600])
+    params = gj.index_params(learned_leaf_bits=14, learned_max_error=16, learned_root_segments=1, mlp_steps=100)
+    got, st = _lookup(gj, kind, keys, probes, params=params)
+    print(f"[case] {kind} fallback_keys={st['fallback_keys']} of {keys.size}", flush=True)
+    assert 0 < st["fallback_keys"] and st["fallback_keys"] * 400 <= keys.size
+    assert np.array_equal(got, orc.lookup_lower_bound(keys, probes))
+
+
 def test_mlp_root_beats_linear_root_on_skewed_keys(gj, gpu):
     # what the trained network gains: on lognormal keys the linear root crowds most
     # keys into a few leaves; the trained monotone MLP root spreads them, so far fewer
@@ -151,7 +168,7 @@ def test_learned_spline_root_matches_oracle(gj, orc, gpu, segments, dist, kb):
     got, st = _lookup(gj, "learned_plr", keys, probes, params=params)
     assert np.array_equal(got, orc.lookup_lower_bound(keys, probes))
     if segments >= 2:
-        assert st["root_segments"] == segments
+        assert 1 <= st["root_segments"] <= segments
 
 
 def test_learned_spline_root_tiny_columns(gj, orc, gpu):
@@ -167,8 +184,9 @@ def test_learned_spline_root_tiny_columns(gj, orc, gpu):
 
 def test_spline_root_spreads_clustered_keys(gj, gpu):
     # 64 narrow clusters: a linear root sends each cluster to one leaf, so every key
-    # falls back to the B-tree; the automatic choice switches to the quantile
-    # spline root, which spreads the keys evenly over the leaves
+    # falls back to the B-tree; the automatic choice switches to a spline of the
+    # column's CDF, which spreads the keys evenly over the leaves (only leaves in the
+    # clusters' sparse tails stay too wide for a line)
     keys = gj.gen_build_keys(1 << 20, 8, _lib.KEYS_CLUSTERED, 63, 0x33)
     col = gj.DeviceArray.from_numpy(keys)
     lin = gj.Index("learned_plr", col, host_column=keys,
@@ -178,8 +196,8 @@ def test_spline_root_spreads_clustered_keys(gj, gpu):
     print(f"[gain] linear root fallback_keys={lin['fallback_keys']}; auto root segments={auto['root_segments']} "
           f"fallback_keys={auto['fallback_keys']} avg_window={auto['avg_window']:.1f}", flush=True)
     assert lin["root_segments"] == 0 and lin["fallback_keys"] > keys.size // 2
-    assert auto["root_segments"] == 2048
-    assert auto["fallback_keys"] * 10 < lin["fallback_keys"]
+    assert 1 <= auto["root_segments"] <= 2048
+    assert auto["fallback_keys"] * 20 < lin["fallback_keys"]
 
 
 def test_mlp_gradients_match_oracle(gj, orc, gpu):
