@@ -23,7 +23,8 @@
 namespace gj {
 
 constexpr int kAggBlock = 256;
-constexpr int kAggQueue = 128;              // queued qualifying rows per wave
+constexpr int kAggUnroll = 8;                          // 64-row slices per wave step
+constexpr int kAggQueue = kWave * (kAggUnroll + 1);    // queued qualifying rows per wave
 constexpr uint64_t kAggTileRows = 8192;     // fact rows per persistent work item
 constexpr uint64_t kLookupTileRows = 16384; // probes per persistent work item
 
@@ -79,7 +80,11 @@ __device__ __forceinline__ void agg_drain(const Dev& d, const uint8_t* lds, AggQ
   q.count -= take;
 }
 
-// Rows [r0, r1) of one fact batch, 64 consecutive rows per wave step.
+// Rows of one fact batch: the wave takes kAggUnroll x 64 rows per step (row
+// base + k * 64 + lane), starting at r0 and advancing by `step`, below r1.  All
+// date loads of a step are issued together, then the fkey / price / discount
+// loads of every qualifying row, so a step costs two memory latencies however
+// many slices qualify; qualifying rows queue in LDS and are looked up 64 at a time.
 template <class Dev>
 __device__ __forceinline__ void agg_rows(const Dev& d, const uint8_t* lds, AggQueue& q, const FactBatchDev& b,
                                          uint64_t r0, uint64_t r1, uint64_t step, int32_t lo, int32_t hi,
@@ -87,22 +92,44 @@ __device__ __forceinline__ void agg_rows(const Dev& d, const uint8_t* lds, AggQu
   const int ln = lane();
   const uint64_t lt = (ln == 0) ? 0ull : (~0ull >> (64 - ln));
   for (uint64_t base = r0; base < r1; base += step) {
-    const uint64_t i = base + static_cast<uint64_t>(ln);
-    bool qual = false;
-    if (i < r1) {
-      const int32_t dt = load_stream(b.date + i);
-      qual = dt >= lo && dt < hi;
+    int32_t dt[kAggUnroll];
+#pragma unroll
+    for (int k = 0; k < kAggUnroll; ++k) {
+      const uint64_t i = base + static_cast<uint64_t>(k * kWave + ln);
+      dt[k] = i < r1 ? load_stream(b.date + i) : 0;
     }
-    const uint64_t mask = ballot(qual);
-    if (mask == 0) continue;
-    if (qual) {
-      const int pos = q.count + __popcll(mask & lt);
-      q.key[pos] = static_cast<uint32_t>(b.fkey[i]);
-      q.val[pos] = b.price[i] * (100 - static_cast<int64_t>(b.disc[i]));
+    uint32_t qm = 0;  // bit k: row of slice k qualifies
+#pragma unroll
+    for (int k = 0; k < kAggUnroll; ++k) {
+      const uint64_t i = base + static_cast<uint64_t>(k * kWave + ln);
+      qm |= ((i < r1 && dt[k] >= lo && dt[k] < hi) ? 1u : 0u) << k;
     }
-    q.count += __popcll(mask);
+    if (ballot(qm != 0) == 0) continue;  // wave-uniform
+    uint32_t fk[kAggUnroll];
+    int64_t sv[kAggUnroll];
+#pragma unroll
+    for (int k = 0; k < kAggUnroll; ++k) {
+      fk[k] = 0;
+      sv[k] = 0;
+      if ((qm >> k) & 1u) {
+        const uint64_t i = base + static_cast<uint64_t>(k * kWave + ln);
+        fk[k] = static_cast<uint32_t>(b.fkey[i]);
+        sv[k] = b.price[i] * (100 - static_cast<int64_t>(b.disc[i]));
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kAggUnroll; ++k) {
+      const bool qual = (qm >> k) & 1u;
+      const uint64_t mask = ballot(qual);
+      if (qual) {
+        const int pos = q.count + __popcll(mask & lt);
+        q.key[pos] = fk[k];
+        q.val[pos] = sv[k];
+      }
+      q.count += __popcll(mask);
+    }
     __builtin_amdgcn_wave_barrier();
-    if (q.count >= kWave) agg_drain(d, lds, q, kWave, flag, acc);
+    while (q.count >= kWave) agg_drain(d, lds, q, kWave, flag, acc);
   }
 }
 
@@ -145,7 +172,8 @@ __global__ __launch_bounds__(kAggBlock) void join_agg_kernel(Dev d, FactBatchDev
   const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (blockDim.x / kWave);
   const uint64_t wave = static_cast<uint64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
   AggAcc acc;
-  agg_rows(d, lds, q, b, wave * kWave, b.rows, waves * kWave, lo, hi, flag, acc);
+  const uint64_t span = static_cast<uint64_t>(kWave) * kAggUnroll;
+  agg_rows(d, lds, q, b, wave * span, b.rows, waves * span, lo, hi, flag, acc);
   agg_flush(d, lds, q, flag, acc, b.result);
 }
 
@@ -181,7 +209,7 @@ __global__ __launch_bounds__(kAggBlock) void join_agg_persistent_kernel(Dev d, c
     const FactBatchDev b = batches[bi];
     const uint64_t r0 = (t - b.tile_start) * kAggTileRows;
     const uint64_t r1 = std::min<uint64_t>(b.rows, r0 + kAggTileRows);
-    agg_rows(d, lds, q, b, r0, r1, kWave, lo, hi, flag, acc);
+    agg_rows(d, lds, q, b, r0, r1, static_cast<uint64_t>(kWave) * kAggUnroll, lo, hi, flag, acc);
     agg_flush(d, lds, q, flag, acc, b.result);
   }
 }
@@ -223,7 +251,7 @@ void launch_join_agg(const Dev& d, const FactBatchDev& b, int32_t lo, int32_t hi
   const size_t lds = (d.lds_bytes() + 15) / 16 * 16 + agg_queue_bytes();
   const void* fn = reinterpret_cast<const void*>(&join_agg_kernel<Dev>);
   uint64_t blocks = static_cast<uint64_t>(blocks_per_cu(fn, kAggBlock, lds)) * cu;
-  blocks = std::min<uint64_t>(blocks, ceil_div(b.rows, kAggBlock * 4));
+  blocks = std::min<uint64_t>(blocks, ceil_div(b.rows, static_cast<uint64_t>(kAggBlock) * kAggUnroll));
   blocks = std::max<uint64_t>(blocks, 1);
   hipLaunchKernelGGL((join_agg_kernel<Dev>), dim3(static_cast<unsigned>(blocks)), dim3(kAggBlock), lds, s, d, b, lo, hi,
                      flag);
