@@ -17,7 +17,7 @@ void fill_defaults(gj_index_params& p) {
   if (!p.harmonia_group_lanes) p.harmonia_group_lanes = 8;  // one 128-byte line per load
   if (!p.btree_lds_bytes) p.btree_lds_bytes = 16384;
   if (!p.learned_leaf_bits) p.learned_leaf_bits = 16;
-  if (!p.learned_max_error) p.learned_max_error = 64;
+  if (!p.learned_max_error) p.learned_max_error = 128;
   if (!p.mlp_hidden) p.mlp_hidden = 32;
   if (!p.mlp_steps) p.mlp_steps = 400;
   if (!p.mlp_batch) p.mlp_batch = 65536;
