@@ -541,8 +541,8 @@ struct HashDev {
   __device__ void stage(uint8_t*) const {}
 
   __device__ __forceinline__ uint32_t lookup(K x, bool active, const uint8_t*) const {
-    if (!active) return kNotFound;
     const uint64_t k = static_cast<uint64_t>(x);
+    if (!active || k == ~0ull) return kNotFound;  // all-ones is the empty marker, never a stored key
     uint64_t s = murmur3_64(k) & mask;
     for (uint64_t probes = 0; probes <= mask; ++probes) {
       const u32x4 e = *reinterpret_cast<const u32x4*>(table + s);

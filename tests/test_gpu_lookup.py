@@ -97,3 +97,47 @@ def test_duplicates_binary_search_first_occurrence(gj, orc, gpu):
     probes = np.array([2, 5, 9, 1, 3, 0, 10], dtype=np.uint64)
     got, _ = _run(gj, "binary_search", keys, probes)
     assert np.array_equal(got, orc.lookup_lower_bound(keys, probes))
+
+
+ALL_KINDS = ["binary_search", "btree", "harmonia", "radix_spline", "learned_plr", "learned_mlp", "hash"]
+
+
+@pytest.mark.parametrize("kind", ALL_KINDS)
+@pytest.mark.parametrize("kb", [4, 8])
+@pytest.mark.parametrize("approach", ["plain", "bws", "partitioning"])
+def test_extreme_keys(gj, orc, gpu, kind, kb, approach):
+    # keys at both ends of the key range (0 and the largest key below the all-ones
+    # value, which the hash table reserves as its empty marker), a ragged column size
+    # (not a multiple of a 128-byte line) and probes at and next to every extreme
+    dt = np.uint64 if kb == 8 else np.uint32
+    top = int(np.iinfo(dt).max) - 1
+    mid = np.unique(np.random.default_rng(kb).integers(2, top - 2, size=4000, dtype=dt))
+    keys = np.unique(np.concatenate([np.array([0, 1, top - 1, top], dtype=dt), mid]))
+    if kind == "harmonia":
+        keys = keys[: keys.size // 32 * 32]
+        keys[-1] = top
+    probes = np.concatenate([keys, np.array([0, 1, 2, top - 2, top - 1, top, top + 1], dtype=dt),
+                             keys[1:-1] + 1, keys[1:-1] - 1])
+    print(f"[case] extreme keys kind={kind} kb={kb} approach={approach} n={keys.size}", flush=True)
+    col = gj.DeviceArray.from_numpy(keys)
+    idx = gj.Index(kind, col, host_column=keys, params=gj.index_params(learned_leaf_bits=6, mlp_steps=50))
+    dp = gj.DeviceArray.from_numpy(probes)
+    out = gj.DeviceArray(probes.size, np.uint32)
+    idx.lookup(dp, out, approach=approach, radix_bits=6, window=1000)
+    gj.synchronize()
+    assert np.array_equal(out.numpy(), orc.lookup_lower_bound(keys, probes))
+
+
+@pytest.mark.parametrize("kind", ["binary_search", "btree", "radix_spline", "learned_plr", "learned_mlp", "hash"])
+@pytest.mark.parametrize("kb", [4, 8])
+def test_single_key_column(gj, orc, gpu, kind, kb):
+    dt = np.uint64 if kb == 8 else np.uint32
+    keys = np.array([12345], dtype=dt)
+    probes = np.array([0, 12344, 12345, 12346, np.iinfo(dt).max - 1], dtype=dt)
+    col = gj.DeviceArray.from_numpy(keys)
+    idx = gj.Index(kind, col, host_column=keys, params=gj.index_params(learned_leaf_bits=4, mlp_steps=20))
+    dp = gj.DeviceArray.from_numpy(probes)
+    out = gj.DeviceArray(probes.size, np.uint32)
+    idx.lookup(dp, out)
+    gj.synchronize()
+    assert np.array_equal(out.numpy(), orc.lookup_lower_bound(keys, probes))
