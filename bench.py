@@ -81,7 +81,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--log2-n", type=int, default=26)
     ap.add_argument("--log2-m", type=int, default=28)
-    ap.add_argument("--indexes", default="binary_search,btree,harmonia,radix_spline,learned_plr,learned_mlp")
+    ap.add_argument("--indexes", default="binary_search,btree,harmonia,radix_spline,learned_plr,learned_mlp,hash")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample-log2", type=int, default=24)
     ap.add_argument("--profile-index", default="", help="run only this index (for rocprofv3 runs)")

@@ -16,7 +16,7 @@ void fill_defaults(gj_index_params& p) {
   if (!p.rs_max_error) p.rs_max_error = 32;
   if (!p.harmonia_group_lanes) p.harmonia_group_lanes = 8;  // one 128-byte line per load
   if (!p.btree_lds_bytes) p.btree_lds_bytes = 16384;
-  if (!p.learned_leaf_bits) p.learned_leaf_bits = 16;
+  // learned_leaf_bits 0: chosen from the column size at build
   if (!p.learned_max_error) p.learned_max_error = 128;
   if (!p.mlp_hidden) p.mlp_hidden = 32;
   if (!p.mlp_steps) p.mlp_steps = 400;

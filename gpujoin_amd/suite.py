@@ -80,7 +80,7 @@ def run(gj, G, quick=False, cpu=True, emit=None):
     # ---- Zipf probe --------------------------------------------------------------
     probes, exp = gj.gen_probes(keys, m, _lib.PROBES_ZIPF, 1.0, 0.0, 63, 0x13)
     dp = gj.DeviceArray.from_numpy(probes, stream)
-    for kind in ("binary_search", "btree", "harmonia", "radix_spline", "learned_plr"):
+    for kind in ("binary_search", "btree", "harmonia", "radix_spline", "learned_plr", "hash"):
         idx = gj.Index(kind, dk, host_column=keys, stream=stream)
         for appr in ("plain", "partitioning"):
             ms = _time(gj, lambda: idx.lookup(dp, out, params=gj.lookup_params(appr), stream=stream), stream, steps=3)
@@ -88,6 +88,17 @@ def run(gj, G, quick=False, cpu=True, emit=None):
                   "lookups_per_s": m / ms * 1e3, "gather_roofline_frac": m / ms * 1e3 / G, "ms": ms,
                   "rowids_match": bool(np.array_equal(out.numpy(), exp))})
         idx.close()
+    del probes, exp
+
+    # ---- hash-join driver: a linear-probing table built over the column on every call ----
+    probes, exp = gj.gen_probes(keys, m, _lib.PROBES_UNIFORM, 1.0, 0.0, 63, 0x14)
+    dp.upload(probes)
+    idx = gj.Index("btree", dk, host_column=keys, stream=stream)
+    ms = _time(gj, lambda: idx.lookup(dp, out, params=gj.lookup_params("hj"), stream=stream), stream, steps=3, warmup=1)
+    emit({"config": f"hash-join driver (table build + probe per call): 2^{n_log} u64 keys, 2^{m_log} uniform probes",
+          "index": "btree", "approach": "hj", "lookups_per_s": m / ms * 1e3, "gather_roofline_frac": m / ms * 1e3 / G,
+          "ms": ms, "rowids_match": bool(np.array_equal(out.numpy(), exp))})
+    idx.close()
     del probes, exp
 
     # ---- partitioned sweeps -------------------------------------------------

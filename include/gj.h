@@ -82,7 +82,7 @@ typedef struct gj_index_params {
   uint32_t rs_max_error;         /* RadixSpline error bound, default 32 */
   uint32_t harmonia_group_lanes; /* lanes per Harmonia node search (8, 16, or 32 for u32 keys), default 8 */
   uint32_t btree_lds_bytes;      /* LDS budget for resident upper B-tree levels, default 16384 */
-  uint32_t learned_leaf_bits;    /* log2 of the number of linear leaves, default 16 */
+  uint32_t learned_leaf_bits;    /* log2 of the number of linear leaves; 0 (default) = max(16, ceil(log2 n) - 10) */
   uint32_t learned_max_error;    /* last-mile bound; leaves with a wider measured window use the B-tree, default 128 */
   uint32_t mlp_hidden;           /* hidden width of the MLP root (<= 64), default 32 */
   uint32_t mlp_steps;            /* training steps of the MLP root, default 400 */

@@ -38,7 +38,7 @@ def test_parse_index_type_names(gj):
 
 def test_param_defaults(gj):
     p = gj.index_params()
-    assert (p.rs_radix_bits, p.rs_max_error, p.harmonia_group_lanes, p.learned_leaf_bits) == (18, 32, 8, 16)
+    assert (p.rs_radix_bits, p.rs_max_error, p.harmonia_group_lanes, p.learned_leaf_bits) == (18, 32, 8, 0)
     lp = gj.lookup_params()
     assert lp.approach == 0 and lp.block_size == 256 and lp.ignore_bits == 0xFFFFFFFF
 
