@@ -444,9 +444,10 @@ __device__ __forceinline__ int64_t leaf_pred(const LeafModel& m, float dkey) {
 // How probes of leaves routed to the B-tree are answered:
 //   kLeafNoFallback  no such leaf exists; the kernel carries no B-tree code
 //   kLeafInline      the B-tree lookup runs inside the same kernel
-//   kLeafMark        the probe's result is kFallbackMark; the plain driver defers
-//                    such probes into a list that a second kernel answers with the
-//                    B-tree, so the hot kernel keeps the no-fallback register count
+//   kLeafMark        the probe's result is kFallbackMark; a second kernel of the
+//                    plain driver scans the results and answers the marked probes
+//                    with the B-tree, so the hot kernel keeps the no-fallback
+//                    register count (used while few keys sit in fallback leaves)
 constexpr int kLeafNoFallback = 0;
 constexpr int kLeafInline = 1;
 constexpr int kLeafMark = 2;
