@@ -60,6 +60,7 @@ class IndexStats(C.Structure):
         ("depth", C.c_uint32), ("lds_levels", C.c_uint32), ("spline_points", C.c_uint64), ("leaves", C.c_uint64),
         ("fallback_leaves", C.c_uint64), ("fallback_keys", C.c_uint64), ("max_window", C.c_uint32),
         ("avg_window", C.c_double), ("train_loss", C.c_float), ("root_segments", C.c_uint32),
+        ("split_leaves", C.c_uint32),
     ]
 
 

@@ -54,7 +54,8 @@ struct gj_index {
   uint64_t rs_points = 0;
 
   // learned two-stage models (B-tree above is their fallback)
-  gj::dbuf lm_leaves, lm_mlp, lm_knots;
+  gj::dbuf lm_leaves, lm_mlp, lm_knots, lm_seps, lm_subs;
+  uint32_t lm_splits = 0;  // learned leaves split into sub-leaves
   gj::RootModel lm_root{};
   uint32_t lm_num_leaves = 0, lm_fallback_any = 0;
   int32_t lm_err_lo = 0, lm_err_hi = 0;  // learned window bounds over all leaves
@@ -122,6 +123,9 @@ struct gj_index {
     d.num_leaves = lm_num_leaves;
     d.err_lo = lm_err_lo;
     d.err_hi = lm_err_hi;
+    d.num_splits = lm_splits;
+    d.seps = lm_seps.as<const uint64_t>();
+    d.subs = lm_subs.as<const gj::LeafModel>();
     d.btree = btree_dev<K>();
     return d;
   }

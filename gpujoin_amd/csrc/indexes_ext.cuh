@@ -280,6 +280,35 @@ __device__ __forceinline__ uint32_t group_window_search(const K* col, uint64_t n
   return found;
 }
 
+// Number of the 16 u64 keys of line `my_line` (of `lines`) that are below each
+// lane's probe, for the lanes with my_live; whole-wave call (4-lane groups, one
+// 128-byte line per round, all rounds' lines requested before any is examined).
+template <class K>
+__device__ __forceinline__ uint32_t group_line_count(const uint64_t* lines, uint32_t my_line, K my_x, bool my_live) {
+  using LP = LineProbe<uint64_t>;
+  constexpr int G = kWinG;
+  const int ln = lane();
+  const int gbase = ln & ~(G - 1);
+  const int gl = ln & (G - 1);
+  const uint64_t live_mask = ballot(my_live);
+  LP pc[G];
+#pragma unroll
+  for (int r = 0; r < G; ++r) {
+    const uint32_t l = shfl32(my_line, gbase + r);
+    if ((live_mask >> (gbase + r)) & 1ull) pc[r].load(lines, ~0ull, l, gl);
+  }
+  uint32_t below = 0;
+#pragma unroll
+  for (int r = 0; r < G; ++r) {
+    if (!((live_mask >> (gbase + r)) & 1ull)) continue;
+    uint32_t b;
+    bool e;
+    pc[r].eval(shfl64(static_cast<uint64_t>(my_x), gbase + r), gbase, b, e);
+    if (gl == r) below = b;
+  }
+  return below;
+}
+
 // ---------------------------------------------------------------------------
 // RadixSpline (radix_spline_index, src/indexes.cuh:176-248; estimate
 // src/rs.cuh:118-154).  Knots are {u64 key, f64 position} pairs, 16 bytes, like
@@ -453,6 +482,15 @@ constexpr int kLeafInline = 1;
 constexpr int kLeafMark = 2;
 constexpr uint32_t kFallbackMark = 0xFFFFFFFEu;  // never a row id: n < 2^32 - 1
 
+// A leaf whose keys no line fits within the bound is split into kSplitWays
+// equal-count sub-leaves: the leaf's model holds slope = NaN and start = the
+// split's index j; separator line j holds the max key of each sub-leaf (the last
+// = all ones), and the number of separators below the probe picks the sub-leaf.
+// A sub-leaf that is still too wide (slope = NaN) routes to the B-tree, as does a
+// leaf with slope = NaN and start = kNoSplit.
+constexpr uint32_t kSplitWays = 16;
+constexpr uint32_t kNoSplit = 0xFFFFFFFFu;
+
 template <class K, int Mode>
 struct LearnedDev {
   static constexpr bool FB = Mode == kLeafInline;
@@ -463,6 +501,9 @@ struct LearnedDev {
   const LeafModel* leaves;
   uint32_t num_leaves;
   int32_t err_lo, err_hi;  // window [pred + err_lo, pred + err_hi]
+  uint32_t num_splits;     // split leaves
+  const uint64_t* seps;    // num_splits separator lines of kSplitWays keys
+  const LeafModel* subs;   // num_splits x kSplitWays sub-leaf models
   BTreeDev<K> btree;       // fallback (used when FB)
 
   __host__ __device__ size_t root_offset() const { return FB ? btree.lds_bytes() : 0; }
@@ -486,14 +527,26 @@ struct LearnedDev {
   }
 
   __device__ __forceinline__ uint32_t lookup(K x, bool active, const uint8_t* lds) const {
-    bool use_tree = false;
+    bool use_tree = false, split = false;
     uint64_t lo = 0, hi = 0, pred = 0;
     bool live = active && x >= min_key && x <= max_key;
+    LeafModel m{};
     if (live) {
-      const LeafModel m = leaves[route(x, lds)];
-      if (Mode != kLeafNoFallback && __builtin_isnan(m.slope)) {  // fallback leaf
-        use_tree = true;
-      } else {
+      m = leaves[route(x, lds)];
+      if (__builtin_isnan(m.slope)) {
+        if (m.start != kNoSplit) split = true;
+        else use_tree = true;  // only with fallback leaves (Mode != kLeafNoFallback)
+      }
+    }
+    if (num_splits && ballot(split) != 0) {  // wave-uniform
+      const uint32_t t = group_line_count<K>(seps, split ? m.start : 0u, x, split);
+      if (split) {
+        m = subs[static_cast<uint64_t>(m.start) * kSplitWays + (t < kSplitWays ? t : kSplitWays - 1)];
+        use_tree = __builtin_isnan(m.slope);
+      }
+    }
+    if (live) {
+      if (!use_tree) {
         const int64_t p = leaf_pred(m, leaf_dkey(x, m.base_key));
         int64_t a = p + err_lo, b = p + err_hi + 1;
         a = a < 0 ? 0 : a;

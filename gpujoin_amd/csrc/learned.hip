@@ -257,14 +257,16 @@ struct LeafFit {
 };
 
 // One block per leaf: least squares of local position on FP32 key offsets.
+// Leaf l covers positions [lo[l], hi[l]).
 template <class K>
 __global__ __launch_bounds__(256) void leaf_fit_kernel(const K* __restrict__ col, uint64_t n,
-                                                       const uint32_t* __restrict__ start, uint32_t leaves,
-                                                       LeafModel* __restrict__ out, LeafFit* __restrict__ fit) {
+                                                       const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi,
+                                                       uint32_t leaves, LeafModel* __restrict__ out,
+                                                       LeafFit* __restrict__ fit) {
   const uint32_t l = blockIdx.x;
   if (l >= leaves) return;
-  const uint64_t s = start[l];
-  const uint64_t e = start[l + 1];
+  const uint64_t s = lo[l];
+  const uint64_t e = hi[l];
   const uint64_t cnt = e > s ? e - s : 0;
   const uint64_t base = cnt ? static_cast<uint64_t>(col[s]) : 0;
   double sd = 0, sdd = 0, sj = 0, sdj = 0;
@@ -346,8 +348,63 @@ __global__ void leaf_finalize_kernel(LeafModel* __restrict__ lm, LeafFit* __rest
     f.err_lo = 1;
     f.err_hi = 0;
     lm[l].slope = __builtin_nanf("");
+    lm[l].start = kNoSplit;
   }
   fit[l] = f;
+}
+
+// Separator line of split j: sub-leaf t covers [sub_lo, sub_hi); its separator is the
+// key at sub_hi - 1 (at the leaf's first position for an empty leading sub-leaf), the
+// last separator is all ones, so the count of separators below a key of the split
+// leaf is the index of the sub-leaf that holds it.
+template <class K>
+__global__ void split_sep_kernel(const K* __restrict__ col, const uint32_t* __restrict__ sub_lo,
+                                 const uint32_t* __restrict__ sub_hi, const uint32_t* __restrict__ leaf_lo,
+                                 uint32_t splits, uint64_t* __restrict__ sep) {
+  const uint64_t g = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (g >= static_cast<uint64_t>(splits) * kSplitWays) return;
+  const uint32_t t = static_cast<uint32_t>(g % kSplitWays);
+  const uint32_t j = static_cast<uint32_t>(g / kSplitWays);
+  if (t == kSplitWays - 1) {
+    sep[g] = ~0ull;
+    return;
+  }
+  const uint32_t h = sub_hi[g];
+  const uint32_t at = h > leaf_lo[j] ? h - 1 : leaf_lo[j];
+  sep[g] = static_cast<uint64_t>(col[at]);
+}
+
+// err_lo/err_hi of the sub-leaves: one block per split leaf, over the leaf's keys,
+// with the separator count and arithmetic of the lookup.
+template <class K>
+__global__ void split_error_kernel(const K* __restrict__ col, const uint32_t* __restrict__ leaf_lo,
+                                   const uint32_t* __restrict__ leaf_hi, const uint64_t* __restrict__ sep,
+                                   const LeafModel* __restrict__ subs, LeafFit* __restrict__ fit) {
+  const uint32_t j = blockIdx.x;
+  const uint64_t* sp = sep + static_cast<uint64_t>(j) * kSplitWays;
+  for (uint64_t i = leaf_lo[j] + threadIdx.x; i < leaf_hi[j]; i += blockDim.x) {
+    const K key = col[i];
+    uint32_t t = 0;
+    for (uint32_t u = 0; u < kSplitWays; ++u) t += sp[u] < static_cast<uint64_t>(key) ? 1u : 0u;
+    t = t < kSplitWays ? t : kSplitWays - 1;
+    const uint64_t g = static_cast<uint64_t>(j) * kSplitWays + t;
+    const LeafModel m = subs[g];
+    int64_t e = static_cast<int64_t>(i) - leaf_pred(m, leaf_dkey(key, m.base_key));
+    e = e < INT_MIN / 2 ? INT_MIN / 2 : (e > INT_MAX / 2 ? INT_MAX / 2 : e);
+    atomicMin(&fit[g].err_lo, static_cast<int>(e));
+    atomicMax(&fit[g].err_hi, static_cast<int>(e));
+  }
+}
+
+// The split leaves' own models: slope = NaN, start = split index.
+__global__ void split_mark_kernel(LeafModel* __restrict__ lm, const uint32_t* __restrict__ leaf_id, uint32_t splits) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= splits) return;
+  LeafModel m;
+  m.base_key = 0;
+  m.start = j;
+  m.slope = __builtin_nanf("");
+  lm[leaf_id[j]] = m;
 }
 
 }  // namespace
@@ -511,8 +568,8 @@ void build_learned(gj_index& ix, hipStream_t s) {
     }
     ix.lm_leaves.alloc(sizeof(LeafModel) * L);
     dbuf fit(sizeof(LeafFit) * L);
-    hipLaunchKernelGGL(leaf_fit_kernel<K>, dim3(L), dim3(256), 0, s, col, n, start.as<const uint32_t>(), L,
-                       ix.lm_leaves.as<LeafModel>(), fit.as<LeafFit>());
+    hipLaunchKernelGGL(leaf_fit_kernel<K>, dim3(L), dim3(256), 0, s, col, n, start.as<const uint32_t>(),
+                       start.as<const uint32_t>() + 1, L, ix.lm_leaves.as<LeafModel>(), fit.as<LeafFit>());
     GJ_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(leaf_error_kernel<K>, dim3(grid_for(n, 256)), dim3(256), 0, s, col, n, route,
                        ix.lm_leaves.as<const LeafModel>(), fit.as<LeafFit>());
@@ -522,14 +579,75 @@ void build_learned(gj_index& ix, hipStream_t s) {
                        fit.as<LeafFit>(), L, max_width);
     GJ_HIP_CHECK(hipGetLastError());
 
-    // window bounds over the leaves that are not routed to the B-tree, and statistics
     std::vector<LeafFit> hf(L);
     GJ_HIP_CHECK(hipMemcpyAsync(hf.data(), fit.p, fit.bytes, hipMemcpyDeviceToHost, s));
     GJ_HIP_CHECK(hipStreamSynchronize(s));
+
+    // split a leaf that is too wide into kSplitWays equal-count sub-leaves, unless its
+    // sub-leaves would still hold over 4 average leaves each (then no line fits them
+    // either, and the extra separator read would only delay the B-tree)
+    const uint64_t split_max = static_cast<uint64_t>(kSplitWays) * 4 * std::max<uint64_t>(1, ceil_div(n, L));
+    std::vector<uint32_t> split_id, split_lo, split_hi, sub_lo, sub_hi;
+    for (uint32_t l = 0; l < L; ++l) {
+      const LeafFit& f = hf[l];
+      if (f.count == 0 || f.err_hi >= f.err_lo || f.count > split_max) continue;
+      split_id.push_back(l);
+      split_lo.push_back(f.start);
+      split_hi.push_back(f.start + f.count);
+      for (uint32_t t = 0; t < kSplitWays; ++t) {
+        sub_lo.push_back(f.start + static_cast<uint32_t>((static_cast<uint64_t>(f.count) * t) / kSplitWays));
+        sub_hi.push_back(f.start + static_cast<uint32_t>((static_cast<uint64_t>(f.count) * (t + 1)) / kSplitWays));
+      }
+    }
+    const uint32_t J = static_cast<uint32_t>(split_id.size());
+    std::vector<LeafFit> hs;
+    ix.lm_splits = J;
+    ix.lm_seps.release();
+    ix.lm_subs.release();
+    if (J) {
+      const uint64_t S = static_cast<uint64_t>(J) * kSplitWays;
+      dbuf d_id(J * 4), d_lo(J * 4), d_hi(J * 4), d_slo(S * 4), d_shi(S * 4), sfit(S * sizeof(LeafFit));
+      GJ_HIP_CHECK(hipMemcpyAsync(d_id.p, split_id.data(), J * 4, hipMemcpyHostToDevice, s));
+      GJ_HIP_CHECK(hipMemcpyAsync(d_lo.p, split_lo.data(), J * 4, hipMemcpyHostToDevice, s));
+      GJ_HIP_CHECK(hipMemcpyAsync(d_hi.p, split_hi.data(), J * 4, hipMemcpyHostToDevice, s));
+      GJ_HIP_CHECK(hipMemcpyAsync(d_slo.p, sub_lo.data(), S * 4, hipMemcpyHostToDevice, s));
+      GJ_HIP_CHECK(hipMemcpyAsync(d_shi.p, sub_hi.data(), S * 4, hipMemcpyHostToDevice, s));
+      ix.lm_seps.alloc(S * 8);
+      ix.lm_subs.alloc(S * sizeof(LeafModel));
+      hipLaunchKernelGGL(split_sep_kernel<K>, dim3(grid_for(S, 256)), dim3(256), 0, s, col, d_slo.as<const uint32_t>(),
+                         d_shi.as<const uint32_t>(), d_lo.as<const uint32_t>(), J, ix.lm_seps.as<uint64_t>());
+      GJ_HIP_CHECK(hipGetLastError());
+      hipLaunchKernelGGL(leaf_fit_kernel<K>, dim3(static_cast<unsigned>(S)), dim3(256), 0, s, col, n,
+                         d_slo.as<const uint32_t>(), d_shi.as<const uint32_t>(), static_cast<uint32_t>(S),
+                         ix.lm_subs.as<LeafModel>(), sfit.as<LeafFit>());
+      GJ_HIP_CHECK(hipGetLastError());
+      hipLaunchKernelGGL(split_error_kernel<K>, dim3(J), dim3(256), 0, s, col, d_lo.as<const uint32_t>(),
+                         d_hi.as<const uint32_t>(), ix.lm_seps.as<const uint64_t>(), ix.lm_subs.as<const LeafModel>(),
+                         sfit.as<LeafFit>());
+      GJ_HIP_CHECK(hipGetLastError());
+      hipLaunchKernelGGL(leaf_finalize_kernel, dim3(grid_for(S, 256)), dim3(256), 0, s, ix.lm_subs.as<LeafModel>(),
+                         sfit.as<LeafFit>(), static_cast<uint32_t>(S), max_width);
+      GJ_HIP_CHECK(hipGetLastError());
+      hipLaunchKernelGGL(split_mark_kernel, dim3(grid_for(J, 256)), dim3(256), 0, s, ix.lm_leaves.as<LeafModel>(),
+                         d_id.as<const uint32_t>(), J);
+      GJ_HIP_CHECK(hipGetLastError());
+      hs.resize(S);
+      GJ_HIP_CHECK(hipMemcpyAsync(hs.data(), sfit.p, sfit.bytes, hipMemcpyDeviceToHost, s));
+      GJ_HIP_CHECK(hipStreamSynchronize(s));
+    }
+
+    // window bounds over the (sub-)leaves that are not routed to the B-tree, and statistics
     uint64_t fb_leaves = 0, fb_keys = 0, wsum = 0, wkeys = 0;
     uint32_t wmax = 0;
     int32_t elo = 0, ehi = 0;
-    for (const auto& f : hf) {
+    std::vector<uint8_t> is_split(L, 0);
+    for (const uint32_t l : split_id) is_split[l] = 1;
+    std::vector<const LeafFit*> live_fits;
+    for (uint32_t l = 0; l < L; ++l)
+      if (!is_split[l]) live_fits.push_back(&hf[l]);  // split leaves count through their sub-leaves
+    for (const auto& f : hs) live_fits.push_back(&f);
+    for (const LeafFit* fp : live_fits) {
+      const LeafFit& f = *fp;
       if (f.count == 0) continue;
       if (f.err_hi < f.err_lo) {
         ++fb_leaves;
@@ -552,6 +670,7 @@ void build_learned(gj_index& ix, hipStream_t s) {
     ix.stats.leaves = L;
     ix.stats.fallback_leaves = fb_leaves;
     ix.stats.fallback_keys = fb_keys;
+    ix.stats.split_leaves = J;
     ix.stats.max_window = wmax;
     ix.stats.avg_window = wkeys ? static_cast<double>(wsum) / static_cast<double>(wkeys) : 0.0;
     ix.stats.depth = 2;

@@ -157,6 +157,7 @@ typedef struct gj_index_stats {
   double avg_window;          /* mean last-mile window over build keys (learned) */
   float train_loss;           /* final MLP training loss (learned_mlp) */
   uint32_t root_segments;     /* learned_plr: segments of the spline root (0: linear root) */
+  uint32_t split_leaves;      /* learned: leaves too wide for one line, split into 16 sub-leaves */
 } gj_index_stats;
 
 int gj_index_get_stats(const gj_index* index, gj_index_stats* out);

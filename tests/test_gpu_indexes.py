@@ -139,6 +139,25 @@ def test_learned_few_fallback_leaves_exact(gj, orc, gpu, kind):
     assert np.array_equal(got, orc.lookup_lower_bound(keys, probes))
 
 
+@pytest.mark.parametrize("kind", ["learned_plr", "learned_mlp"])
+@pytest.mark.parametrize("kb", [4, 8])
+def test_split_leaves_exact(gj, orc, gpu, kind, kb):
+    # clustered keys under a linear root: whole clusters land in single leaves, which are
+    # split into 16 equal-count sub-leaves (one separator line picks the sub-leaf); the
+    # sub-leaves that are still too wide go to the B-tree; every answer stays exact
+    keys = gj.gen_build_keys(1 << 18, kb, _lib.KEYS_CLUSTERED, 0, 0x51 + kb)
+    probes = _probes(gj, keys, 60000, 0x52 + kb)
+    params = gj.index_params(learned_leaf_bits=12, learned_max_error=32, learned_root_segments=1, mlp_steps=60)
+    got, st = _lookup(gj, kind, keys, probes, params=params)
+    print(f"[case] {kind} kb={kb} split_leaves={st['split_leaves']} fallback_keys={st['fallback_keys']}", flush=True)
+    assert st["split_leaves"] > 0
+    assert np.array_equal(got, orc.lookup_lower_bound(keys, probes))
+    # the same through every driver
+    for approach in ("bws", "partitioning"):
+        got2, _ = _lookup(gj, kind, keys, probes, params=params, approach=approach, radix_bits=8, window=20000)
+        assert np.array_equal(got2, got), approach
+
+
 def test_mlp_root_beats_linear_root_on_skewed_keys(gj, gpu):
     # what the trained network gains: on lognormal keys the linear root crowds most
     # keys into a few leaves; the trained monotone MLP root spreads them, so far fewer
@@ -183,10 +202,9 @@ def test_learned_spline_root_tiny_columns(gj, orc, gpu):
 
 
 def test_spline_root_spreads_clustered_keys(gj, gpu):
-    # 64 narrow clusters: a linear root sends each cluster to one leaf, so every key
-    # falls back to the B-tree; the automatic choice switches to a spline of the
-    # column's CDF, which spreads the keys evenly over the leaves (only leaves in the
-    # clusters' sparse tails stay too wide for a line)
+    # 64 narrow clusters: a linear root sends each cluster to one leaf, and even the
+    # leaves' 16-way splits leave many keys to the B-tree; the automatic choice switches
+    # to a spline of the column's CDF, which spreads the keys evenly over the leaves
     keys = gj.gen_build_keys(1 << 20, 8, _lib.KEYS_CLUSTERED, 63, 0x33)
     col = gj.DeviceArray.from_numpy(keys)
     lin = gj.Index("learned_plr", col, host_column=keys,
@@ -195,7 +213,7 @@ def test_spline_root_spreads_clustered_keys(gj, gpu):
                     params=gj.index_params(learned_leaf_bits=12, learned_max_error=32)).stats()
     print(f"[gain] linear root fallback_keys={lin['fallback_keys']}; auto root segments={auto['root_segments']} "
           f"fallback_keys={auto['fallback_keys']} avg_window={auto['avg_window']:.1f}", flush=True)
-    assert lin["root_segments"] == 0 and lin["fallback_keys"] > keys.size // 2
+    assert lin["root_segments"] == 0 and lin["fallback_keys"] > 0 and lin["split_leaves"] > 0
     assert 1 <= auto["root_segments"] <= 2048
     assert auto["fallback_keys"] * 20 < lin["fallback_keys"]
 
