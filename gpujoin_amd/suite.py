@@ -101,6 +101,31 @@ def run(gj, G, quick=False, cpu=True, emit=None):
     idx.close()
     del probes, exp
 
+    # ---- lookup plans: uneven probe batches against one resident index ----------------
+    rng = np.random.default_rng(0x15)
+    sizes, total = [], 0
+    while total < m:
+        sz = min(int(np.exp(rng.uniform(np.log(1e3), np.log(1e7)))), m - total)
+        sizes.append(sz)
+        total += sz
+    probes, exp = gj.gen_probes(keys, m, _lib.PROBES_UNIFORM, 1.0, 0.0, 63, 0x16)
+    dp.upload(probes)
+    idx = gj.Index("learned_plr", dk, host_column=keys, stream=stream)
+    batches, off = [], 0
+    for sz in sizes:
+        batches.append((dp.offset(off), out.offset(off), sz))
+        off += sz
+    for mode, mname in ((_lib.BATCH_LAUNCH, "launch"), (_lib.BATCH_GRAPH, "hipgraph"),
+                        (_lib.BATCH_PERSISTENT, "persistent")):
+        plan = gj.LookupPlan(idx, batches, mode, stream=stream)
+        ms = _time(gj, lambda: plan.run(stream), stream, steps=3, warmup=1)
+        emit({"config": f"lookup plan, {len(sizes)} log-uniform probe batches (1e3..1e7), 2^{n_log} u64 keys",
+              "index": "learned_plr", "mode": mname, "lookups_per_s": m / ms * 1e3, "ms": ms,
+              "rowids_match": bool(np.array_equal(out.numpy(), exp))})
+        plan.close()
+    idx.close()
+    del probes, exp
+
     # ---- partitioned sweeps -------------------------------------------------
     probes, exp = gj.gen_probes(keys, m, _lib.PROBES_UNIFORM, 1.0, 0.0, 63, 0x21)
     dp.upload(probes)
