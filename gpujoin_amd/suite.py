@@ -7,6 +7,8 @@ Writes one JSON object per measurement to stdout-compatible lines via `emit`.
 
   u32 probe      2^20 unique u32 keys from [0, 2^32), 2^24 uniform probes: row ids + checksum
   Zipf probe     2^26 u64 keys, 2^28 probes, Zipf theta = 1.0 (rank 0 never drawn, as in the reference)
+  sorted / dense probes in key order; dense keys (key = row id) as in the reference's scripts
+  lookup plans   uneven probe batches through launches, a hipGraph and the persistent kernel
   partitioned    fan-out 2^8 / 2^10 / 2^12, 1 and 2 passes, windows 2^16..2^22;
                  index-size sweep 2^20..2^30 keys (B-tree and learned)
   learned        RadixSpline err 8/32/64 and the PLR / MLP models over uniform,
@@ -100,6 +102,32 @@ def run(gj, G, quick=False, cpu=True, emit=None):
           "ms": ms, "rowids_match": bool(np.array_equal(out.numpy(), exp))})
     idx.close()
     del probes, exp
+
+    # ---- sorted probes (the reference's -s true) and dense keys (its --dataset dense) ----
+    probes, exp = gj.gen_probes(keys, m, _lib.PROBES_UNIFORM, 1.0, 0.0, 63, 0x17)
+    order = np.argsort(probes, kind="stable")
+    dp.upload(probes[order])
+    exp_sorted = exp[order]
+    del probes, exp, order
+    for kind in ("btree", "learned_plr", "radix_spline", "harmonia"):
+        idx = gj.Index(kind, dk, host_column=keys, stream=stream)
+        ms = _time(gj, lambda: idx.lookup(dp, out, params=gj.lookup_params("plain"), stream=stream), stream, steps=3)
+        emit({"config": f"sorted probes: 2^{n_log} u64 keys, 2^{m_log} uniform probes in key order", "index": kind,
+              "lookups_per_s": m / ms * 1e3, "ms": ms, "rowids_match": bool(np.array_equal(out.numpy(), exp_sorted))})
+        idx.close()
+    del exp_sorted
+    dense = gj.gen_build_keys(n, 8, _lib.KEYS_DENSE, 0, 0)
+    dd = gj.DeviceArray.from_numpy(dense, stream)
+    probes, exp = gj.gen_probes(dense, m, _lib.PROBES_UNIFORM, 1.0, 0.0, 0, 0x18)
+    dp.upload(probes)
+    for kind in ("binary_search", "btree", "harmonia", "radix_spline", "learned_plr"):
+        idx = gj.Index(kind, dd, host_column=dense, stream=stream)
+        ms = _time(gj, lambda: idx.lookup(dp, out, params=gj.lookup_params("plain"), stream=stream), stream, steps=3)
+        emit({"config": f"dense keys (key = row id): 2^{n_log} u64 keys, 2^{m_log} uniform probes", "index": kind,
+              "lookups_per_s": m / ms * 1e3, "gather_roofline_frac": m / ms * 1e3 / G, "ms": ms,
+              "rowids_match": bool(np.array_equal(out.numpy(), exp))})
+        idx.close()
+    del dd, dense, probes, exp
 
     # ---- lookup plans: uneven probe batches against one resident index ----------------
     rng = np.random.default_rng(0x15)
