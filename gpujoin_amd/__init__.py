@@ -137,6 +137,43 @@ class DeviceArray:
             pass
 
 
+class HostArray:
+    """Page-locked host memory (gj_host_malloc), mapped into the GPU's address space, so
+    kernels read it in place over the host link: an index column that stays in host
+    memory (the reference's index_lookup_config placement, src/index_lookup_config.hpp:52-66).
+    Same attributes as DeviceArray (ptr, dtype, count) where an index takes its column."""
+
+    def __init__(self, count: int, dtype):
+        self.dtype = np.dtype(dtype)
+        self.count = int(count)
+        self.nbytes = self.count * self.dtype.itemsize
+        p = C.c_void_p()
+        check(_lib.call("gj_host_malloc", C.byref(p), max(self.nbytes, 16)))
+        self.ptr = p
+
+    @classmethod
+    def from_numpy(cls, a: np.ndarray) -> "HostArray":
+        a = np.ascontiguousarray(a)
+        h = cls(a.size, a.dtype)
+        h.view()[:] = a
+        return h
+
+    def view(self) -> np.ndarray:
+        buf = (C.c_uint8 * max(self.nbytes, 1)).from_address(int(self.ptr.value))
+        return np.frombuffer(buf, dtype=np.uint8, count=self.nbytes).view(self.dtype)
+
+    def free(self):
+        if self.ptr:
+            _lib.call("gj_host_free", self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 def device_info() -> dict:
     name = C.create_string_buffer(256)
     cu = C.c_int()

@@ -123,7 +123,15 @@ int gj_host_malloc(void** h_ptr, uint64_t bytes) {
     GJ_REQUIRE(h_ptr, GJ_ERR_INVALID, "h_ptr is null");
     *h_ptr = nullptr;
     if (bytes == 0) return;
-    GJ_HIP_CHECK(hipHostMalloc(h_ptr, bytes, hipHostMallocDefault));
+    GJ_HIP_CHECK(hipHostMalloc(h_ptr, bytes, hipHostMallocMapped | hipHostMallocPortable));
+    // kernels use the host address directly: require the GPU mapping at that address
+    void* d = nullptr;
+    const hipError_t e = hipHostGetDevicePointer(&d, *h_ptr, 0);
+    if (e != hipSuccess || d != *h_ptr) {
+      (void)hipHostFree(*h_ptr);
+      *h_ptr = nullptr;
+      GJ_REQUIRE(false, GJ_ERR_UNSUPPORTED, "page-locked host memory is not mapped at the same address on the GPU");
+    }
   });
 }
 

@@ -9,6 +9,7 @@ Writes one JSON object per measurement to stdout-compatible lines via `emit`.
   Zipf probe     2^26 u64 keys, 2^28 probes, Zipf theta = 1.0 (rank 0 never drawn, as in the reference)
   sorted / dense probes in key order; dense keys (key = row id) as in the reference's scripts
   lookup plans   uneven probe batches through launches, a hipGraph and the persistent kernel
+  host column    the index column in page-locked host memory, plain vs partitioned
   partitioned    fan-out 2^8 / 2^10 / 2^12, 1 and 2 passes, windows 2^16..2^22;
                  index-size sweep 2^20..2^30 keys (B-tree and learned)
   learned        RadixSpline err 8/32/64 and the PLR / MLP models over uniform,
@@ -186,6 +187,29 @@ def run(gj, G, quick=False, cpu=True, emit=None):
             idx.close()
         del dk2, k2, p2, e2
     log("partitioned sweeps done")
+
+    # ---- index column in host memory (page-locked, read in place over the host link) ----
+    mh_log = 22 if quick else 24
+    mh = 1 << mh_log
+    kh = gj.gen_build_keys(n, 8, _lib.KEYS_UNIFORM, 63, 0x24)
+    ph, eh = gj.gen_probes(kh, mh, _lib.PROBES_UNIFORM, 1.0, 0.0, 63, 0x25)
+    hcol = gj.HostArray.from_numpy(kh)
+    dph = gj.DeviceArray.from_numpy(ph, stream)
+    doh = gj.DeviceArray(mh, np.uint32)
+    for kind in ("btree", "learned_plr"):
+        idx = gj.Index(kind, hcol, host_column=kh, stream=stream)
+        for appr, kw in (("plain", {}), ("partitioning", {"radix_bits": 12, "window": mh}),
+                         ("partitioning", {"radix_bits": 12, "passes": 2, "window": mh})):
+            lp = gj.lookup_params(appr, **kw)
+            ms = _time(gj, lambda: idx.lookup(dph, doh, params=lp, stream=stream), stream, steps=2, warmup=1)
+            emit({"config": f"column in host memory: 2^{n_log} u64 keys in page-locked host memory, "
+                            f"2^{mh_log} uniform probes in HBM", "index": kind, "approach": appr,
+                  "passes": kw.get("passes", 1), "fanout": 1 << kw.get("radix_bits", 0) if kw else None,
+                  "lookups_per_s": mh / ms * 1e3, "ms": ms, "rowids_match": bool(np.array_equal(doh.numpy(), eh))})
+        idx.close()
+    hcol.free()
+    del kh, ph, eh, dph, doh
+    log("host-memory column done")
 
     # ---- learned indexes with misses ------------------------------------------------
     for dist, dname in ((_lib.KEYS_UNIFORM, "uniform"), (_lib.KEYS_LOGNORMAL, "lognormal"),

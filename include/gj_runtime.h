@@ -32,7 +32,8 @@ int gj_device_count(int* count);
 int gj_device_info(char* name, size_t len, int* compute_units, int* gcn_arch_major);
 int gj_malloc(void** d_ptr, uint64_t bytes);
 int gj_free(void* d_ptr);
-/* Page-locked host memory (hipHostMalloc). */
+/* Page-locked host memory (hipHostMalloc), mapped into every GPU's address space at the
+ * same address: kernels may read it in place (e.g. an index column kept in host memory). */
 int gj_host_malloc(void** h_ptr, uint64_t bytes);
 int gj_host_free(void* h_ptr);
 int gj_memcpy_htod(void* d_dst, const void* h_src, uint64_t bytes, gj_stream_t stream);

@@ -141,3 +141,21 @@ def test_single_key_column(gj, orc, gpu, kind, kb):
     idx.lookup(dp, out)
     gj.synchronize()
     assert np.array_equal(out.numpy(), orc.lookup_lower_bound(keys, probes))
+
+
+@pytest.mark.parametrize("kind", ["binary_search", "btree", "radix_spline", "learned_plr", "hash"])
+@pytest.mark.parametrize("approach", ["plain", "partitioning"])
+def test_host_resident_column(gj, orc, gpu, kind, approach):
+    # the column stays in page-locked host memory; the index's own structures and the
+    # probes are in HBM, and the kernels read column lines over the host link
+    keys = _column(gj, 100003, 8, seed=0x71)
+    probes = _probes(gj, keys, 30000, seed=0x72)
+    col = gj.HostArray.from_numpy(keys)
+    idx = gj.Index(kind, col, host_column=keys)
+    dp = gj.DeviceArray.from_numpy(probes)
+    out = gj.DeviceArray(probes.size, np.uint32)
+    idx.lookup(dp, out, approach=approach, radix_bits=8, window=10000)
+    gj.synchronize()
+    assert np.array_equal(out.numpy(), orc.lookup_lower_bound(keys, probes))
+    idx.close()
+    col.free()
