@@ -150,6 +150,16 @@ def main():
     del d_next, nxt, perm
     log(f"pointer chase: {chase_ns:.0f} ns per dependent access")
 
+    # random 8 B gathers from page-locked host memory (the reference's random_access
+    # setting: a zero-copy host array read over the interconnect)
+    host_arr = gj.HostArray.from_numpy(keys)
+    mh = min(m, 1 << 24)
+    per, _ = time_steps(lambda: gj.gather(host_arr, d_probes, d_out, mh, 256, 8, stream), 3, 1, stream, gj)
+    G_host = mh / (statistics.median(per) * 1e-3)
+    host_arr.free()
+    del host_arr
+    log(f"gather from host memory: {G_host / 1e9:.3f} G gathers/s")
+
     if args.suite:
         # the other workload configurations (one JSON object per line; not the headline run)
         from gpujoin_amd import suite
@@ -268,6 +278,7 @@ def main():
                         "source": "rocprofv3 FETCH_SIZE x2 + WRITE_SIZE (copy-calibrated), profiles/traffic.txt"},
         "stream_copy_GBps": copy_gbps,
         "pointer_chase_ns": chase_ns,
+        "host_gather_per_s": G_host,
     }
     print(json.dumps(line), flush=True)
 
