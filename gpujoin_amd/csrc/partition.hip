@@ -123,8 +123,8 @@ __global__ void part_pad_kernel(const uint64_t* __restrict__ start, const uint64
 // Stable scatter.  in_payload == nullptr: payload = payload_base + input position.
 // A block walks its chunk in tiles of kPartTile items held in registers (wave w
 // owns items [w * 1024, (w + 1) * 1024) of the tile, lane-minor).  Per tile:
-//   count   each wave counts its items per partition (LDS row of the wave; the
-//           lowest lane of each group of equal digits adds the group size)
+//   count   each wave counts its items per partition (LDS atomic adds into the
+//           wave's row)
 //   cursor  per partition: wave cursors = block cursor + counts of earlier waves
 //   scatter each wave writes its items at its cursors (rank among equal digits
 //           from ballots) and advances them; no ordering between waves is needed
@@ -175,12 +175,11 @@ __global__ __launch_bounds__(kPartBlock) void part_scatter_kernel(
       }
       return (i < hi) ? peers : 0ull;
     };
+    // counts need no ranks: one LDS atomic add per item
 #pragma unroll
     for (int j = 0; j < kPartItems; ++j) {
-      const uint32_t d = digit_of(k[j], shift, fan - 1);
-      const uint64_t peers = peers_of(j, d);
-      if (peers && (peers & lt) == 0) mine[d] += static_cast<uint32_t>(__popcll(peers));
-      __builtin_amdgcn_wave_barrier();  // keeps the wave's LDS accesses in program order
+      const uint64_t i = wbase + static_cast<uint64_t>(j * kWave + ln);
+      if (i < hi) atomicAdd(&mine[digit_of(k[j], shift, fan - 1)], 1u);
     }
     __syncthreads();
     // a cursor is written only where the wave has items of the partition: those
