@@ -61,8 +61,9 @@ void lookup_partitioned(const gj_index& ix, const K* probes, uint64_t m, uint32_
   uint64_t* count = reinterpret_cast<uint64_t*>(base + o_count);
   for (uint64_t w0 = 0; w0 < m; w0 += window) {
     const uint64_t w = std::min<uint64_t>(window, m - w0);
-    partition_keys<K>(probes + w0, w, static_cast<uint32_t>(w0), bits, ignore, passes, tkeys, tpay, start, count,
-                      base + o_part, ix.cu, s);
+    // unordered: the payload carries each probe's position back to `out`
+    partition_keys<K>(probes + w0, w, static_cast<uint32_t>(w0), bits, ignore, passes, false, tkeys, tpay, start,
+                      count, base + o_part, ix.cu, s);
     with_index_dev<K>(ix, [&](auto d) {
       launch_tuples(d, tkeys, tpay, start + fan, partition_capacity(w, sizeof(K), bits), out, p, s, ix.cu);
     });

@@ -127,13 +127,14 @@ template <class K>
 void build_hash_table(const gj_index& ix, dbuf& table, uint64_t& cap, hipStream_t s);
 uint64_t hash_capacity(uint64_t n);
 
-// Radix partitioner (partition.hip).
+// Radix partitioner (partition.hip).  stable: input order kept inside every
+// partition; otherwise the order inside a partition is arbitrary.
 uint64_t partition_capacity(uint64_t m, int key_bytes, uint32_t bits);
 uint64_t partition_scratch(uint64_t m, int key_bytes, uint32_t bits, uint32_t passes);
 template <class K>
 void partition_keys(const K* keys, uint64_t m, uint32_t payload_base, uint32_t bits, uint32_t ignore, uint32_t passes,
-                    K* out_keys, uint32_t* out_payload, uint64_t* start, uint64_t* count, uint8_t* scratch, int cu,
-                    hipStream_t s);
+                    bool stable, K* out_keys, uint32_t* out_payload, uint64_t* start, uint64_t* count, uint8_t* scratch,
+                    int cu, hipStream_t s);
 
 // Bytes of index-private device memory held by the extended kinds.
 uint64_t extended_memory(const gj_index& ix);

@@ -37,10 +37,10 @@ __device__ __forceinline__ uint32_t digit_of(K k, uint32_t shift, uint32_t mask)
   return static_cast<uint32_t>(static_cast<uint64_t>(k) >> shift) & mask;
 }
 
+// One block per chunk (256 or 1024 threads), four loads in flight per thread.
 template <class K>
-__global__ __launch_bounds__(kPartBlock) void part_hist_kernel(const K* __restrict__ keys, uint64_t m, uint64_t chunk,
-                                                               uint32_t shift, uint32_t bits,
-                                                               uint32_t* __restrict__ hist) {
+__global__ __launch_bounds__(1024) void part_hist_kernel(const K* __restrict__ keys, uint64_t m, uint64_t chunk,
+                                                         uint32_t shift, uint32_t bits, uint32_t* __restrict__ hist) {
   extern __shared__ uint32_t h[];
   const uint32_t fan = 1u << bits;
   for (uint32_t p = threadIdx.x; p < fan; p += blockDim.x) h[p] = 0;
@@ -48,7 +48,18 @@ __global__ __launch_bounds__(kPartBlock) void part_hist_kernel(const K* __restri
   const uint64_t c = xcd_logical_block();  // chunk index; the scatter maps blocks the same way
   const uint64_t lo = c * chunk;
   const uint64_t hi = std::min<uint64_t>(m, lo + chunk);
-  for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) atomicAdd(&h[digit_of(load_stream(keys + i), shift, fan - 1)], 1u);
+  constexpr int kIn = 4;
+  for (uint64_t i0 = lo + threadIdx.x; i0 < hi; i0 += kIn * blockDim.x) {
+    K v[kIn];
+#pragma unroll
+    for (int u = 0; u < kIn; ++u) {
+      const uint64_t i = i0 + static_cast<uint64_t>(u) * blockDim.x;
+      v[u] = i < hi ? load_stream(keys + i) : K(0);
+    }
+#pragma unroll
+    for (int u = 0; u < kIn; ++u)
+      if (i0 + static_cast<uint64_t>(u) * blockDim.x < hi) atomicAdd(&h[digit_of(v[u], shift, fan - 1)], 1u);
+  }
   __syncthreads();
   for (uint32_t p = threadIdx.x; p < fan; p += blockDim.x) hist[c * fan + p] = h[p];
 }
@@ -222,6 +233,299 @@ __global__ __launch_bounds__(kPartBlock) void part_scatter_kernel(
   }
 }
 
+// Unordered scatter for callers that carry each tuple's position in its payload
+// (the partitioned lookup driver): an item's slot is an LDS atomic add on the
+// block's cursor of its partition, so a tile needs no counts, ranks or barriers.
+// The block's items of partition p still fill exactly the run the histogram gave
+// the block, [start[p] + block_off[c][p], + hist[c][p]); only their order inside
+// it is arbitrary.
+template <class K>
+__global__ __launch_bounds__(kPartBlock) void part_scatter_any_kernel(
+    const K* __restrict__ keys, const uint32_t* __restrict__ in_payload, uint32_t payload_base, uint64_t m,
+    uint64_t chunk, uint32_t shift, uint32_t bits, const uint32_t* __restrict__ block_off,
+    const uint64_t* __restrict__ start, K* __restrict__ out_keys, uint32_t* __restrict__ out_payload, uint64_t cap) {
+  extern __shared__ uint32_t brun[];  // [fan]: block cursor (absolute position)
+  const uint32_t fan = 1u << bits;
+  const uint64_t c = xcd_logical_block();
+  for (uint32_t p = threadIdx.x; p < fan; p += blockDim.x)
+    brun[p] = static_cast<uint32_t>(start[p]) + block_off[c * fan + p];
+  __syncthreads();
+  const uint64_t lo = c * chunk;
+  const uint64_t hi = std::min<uint64_t>(m, lo + chunk);
+  for (uint64_t tile = lo; tile < hi; tile += kPartTile) {
+    K k[kPartItems];
+    uint32_t pay[kPartItems];
+#pragma unroll
+    for (int j = 0; j < kPartItems; ++j) {
+      const uint64_t i = tile + static_cast<uint64_t>(j * kPartBlock + threadIdx.x);
+      const bool valid = i < hi;
+      k[j] = valid ? load_stream(keys + i) : K(0);
+      pay[j] = valid ? (in_payload ? load_stream(in_payload + i) : payload_base + static_cast<uint32_t>(i)) : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < kPartItems; ++j) {
+      const uint64_t i = tile + static_cast<uint64_t>(j * kPartBlock + threadIdx.x);
+      if (i < hi) {
+        const uint32_t pos = atomicAdd(&brun[digit_of(k[j], shift, fan - 1)], 1u);
+        if (pos < cap) {  // always true for a consistent histogram; guards the buffers regardless
+          out_keys[pos] = k[j];
+          out_payload[pos] = pay[j];
+        }
+      }
+    }
+  }
+}
+
+// Write-combining scatter.  A random scatter into 2^bits runs leaves most output
+// lines partly written when L2 evicts them; each eviction then writes a partial
+// line, and a line is written back several times.  Here a block stages its items
+// in LDS and stores only whole aligned segments of L = 2^seg_bits slots (64 or
+// 128 bytes of keys), written by L consecutive lanes.
+//
+// LDS per block: a carry buffer of L slots per partition holding the items of the
+// partition's open segment, and the current tile of kWcTile items in partition
+// order.  Per tile:
+//   count   per partition (unordered: LDS atomic add, whose result is the item's
+//           rank in the tile; stable: per-wave rows as in part_scatter_kernel)
+//   scan    one block scan of (items, complete segments) per partition gives the
+//           tile offset of each partition and the list of segments to store
+//   place   items into the tile buffer at tile offset + rank
+//   store   each complete segment by L lanes: slots below the partition cursor
+//           come from the carry buffer, the others from the tile buffer
+//   carry   items past the last complete segment move into the carry buffer
+// At the end of the chunk the open segments are stored as they are.  The first
+// segment of a block's run is shared with the previous block: slots below the
+// run's first position (run0) are never stored by this block.
+constexpr int kWcTile = 2048;
+// Block size: the carry buffer fills most of the LDS, so one block runs per CU;
+// the unordered scatter uses 1024 threads (4 waves per SIMD) to keep loads in
+// flight, the stable one 256 (its per-wave count rows grow with the waves).
+constexpr int wc_block(bool stable) { return stable ? 256 : 1024; }
+
+__host__ __device__ constexpr uint32_t wc_max_segments(uint32_t fan, uint32_t seg_bits) {
+  // a tile completes at most one segment per partition beyond kWcTile / L
+  return (static_cast<uint32_t>(kWcTile) >> seg_bits) + (fan < static_cast<uint32_t>(kWcTile) ? fan : kWcTile);
+}
+
+__host__ __device__ constexpr size_t wc_lds_bytes(uint32_t fan, uint32_t seg_bits, int key_bytes, bool stable) {
+  const size_t item = static_cast<size_t>(key_bytes) + 4;
+  return static_cast<size_t>(fan) * (1u << seg_bits) * item      // carry keys + payloads
+         + static_cast<size_t>(kWcTile) * item                   // tile keys + payloads
+         + static_cast<size_t>(fan) * 4 * 4                      // cur, run0, off, endp
+         + static_cast<size_t>(fan) * 4 * (stable ? wc_block(true) / kWave : 1)  // counts
+         + (static_cast<size_t>(wc_max_segments(fan, seg_bits)) * 2 + 3) / 4 * 4;  // segment list
+}
+
+template <class K, bool Stable>
+__global__ __launch_bounds__(wc_block(Stable)) void part_scatter_wc_kernel(
+    const K* __restrict__ keys, const uint32_t* __restrict__ in_payload, uint32_t payload_base, uint64_t m,
+    uint64_t chunk, uint32_t shift, uint32_t bits, uint32_t seg_bits, const uint32_t* __restrict__ block_off,
+    const uint64_t* __restrict__ start, K* __restrict__ out_keys, uint32_t* __restrict__ out_payload, uint64_t cap) {
+  constexpr int kBlock = wc_block(Stable);
+  constexpr int kWaves = kBlock / kWave;
+  constexpr int kItems = kWcTile / kBlock;  // items per lane per tile
+  constexpr int kRows = Stable ? kWaves : 1;
+  extern __shared__ uint64_t wsm_[];
+  __shared__ uint32_t wsum[kWaves];
+  const uint32_t fan = 1u << bits;
+  const uint32_t L = 1u << seg_bits;
+  const uint32_t lmask = L - 1;
+  K* ck = reinterpret_cast<K*>(wsm_);              // [fan][L] carry keys
+  K* sk = ck + static_cast<size_t>(fan) * L;        // [kWcTile] tile keys, partition order
+  uint32_t* cp = reinterpret_cast<uint32_t*>(sk + kWcTile);  // [fan][L] carry payloads
+  uint32_t* sp = cp + static_cast<size_t>(fan) * L;  // [kWcTile] tile payloads
+  uint32_t* cur = sp + kWcTile;                      // [fan] next output position of the block's run
+  uint32_t* run0 = cur + fan;                        // [fan] first position of the block's run
+  uint32_t* off = run0 + fan;                        // [fan] tile offset | first segment << 16
+  uint32_t* endp = off + fan;                        // [fan] cursor after the tile
+  uint32_t* cnt = endp + fan;                        // [kRows][fan]
+  uint16_t* segp = reinterpret_cast<uint16_t*>(cnt + kRows * fan);  // segment -> partition
+
+  const uint64_t c = xcd_logical_block();
+  for (uint32_t p = threadIdx.x; p < fan; p += blockDim.x) {
+    const uint32_t r = static_cast<uint32_t>(start[p]) + block_off[c * fan + p];
+    cur[p] = r;
+    run0[p] = r;
+#pragma unroll
+    for (int v = 0; v < kRows; ++v) cnt[v * fan + p] = 0;
+  }
+  __syncthreads();
+  const uint64_t lo = c * chunk;
+  const uint64_t hi = std::min<uint64_t>(m, lo + chunk);
+  const int w = threadIdx.x / kWave;
+  const int ln = lane();
+  const uint64_t lt = (ln == 0) ? 0ull : (~0ull >> (64 - ln));
+  // partitions owned by this thread in the scan: [p0, p0 + per)
+  const uint32_t per = fan > static_cast<uint32_t>(kBlock) ? fan / kBlock : 1u;
+  const uint32_t p0 = threadIdx.x * per;
+  const uint32_t groups = static_cast<uint32_t>(kBlock) >> seg_bits;  // L lanes per segment store
+  const uint32_t g = threadIdx.x >> seg_bits;
+  const uint32_t q = threadIdx.x & lmask;
+
+  for (uint64_t tile = lo; tile < hi; tile += kWcTile) {
+    const uint64_t wbase = tile + static_cast<uint64_t>(w) * (kItems * kWave);
+    K k[kItems];
+    uint32_t pay[kItems];
+    uint32_t rk[kItems];
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+      const uint64_t i = wbase + static_cast<uint64_t>(j * kWave + ln);
+      const bool valid = i < hi;
+      k[j] = valid ? load_stream(keys + i) : K(0);
+      pay[j] = valid ? (in_payload ? load_stream(in_payload + i) : payload_base + static_cast<uint32_t>(i)) : 0u;
+    }
+    // count
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+      const uint64_t i = wbase + static_cast<uint64_t>(j * kWave + ln);
+      rk[j] = 0;
+      if (i < hi) {
+        const uint32_t d = digit_of(k[j], shift, fan - 1);
+        if constexpr (Stable)
+          atomicAdd(&cnt[w * fan + d], 1u);
+        else
+          rk[j] = atomicAdd(&cnt[d], 1u);
+      }
+    }
+    __syncthreads();
+    // scan: per partition v = items | complete segments << 16 (both sums stay below 2^16)
+    uint32_t local = 0;
+    for (uint32_t u = 0; u < per; ++u) {
+      const uint32_t p = p0 + u;
+      if (p < fan) {
+        uint32_t n = 0;
+#pragma unroll
+        for (int v = 0; v < kRows; ++v) n += cnt[v * fan + p];
+        const uint32_t a = cur[p] & ~lmask;
+        const uint32_t e = cur[p] + n;
+        endp[p] = e;
+        local += n | (((e & ~lmask) - a) >> seg_bits) << 16;
+      }
+    }
+    uint32_t x = local;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+      const uint32_t y = __shfl_up(x, o, kWave);
+      if (ln >= o) x += y;
+    }
+    if (ln == kWave - 1) wsum[w] = x;
+    __syncthreads();
+    uint32_t run = x - local;
+    uint32_t total = 0;
+#pragma unroll
+    for (int v = 0; v < kWaves; ++v) {
+      if (v < w) run += wsum[v];
+      total += wsum[v];
+    }
+    const uint32_t nseg_total = total >> 16;
+    for (uint32_t u = 0; u < per; ++u) {
+      const uint32_t p = p0 + u;
+      if (p < fan) {
+        off[p] = run;
+        uint32_t n = 0;
+        if constexpr (Stable) {
+          // wave rows become tile cursors: tile offset + items of earlier waves
+          uint32_t r = run & 0xFFFFu;
+#pragma unroll
+          for (int v = 0; v < kWaves; ++v) {
+            const uint32_t t = cnt[v * fan + p];
+            cnt[v * fan + p] = r;
+            r += t;
+          }
+          n = r - (run & 0xFFFFu);
+        } else {
+          n = cnt[p];
+        }
+        const uint32_t ns = ((endp[p] & ~lmask) - (cur[p] & ~lmask)) >> seg_bits;
+        for (uint32_t t = 0; t < ns; ++t) segp[(run >> 16) + t] = static_cast<uint16_t>(p);
+        run += n | (ns << 16);
+      }
+    }
+    __syncthreads();
+    // place
+    uint32_t pos[kItems];
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+      const uint64_t i = wbase + static_cast<uint64_t>(j * kWave + ln);
+      const bool valid = i < hi;
+      const uint32_t d = digit_of(k[j], shift, fan - 1);
+      uint32_t slot = 0;
+      if constexpr (Stable) {
+        uint64_t peers = ballot(valid);
+        for (uint32_t b = 0; b < bits; ++b) {
+          const uint64_t set = ballot((d >> b) & 1u);
+          peers &= ((d >> b) & 1u) ? set : ~set;
+        }
+        if (valid) {
+          const uint32_t at = cnt[w * fan + d];  // read by every lane of the group before the leader moves it
+          slot = at + static_cast<uint32_t>(__popcll(peers & lt));
+          if ((peers & lt) == 0) cnt[w * fan + d] = at + static_cast<uint32_t>(__popcll(peers));
+        }
+        __builtin_amdgcn_wave_barrier();
+      } else {
+        slot = (off[d] & 0xFFFFu) + rk[j];
+      }
+      pos[j] = 0xFFFFFFFFu;
+      if (valid && slot < static_cast<uint32_t>(kWcTile)) {  // slot < kWcTile for a consistent count
+        sk[slot] = k[j];
+        sp[slot] = pay[j];
+        pos[j] = cur[d] + slot - (off[d] & 0xFFFFu);
+      }
+    }
+    __syncthreads();
+    // store the complete segments, L lanes each
+    for (uint32_t sg = g; sg < nseg_total; sg += groups) {
+      const uint32_t p = segp[sg];
+      const uint32_t o = off[p];
+      const uint32_t cp0 = cur[p];
+      const uint32_t at = (cp0 & ~lmask) + ((sg - (o >> 16)) << seg_bits) + q;
+      if (at >= run0[p] && at < cap) {
+        K kk;
+        uint32_t pp;
+        if (at < cp0) {
+          kk = ck[p * L + (at & lmask)];
+          pp = cp[p * L + (at & lmask)];
+        } else {
+          const uint32_t t = (o & 0xFFFFu) + (at - cp0);
+          kk = sk[t];
+          pp = sp[t];
+        }
+        out_keys[at] = kk;
+        out_payload[at] = pp;
+      }
+    }
+    __syncthreads();
+    // carry the items of each open segment; cursors move to the tile's end
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+      if (pos[j] != 0xFFFFFFFFu) {
+        const uint32_t d = digit_of(k[j], shift, fan - 1);
+        if (pos[j] >= (endp[d] & ~lmask)) {
+          ck[d * L + (pos[j] & lmask)] = k[j];
+          cp[d * L + (pos[j] & lmask)] = pay[j];
+        }
+      }
+    }
+    for (uint32_t u = 0; u < per; ++u) {
+      const uint32_t p = p0 + u;
+      if (p < fan) {
+        cur[p] = endp[p];
+#pragma unroll
+        for (int v = 0; v < kRows; ++v) cnt[v * fan + p] = 0;
+      }
+    }
+    __syncthreads();
+  }
+  // open segments: the slots of the block's run in [segment start, cursor)
+  for (uint32_t p = g; p < fan; p += groups) {
+    const uint32_t at = (cur[p] & ~lmask) + q;
+    if (at >= run0[p] && at < cur[p] && at < cap) {
+      out_keys[at] = ck[p * L + q];
+      out_payload[at] = cp[p * L + q];
+    }
+  }
+}
+
 namespace {
 
 struct PartPlan {
@@ -229,12 +533,15 @@ struct PartPlan {
   uint64_t chunk;
 };
 
-PartPlan plan(uint64_t m, uint32_t bits, int cu) {
+constexpr uint32_t kMaxBlocksPerCu = 4;
+constexpr size_t kMaxLds = 160 * 1024;
+
+// Whole tiles per block, about per_cu blocks per CU; each block's histogram row
+// lives in scratch.
+PartPlan plan(uint64_t m, uint32_t bits, int cu, uint32_t per_cu) {
   PartPlan p;
   p.fan = 1u << bits;
-  // whole tiles per block; the scatter's LDS (5 x 4 B per partition) allows two
-  // blocks per CU at fan-out 4096, more below; each block's histogram row lives in scratch
-  const uint64_t want = static_cast<uint64_t>(cu) * (bits > 10 ? 2 : 4);
+  const uint64_t want = static_cast<uint64_t>(cu) * per_cu;
   uint64_t chunk = std::max<uint64_t>(kPartTile, ceil_div(m, want));
   chunk = ceil_div(chunk, kPartTile) * kPartTile;
   p.chunk = chunk;
@@ -242,10 +549,25 @@ PartPlan plan(uint64_t m, uint32_t bits, int cu) {
   return p;
 }
 
+// Histogram rows for the plan with the most blocks any pass uses.
+uint64_t hist_bytes(uint64_t m, uint32_t bits, int cu) {
+  const PartPlan p = plan(m, bits, cu, kMaxBlocksPerCu);
+  return ceil_div(static_cast<uint64_t>(p.blocks) * p.fan * 4 + 256, 256) * 256;
+}
+
+// Segment size (log2 of slots) of the write-combining scatter: the largest with
+// segments of at most 128 bytes of keys whose LDS fits one block per CU; 0 when
+// not even 4-slot segments fit (fan-out 2048 and more).
+uint32_t wc_seg_bits(uint32_t bits, int key_bytes, bool stable) {
+  const uint32_t fan = 1u << bits;
+  for (uint32_t sb = key_bytes == 4 ? 5u : 4u; sb >= 2; --sb)
+    if (wc_lds_bytes(fan, sb, key_bytes, stable) <= kMaxLds) return sb;
+  return 0;
+}
+
 // histograms | (2 passes: first-pass keys and payloads) | partition starts for gj_partition
 uint64_t scratch_for(uint64_t m, int key_bytes, uint32_t bits, uint32_t passes, int cu) {
-  const PartPlan p = plan(m, bits, cu);
-  uint64_t b = static_cast<uint64_t>(p.blocks) * p.fan * 4 + 256;
+  uint64_t b = hist_bytes(m, bits, cu);
   if (passes == 2) b += (m + 32) * (key_bytes + 4) + 512;  // first-pass keys + payloads
   b += ((1ull << bits) + 2) * 8 + 16;
   return b;
@@ -253,16 +575,25 @@ uint64_t scratch_for(uint64_t m, int key_bytes, uint32_t bits, uint32_t passes, 
 
 }  // namespace
 
-// One stable partition pass with digit (k >> shift) & (2^bits - 1).  `align`
-// pads partition starts (1 = unpadded).  Writes start[fan] and count[fan].
+// One partition pass with digit (k >> shift) & (2^bits - 1), stable (input order
+// inside every partition) or unordered.  `align` pads partition starts (1 =
+// unpadded).  Writes start[fan] and count[fan].
 template <class K>
 void partition_pass(const K* keys, const uint32_t* in_payload, uint32_t payload_base, uint64_t m, uint32_t shift,
-                    uint32_t bits, uint32_t align, K* out_keys, uint32_t* out_payload, uint64_t* start,
+                    uint32_t bits, uint32_t align, bool stable, K* out_keys, uint32_t* out_payload, uint64_t* start,
                     uint64_t* count, uint32_t* hist, int cu, hipStream_t s) {
-  const PartPlan p = plan(m, bits, cu);
+  const uint32_t sb = wc_seg_bits(bits, sizeof(K), stable);
+  size_t slds = 0;
+  uint32_t per_cu = bits > 10 ? 2 : 4;  // the register-tile scatter holds 5 x 4 B per partition
+  if (sb) {
+    slds = wc_lds_bytes(1u << bits, sb, sizeof(K), stable);
+    per_cu = static_cast<uint32_t>(std::min<size_t>(kMaxBlocksPerCu, std::max<size_t>(1, kMaxLds / slds)));
+  }
+  const PartPlan p = plan(m, bits, cu, per_cu);
   const size_t lds = p.fan * sizeof(uint32_t);
-  hipLaunchKernelGGL(part_hist_kernel<K>, dim3(p.blocks), dim3(kPartBlock), lds, s, keys, m, p.chunk, shift, bits,
-                     hist);
+  // with one or two chunks per CU a chunk gets 1024 threads
+  hipLaunchKernelGGL(part_hist_kernel<K>, dim3(p.blocks), dim3(per_cu <= 2 ? 1024 : kPartBlock), lds, s, keys, m,
+                     p.chunk, shift, bits, hist);
   GJ_HIP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(part_colscan_kernel, dim3((p.fan + 3) / 4), dim3(256), 0, s, hist, p.blocks, p.fan, count);
   GJ_HIP_CHECK(hipGetLastError());
@@ -273,13 +604,34 @@ void partition_pass(const K* keys, const uint32_t* in_payload, uint32_t payload_
                        out_payload);
     GJ_HIP_CHECK(hipGetLastError());
   }
-  const size_t slds = static_cast<size_t>(kPartWaves + 1) * p.fan * sizeof(uint32_t);  // <= 80 KiB of the 160
-  if (slds > 65536)
+  const uint64_t cap = m + (align > 1 ? static_cast<uint64_t>(p.fan) * align : 0ull);
+  if (sb) {
+    auto launch = [&](auto kernel) {
+      if (slds > 65536)
+        GJ_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(slds)));
+      hipLaunchKernelGGL(kernel, dim3(p.blocks), dim3(wc_block(stable)), slds, s, keys, in_payload, payload_base, m,
+                         p.chunk, shift, bits, sb, hist, start, out_keys, out_payload, cap);
+      GJ_HIP_CHECK(hipGetLastError());
+    };
+    if (stable)
+      launch(&part_scatter_wc_kernel<K, true>);
+    else
+      launch(&part_scatter_wc_kernel<K, false>);
+    return;
+  }
+  if (!stable) {
+    hipLaunchKernelGGL(part_scatter_any_kernel<K>, dim3(p.blocks), dim3(kPartBlock), lds, s, keys, in_payload,
+                       payload_base, m, p.chunk, shift, bits, hist, start, out_keys, out_payload, cap);
+    GJ_HIP_CHECK(hipGetLastError());
+    return;
+  }
+  const size_t rlds = static_cast<size_t>(kPartWaves + 1) * p.fan * sizeof(uint32_t);  // <= 80 KiB of the 160
+  if (rlds > 65536)
     GJ_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&part_scatter_kernel<K>),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(slds)));
-  hipLaunchKernelGGL(part_scatter_kernel<K>, dim3(p.blocks), dim3(kPartBlock), slds, s, keys, in_payload, payload_base,
-                     m, p.chunk, shift, bits, hist, start, out_keys, out_payload,
-                     m + (align > 1 ? static_cast<uint64_t>(p.fan) * align : 0ull));
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(rlds)));
+  hipLaunchKernelGGL(part_scatter_kernel<K>, dim3(p.blocks), dim3(kPartBlock), rlds, s, keys, in_payload, payload_base,
+                     m, p.chunk, shift, bits, hist, start, out_keys, out_payload, cap);
   GJ_HIP_CHECK(hipGetLastError());
 }
 
@@ -295,28 +647,28 @@ uint64_t partition_scratch(uint64_t m, int key_bytes, uint32_t bits, uint32_t pa
 // start must hold 2^bits + 1 entries (start[2^bits] = padded total).
 template <class K>
 void partition_keys(const K* keys, uint64_t m, uint32_t payload_base, uint32_t bits, uint32_t ignore, uint32_t passes,
-                    K* out_keys, uint32_t* out_payload, uint64_t* start, uint64_t* count, uint8_t* scratch, int cu,
-                    hipStream_t s) {
-  const PartPlan p = plan(m, bits, cu);
+                    bool stable, K* out_keys, uint32_t* out_payload, uint64_t* start, uint64_t* count, uint8_t* scratch,
+                    int cu, hipStream_t s) {
   uint32_t* hist = reinterpret_cast<uint32_t*>(scratch);
   const uint32_t align = part_align(sizeof(K));
   if (passes == 1 || bits < 2) {
-    partition_pass<K>(keys, nullptr, payload_base, m, ignore, bits, align, out_keys, out_payload, start, count, hist,
-                      cu, s);
+    partition_pass<K>(keys, nullptr, payload_base, m, ignore, bits, align, stable, out_keys, out_payload, start, count,
+                      hist, cu, s);
     return;
   }
   // pass 1: the high half of the digit, unpadded, into scratch; pass 2: the full digit.
   const uint32_t b_hi = (bits + 1) / 2;
-  uint8_t* tmp = scratch + ceil_div(static_cast<uint64_t>(p.blocks) * p.fan * 4 + 256, 256) * 256;
+  uint8_t* tmp = scratch + hist_bytes(m, bits, cu);
   K* tk = reinterpret_cast<K*>(tmp);
   uint32_t* tp = reinterpret_cast<uint32_t*>(tmp + ceil_div((m + 32) * sizeof(K), 256) * 256);
-  partition_pass<K>(keys, nullptr, payload_base, m, ignore + (bits - b_hi), b_hi, 1, tk, tp, start, count, hist, cu, s);
-  partition_pass<K>(tk, tp, 0, m, ignore, bits, align, out_keys, out_payload, start, count, hist, cu, s);
+  partition_pass<K>(keys, nullptr, payload_base, m, ignore + (bits - b_hi), b_hi, 1, stable, tk, tp, start, count,
+                    hist, cu, s);
+  partition_pass<K>(tk, tp, 0, m, ignore, bits, align, stable, out_keys, out_payload, start, count, hist, cu, s);
 }
-template void partition_keys<uint32_t>(const uint32_t*, uint64_t, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t*,
-                                       uint32_t*, uint64_t*, uint64_t*, uint8_t*, int, hipStream_t);
-template void partition_keys<uint64_t>(const uint64_t*, uint64_t, uint32_t, uint32_t, uint32_t, uint32_t, uint64_t*,
-                                       uint32_t*, uint64_t*, uint64_t*, uint8_t*, int, hipStream_t);
+template void partition_keys<uint32_t>(const uint32_t*, uint64_t, uint32_t, uint32_t, uint32_t, uint32_t, bool,
+                                       uint32_t*, uint32_t*, uint64_t*, uint64_t*, uint8_t*, int, hipStream_t);
+template void partition_keys<uint64_t>(const uint64_t*, uint64_t, uint32_t, uint32_t, uint32_t, uint32_t, bool,
+                                       uint64_t*, uint32_t*, uint64_t*, uint64_t*, uint8_t*, int, hipStream_t);
 
 }  // namespace gj
 
@@ -366,7 +718,7 @@ int gj_partition(const void* d_keys, uint64_t m, int key_bytes, const gj_partiti
     uint64_t* start = reinterpret_cast<uint64_t*>(sc + need - ((1ull << bits) + 1) * 8 - 8);
     start = reinterpret_cast<uint64_t*>(reinterpret_cast<uintptr_t>(start) & ~static_cast<uintptr_t>(7));
     dispatch_key(key_bytes, [&]<class K>() {
-      partition_keys<K>(static_cast<const K*>(d_keys), m, 0, bits, params->ignore_bits, params->passes,
+      partition_keys<K>(static_cast<const K*>(d_keys), m, 0, bits, params->ignore_bits, params->passes, true,
                         static_cast<K*>(out->d_keys), out->d_payloads, start, out->d_counts, sc, cu, s);
     });
     GJ_HIP_CHECK(hipMemcpyAsync(out->d_offsets, start, (1ull << bits) * 8, hipMemcpyDeviceToDevice, s));
