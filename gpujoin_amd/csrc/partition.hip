@@ -556,12 +556,17 @@ uint64_t hist_bytes(uint64_t m, uint32_t bits, int cu) {
 }
 
 // Segment size (log2 of slots) of the write-combining scatter: the largest with
-// segments of at most 128 bytes of keys whose LDS fits one block per CU; 0 when
-// not even 4-slot segments fit (fan-out 2048 and more).
+// segments of at most 128 bytes of keys whose LDS fits the block; 0 selects the
+// register-tile scatter.  The unordered scatter runs 1024 threads in one block
+// per CU and fits up to fan-out 1024.  The stable one runs 256 threads and pays
+// off only with two blocks per CU (fan-out up to 128 for u64 keys); with one it
+// is slower than the register-tile scatter.
 uint32_t wc_seg_bits(uint32_t bits, int key_bytes, bool stable) {
   const uint32_t fan = 1u << bits;
-  for (uint32_t sb = key_bytes == 4 ? 5u : 4u; sb >= 2; --sb)
-    if (wc_lds_bytes(fan, sb, key_bytes, stable) <= kMaxLds) return sb;
+  const uint32_t sb = key_bytes == 4 ? 5u : 4u;
+  if (stable) return wc_lds_bytes(fan, sb, key_bytes, true) <= kMaxLds / 2 ? sb : 0u;
+  for (uint32_t b = sb; b >= 2; --b)
+    if (wc_lds_bytes(fan, b, key_bytes, false) <= kMaxLds) return b;
   return 0;
 }
 

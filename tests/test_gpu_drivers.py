@@ -47,10 +47,11 @@ def test_partition_matches_oracle(gj, orc, gpu, kb, bits, passes, m):
         assert np.all(gp[o + c:pad_end] == 0xFFFFFFFF)
 
 
-@pytest.mark.parametrize("bits,passes", [(10, 1), (12, 1), (12, 2)])
+@pytest.mark.parametrize("bits,passes", [(6, 1), (7, 1), (10, 1), (12, 1), (12, 2)])
 def test_partition_many_tiles_per_block(gj, orc, gpu, bits, passes):
-    # 5e6 keys: every block of the scatter walks several 4096-item register tiles,
-    # and the last tile of the last block is partial
+    # 5e6 keys: every block of the scatter walks several tiles (write-combining
+    # scatter up to 7 bits, register tiles above), and the last tile of the last
+    # block is partial
     m = 5_000_003
     print(f"[case] partition many tiles bits={bits} passes={passes} m={m}", flush=True)
     rng = np.random.default_rng(bits + 17 * passes)
@@ -107,6 +108,22 @@ def test_approaches_match_oracle(gj, orc, gpu, kind, kb, approach, extra):
     if kind == "no_op" and approach != "hj":
         want = np.full_like(want, gj.NOT_FOUND)
     assert np.array_equal(out.numpy(), want)
+
+
+@pytest.mark.parametrize("kb,bits", [(8, 8), (8, 10), (4, 10), (4, 11)])
+def test_partitioned_lookup_many_tiles(gj, orc, gpu, kb, bits):
+    # 3e6 probes in one window: each block of the unordered write-combining scatter
+    # walks several tiles and carries open segments between them (u32 keys at
+    # fan-out 2048 use 4-slot segments)
+    print(f"[case] partitioned lookup many tiles kb={kb} bits={bits}", flush=True)
+    keys, col, idx = _setup(gj, "btree", 1 << 20, kb)
+    probes, _ = gj.gen_probes(keys, 3_000_017, _lib.PROBES_UNIFORM, 1.0, 0.05, 0, 0x24 + bits)
+    dp = gj.DeviceArray.from_numpy(probes)
+    out = gj.DeviceArray(probes.size, np.uint32)
+    out.fill(0x5A)
+    idx.lookup(dp, out, approach="partitioning", radix_bits=bits, window=1 << 22)
+    gj.synchronize()
+    assert np.array_equal(out.numpy(), orc.lookup_lower_bound(keys, probes))
 
 
 def test_partitioned_zipf_probes(gj, orc, gpu):
