@@ -79,10 +79,12 @@ __device__ __forceinline__ uint32_t xcd_logical_block() {
 // Lookup over a partitioned relation: tuple t has key keys[t] and original
 // probe index payload[t]; the row id is scattered back to the original
 // position (partitioned_lookup_kernel, src/index_lookup_partitioning.cu:218-248).
-// Padding slots carry payload 0xFFFFFFFF.  Each workgroup takes one contiguous
-// chunk of the tuples, XCD-major, so neighbouring partitions (neighbouring key
-// ranges, hence the same index nodes and column lines) share an L2.  The tuple
-// count is read from *d_total when given (it is known only on the device).
+// Padding slots carry payload 0xFFFFFFFF.  Each XCD takes one contiguous eighth
+// of the tuples, and the workgroups of an XCD sweep it together, one
+// workgroup-wide step apart, so the key range in flight on an XCD (hence the
+// index nodes and column lines its L2 must hold) spans a few partitions rather
+// than one partition per workgroup.  The tuple count is read from *d_total when
+// given (it is known only on the device).
 template <class Dev, class K>
 __global__ __launch_bounds__(256) void lookup_tuples_kernel(Dev d, const K* __restrict__ keys,
                                                             const uint32_t* __restrict__ payload,
@@ -93,12 +95,15 @@ __global__ __launch_bounds__(256) void lookup_tuples_kernel(Dev d, const K* __re
   __syncthreads();
   uint64_t total = total_max;
   if (d_total) total = *d_total < total_max ? *d_total : total_max;
-  const uint64_t chunk = ((total + gridDim.x - 1) / gridDim.x + kWave - 1) / kWave * kWave;
-  const uint64_t lo = static_cast<uint64_t>(xcd_logical_block()) * chunk;
+  const uint32_t groups = (gridDim.x % 8 == 0) ? 8u : 1u;  // XCDs
+  const uint32_t per = gridDim.x / groups;                 // workgroups per XCD
+  const uint64_t chunk = ((total + groups - 1) / groups + kWave - 1) / kWave * kWave;
+  const uint64_t lo = static_cast<uint64_t>(blockIdx.x % groups) * chunk;
   const uint64_t hi = (lo + chunk < total) ? lo + chunk : total;
-  const uint64_t waves = blockDim.x / kWave;
   const uint64_t w = threadIdx.x / kWave;
-  for (uint64_t base = lo + w * kWave; base < hi; base += waves * kWave) {
+  const uint64_t step = static_cast<uint64_t>(per) * blockDim.x;
+  for (uint64_t base = lo + static_cast<uint64_t>(blockIdx.x / groups) * blockDim.x + w * kWave; base < hi;
+       base += step) {
     const uint64_t i = base + static_cast<uint64_t>(lane());
     uint32_t dst = 0xFFFFFFFFu;
     K x = K(0);
