@@ -49,7 +49,7 @@ void gj_lookup_params_default(gj_lookup_params* p) {
   p->radix_bits = 10;
   p->passes = 1;
   p->ignore_bits = 0xFFFFFFFFu;
-  p->window = 1ull << 22;
+  p->window = 1ull << 24;
 }
 
 int gj_parse_index_type(const char* name, gj_index_kind* out) {

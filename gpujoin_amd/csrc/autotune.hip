@@ -35,7 +35,7 @@ extern "C" int gj_autotune_lookup(const gj_index* index, const void* d_probe_sam
       cand.push_back(p);
     }
     for (uint32_t bits : {8u, 10u, 12u}) {
-      for (uint64_t window : {1ull << 20, 1ull << 22, 1ull << 24}) {
+      for (uint64_t window : {1ull << 20, 1ull << 22, 1ull << 24, 1ull << 26}) {
         if (window > 4 * m && window != (1ull << 20)) continue;
         for (uint32_t passes : {1u, 2u}) {
           if (passes == 2 && bits < 10) continue;

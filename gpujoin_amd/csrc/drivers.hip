@@ -41,7 +41,7 @@ void lookup_partitioned(const gj_index& ix, const K* probes, uint64_t m, uint32_
     ignore = w > bits ? w - bits : 0u;
   }
   GJ_REQUIRE(ignore < 8 * sizeof(K), GJ_ERR_INVALID, "ignore_bits must be below the key width");
-  const uint64_t window = std::min<uint64_t>(p.window ? p.window : (1ull << 22), m);
+  const uint64_t window = std::min<uint64_t>(p.window ? p.window : (1ull << 24), m);
   const uint64_t fan = 1ull << bits;
   const uint64_t cap = partition_capacity(window, sizeof(K), bits);
   GJ_REQUIRE(cap < 0xFFFFFFFFull, GJ_ERR_OVERFLOW, "window too large");

@@ -10,7 +10,7 @@ Writes one JSON object per measurement to stdout-compatible lines via `emit`.
   sorted / dense probes in key order; dense keys (key = row id) as in the reference's scripts
   lookup plans   uneven probe batches through launches, a hipGraph and the persistent kernel
   host column    the index column in page-locked host memory, plain vs partitioned
-  partitioned    fan-out 2^8 / 2^10 / 2^12, 1 and 2 passes, windows 2^16..2^22;
+  partitioned    fan-out 2^8 / 2^10 / 2^12, 1 and 2 passes, windows 2^18..2^26;
                  index-size sweep 2^20..2^30 keys (B-tree and learned)
   learned        RadixSpline err 8/32/64 and the PLR / MLP models over uniform,
                  lognormal and clustered keys, 10 % misses; B-tree fallbacks counted
@@ -161,7 +161,7 @@ def run(gj, G, quick=False, cpu=True, emit=None):
     for kind in ("btree", "learned_plr"):
         idx = gj.Index(kind, dk, host_column=keys, stream=stream)
         for bits, passes in ((8, 1), (10, 1), (12, 1), (12, 2)):
-            for wlog in ((16, 18, 20, 22) if not quick else (18, 22)):
+            for wlog in ((18, 20, 22, 24, 26) if not quick else (20, 24)):
                 lp = gj.lookup_params("partitioning", radix_bits=bits, passes=passes, window=1 << wlog)
                 ms = _time(gj, lambda: idx.lookup(dp, out, params=lp, stream=stream), stream, steps=3, warmup=1)
                 emit({"config": f"partitioned probe: 2^{n_log} u64 keys, 2^{m_log} uniform probes", "index": kind,

@@ -105,7 +105,7 @@ typedef struct gj_lookup_params {
   uint32_t passes;           /* partitioning: 1 or 2 (default 1) */
   uint32_t ignore_bits;      /* partitioning: low key bits skipped; 0xFFFFFFFF = derive from the max key
                                 (partitioning_approach_dynamic_bit_range, src/index_lookup_new.cu:63-67) */
-  uint64_t window;           /* partitioning: probes per window (default 2^22) */
+  uint64_t window;           /* partitioning: probes per window (default 2^24) */
 } gj_lookup_params;
 
 void gj_lookup_params_default(gj_lookup_params* p);
