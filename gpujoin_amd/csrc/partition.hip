@@ -8,15 +8,20 @@
 // partition starts on a 128-byte boundary (the reference pads partitions by
 // GPU_CACHE_LINE_SIZE, src/partitioned_relation.hpp:7-10); payload = the
 // tuple's position in the input.  Structure of arrays instead of Tuple{key,
-// payload}, and a stable order inside each partition.
+// payload}.  gj_partition keeps the input order inside each partition; the
+// partitioned lookup driver lets it float (the payload carries the position).
 //
 // Kernels (no cooperative launch, no device-wide barrier):
 //   hist     one block per contiguous chunk, LDS histogram of 2^radix_bits u32 bins
 //   colscan  one wave per partition: column prefix over the blocks and the partition total
 //   starts   one block: aligned exclusive prefix over partitions -> partition starts
-//   scatter  one block per chunk, in register tiles of 4096 items; per tile each wave
-//            counts its items per partition, wave cursors follow from the counts, and
-//            each wave writes its items in order (ranks from ballots)
+//   scatter  one block per chunk (the same chunks as hist), three variants:
+//            write-combining (LDS carry buffers, whole aligned segments; unordered,
+//              or stable where two blocks fit per CU)
+//            register tiles with per-wave cursors (stable; fan-outs the
+//              write-combining buffers do not fit)
+//            one store per item at an LDS-atomic slot (unordered, where the
+//              write-combining buffers do not fit)
 #include <algorithm>
 
 #include "common.hpp"
