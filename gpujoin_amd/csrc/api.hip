@@ -104,6 +104,13 @@ int gj_index_build(gj_index_kind kind, const void* h_sorted_keys, const void* d_
     if (params) ix->params = *params;
     fill_defaults(ix->params);
     ix->cu = device_cu_count();
+    if (d_sorted_keys) {
+      hipPointerAttribute_t attr{};
+      if (hipPointerGetAttributes(&attr, d_sorted_keys) == hipSuccess)
+        ix->col_in_host = attr.type == hipMemoryTypeHost;
+      else
+        (void)hipGetLastError();  // memory HIP does not know: treated as device memory
+    }
     const hipStream_t s = S(stream);
     if (d_sorted_keys) {
       ix->min_key = 0;

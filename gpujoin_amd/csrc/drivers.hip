@@ -59,13 +59,18 @@ void lookup_partitioned(const gj_index& ix, const K* probes, uint64_t m, uint32_
   uint32_t* tpay = reinterpret_cast<uint32_t*>(base + o_pay);
   uint64_t* start = reinterpret_cast<uint64_t*>(base + o_start);
   uint64_t* count = reinterpret_cast<uint64_t*>(base + o_count);
+  // a column in host memory: one workgroup per CU for the lookup, so each XCD
+  // sweeps a narrow key range and the host pages in flight stay within reach of
+  // the GPU's address translation (2x on a 2^26-key column)
+  gj_lookup_params tp = p;
+  if (ix.col_in_host && !tp.grid_blocks) tp.grid_blocks = static_cast<uint32_t>(ix.cu);
   for (uint64_t w0 = 0; w0 < m; w0 += window) {
     const uint64_t w = std::min<uint64_t>(window, m - w0);
     // unordered: the payload carries each probe's position back to `out`
     partition_keys<K>(probes + w0, w, static_cast<uint32_t>(w0), bits, ignore, passes, false, tkeys, tpay, start,
                       count, base + o_part, ix.cu, s);
     with_index_dev<K>(ix, [&](auto d) {
-      launch_tuples(d, tkeys, tpay, start + fan, partition_capacity(w, sizeof(K), bits), out, p, s, ix.cu);
+      launch_tuples(d, tkeys, tpay, start + fan, partition_capacity(w, sizeof(K), bits), out, tp, s, ix.cu);
     });
   }
 }

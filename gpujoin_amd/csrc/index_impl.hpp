@@ -33,6 +33,7 @@ struct gj_index {
   gj_index_params params{};
   uint64_t min_key = 0, max_key = 0;
   int cu = 0;
+  bool col_in_host = false;  // the column is page-locked host memory (read over the host link)
 
   // binary search: bucket maxima staged in LDS
   gj::dbuf samples;

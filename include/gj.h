@@ -100,7 +100,9 @@ void gj_index_params_default(gj_index_params* p);
 typedef struct gj_lookup_params {
   gj_approach approach;
   uint32_t block_size;       /* threads per block, default 256 */
-  uint32_t grid_blocks;      /* 0 = enough to cover the input (plain) or 4 x CUs (persistent kinds) */
+  uint32_t grid_blocks;      /* 0 = enough to cover the input (plain) or 4 x CUs (persistent kinds);
+                                the partitioned driver's lookup then uses one workgroup per CU
+                                when the column is page-locked host memory */
   uint32_t radix_bits;       /* partitioning: log2 fan-out, 1..12 (default 10) */
   uint32_t passes;           /* partitioning: 1 or 2 (default 1) */
   uint32_t ignore_bits;      /* partitioning: low key bits skipped; 0xFFFFFFFF = derive from the max key
