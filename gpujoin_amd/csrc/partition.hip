@@ -302,9 +302,10 @@ __global__ __launch_bounds__(kPartBlock) void part_scatter_any_kernel(
 // segment of a block's run is shared with the previous block: slots below the
 // run's first position (run0) are never stored by this block.
 constexpr int kWcTile = 2048;
-// Block size: the carry buffer fills most of the LDS, so one block runs per CU;
-// the unordered scatter uses 1024 threads (4 waves per SIMD) to keep loads in
-// flight, the stable one 256 (its per-wave count rows grow with the waves).
+// Block size: at the larger fan-outs the carry buffer fills most of the LDS and
+// one block runs per CU, so the unordered scatter uses 1024 threads (4 waves per
+// SIMD) to keep loads in flight; the stable one uses 256 (its per-wave count rows
+// grow with the waves) and runs only where two blocks fit per CU.
 constexpr int wc_block(bool stable) { return stable ? 256 : 1024; }
 
 __host__ __device__ constexpr uint32_t wc_max_segments(uint32_t fan, uint32_t seg_bits) {
