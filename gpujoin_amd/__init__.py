@@ -25,7 +25,7 @@ __all__ = [
     "DeviceArray", "Stream", "Event", "Index", "NOT_FOUND", "GjError", "gen_build_keys", "gen_probes",
     "gen_dimension", "gen_fact", "sort_dimension", "device_info", "gather", "pointer_chase", "stream_copy",
     "index_params", "lookup_params", "partition", "join_aggregate", "promo_revenue", "LookupPlan",
-    "AggregatePlan", "mlp_loss_grad", "mlp_forward", "autotune",
+    "AggregatePlan", "mlp_loss_grad", "mlp_forward", "autotune", "autotune_index",
 ]
 
 
@@ -406,6 +406,23 @@ def autotune(index: Index, probes: DeviceArray, scratch: DeviceArray, m: Optiona
     mm = probes.count if m is None else int(m)
     check(_lib.call("gj_autotune_lookup", index.handle, probes.ptr, mm, scratch.ptr, C.byref(best), C.byref(rate),
                     _s(stream)))
+    return best, rate.value
+
+
+def autotune_index(kind, column: DeviceArray, probes: DeviceArray, scratch: DeviceArray,
+                   host_column: Optional[np.ndarray] = None, base: Optional[IndexParams] = None,
+                   m: Optional[int] = None, n: Optional[int] = None, stream: Optional[Stream] = None):
+    """Index parameters (threads per lookup, LDS levels, radix bits, error bounds, leaves) of
+    the fastest candidate build of `kind` over `column` (gj_autotune_index); returns (params, rate)."""
+    k = INDEX_NAMES[kind] if isinstance(kind, str) else int(kind)
+    best = IndexParams()
+    rate = C.c_double()
+    nn = column.count if n is None else int(n)
+    mm = probes.count if m is None else int(m)
+    hp = _ptr(host_column) if host_column is not None else C.c_void_p()
+    bp = C.byref(base) if base is not None else None
+    check(_lib.call("gj_autotune_index", k, hp, column.ptr, nn, column.dtype.itemsize, bp, probes.ptr, mm,
+                    scratch.ptr, C.byref(best), C.byref(rate), _s(stream)))
     return best, rate.value
 
 

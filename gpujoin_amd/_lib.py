@@ -122,6 +122,8 @@ SIGNATURES = {
     "gj_batch_plan_run": (_I, [_P, _P]),
     "gj_batch_plan_destroy": (None, [_P]),
     "gj_autotune_lookup": (_I, [_P, _P, _U64, _P, C.POINTER(LookupParams), C.POINTER(C.c_double), _P]),
+    "gj_autotune_index": (_I, [_I, _P, _P, _U64, _I, C.POINTER(IndexParams), _P, _U64, _P, C.POINTER(IndexParams),
+                               C.POINTER(C.c_double), _P]),
     "gj_mlp_loss_grad": (_I, [_P, _U32, _P, _P, _U64, _P, _P, _P]),
     "gj_mlp_forward": (_I, [_P, _U32, _P, _U64, _P, _P]),
     "gj_gather": (_I, [_P, _U64, _P, _U64, _P, _U32, _U32, _P]),

@@ -83,3 +83,109 @@ extern "C" int gj_autotune_lookup(const gj_index* index, const void* d_probe_sam
     if (best_lookups_per_s) *best_lookups_per_s = static_cast<double>(m) / (best_ms * 1e-3);
   });
 }
+
+// ---- index autotuner -------------------------------------------------------
+// Picks the index's own parameters: threads per lookup (Harmonia's lanes per
+// node search), the LDS-resident part of the B-tree, RadixSpline's radix bits
+// and error bound, the learned index's leaf count and error bound.  Each
+// candidate is built over the caller's column and timed with the plain driver
+// on the probe sample; the candidates are destroyed again.
+namespace {
+
+std::vector<gj_index_params> index_candidates(gj_index_kind kind, uint64_t n, int key_bytes,
+                                              const gj_index_params& base) {
+  std::vector<gj_index_params> c;
+  auto add = [&](auto set) {
+    gj_index_params p = base;
+    set(p);
+    c.push_back(p);
+  };
+  switch (kind) {
+    case GJ_HARMONIA:
+      for (uint32_t g : {8u, 16u, 32u})
+        if (g != 32 || key_bytes == 4) add([&](gj_index_params& p) { p.harmonia_group_lanes = g; });
+      break;
+    case GJ_BTREE:
+      for (uint32_t b : {4096u, 16384u, 32768u}) add([&](gj_index_params& p) { p.btree_lds_bytes = b; });
+      break;
+    case GJ_RADIX_SPLINE:
+      for (uint32_t r : {16u, 18u, 20u})
+        for (uint32_t e : {16u, 32u, 64u})
+          add([&](gj_index_params& p) {
+            p.rs_radix_bits = r;
+            p.rs_max_error = e;
+          });
+      break;
+    case GJ_LEARNED_PLR:
+    case GJ_LEARNED_MLP: {
+      const int a = std::clamp(static_cast<int>(ilog2_ceil(n)) - 10, 16, 24);  // the build's default leaf bits
+      for (int d : {-1, 0, 1})
+        for (uint32_t e : {64u, 128u, 256u}) {
+          if (kind == GJ_LEARNED_MLP && e != 128u) continue;  // each MLP candidate retrains the root
+          add([&](gj_index_params& p) {
+            p.learned_leaf_bits = static_cast<uint32_t>(std::clamp(a + d, 1, 24));
+            p.learned_max_error = e;
+          });
+        }
+      break;
+    }
+    default:
+      c.push_back(base);
+  }
+  return c;
+}
+
+}  // namespace
+
+extern "C" int gj_autotune_index(gj_index_kind kind, const void* h_sorted_keys, const void* d_sorted_keys, uint64_t n,
+                                 int key_bytes, const gj_index_params* base, const void* d_probe_sample, uint64_t m,
+                                 uint32_t* d_rowids_scratch, gj_index_params* best, double* best_lookups_per_s,
+                                 gj_stream_t stream) {
+  return guarded([&] {
+    GJ_REQUIRE(d_probe_sample && d_rowids_scratch && best, GJ_ERR_INVALID, "null argument");
+    GJ_REQUIRE(m > 0, GJ_ERR_INVALID, "empty probe sample");
+    gj_index_params b;
+    gj_index_params_default(&b);
+    if (base) b = *base;
+    const hipStream_t s = S(stream);
+    gj_lookup_params lp;
+    gj_lookup_params_default(&lp);
+    hipEvent_t e0, e1;
+    GJ_HIP_CHECK(hipEventCreate(&e0));
+    GJ_HIP_CHECK(hipEventCreate(&e1));
+    double best_ms = -1.0;
+    gj_index* ix = nullptr;
+    try {
+      for (const gj_index_params& p : index_candidates(kind, n, key_bytes, b)) {
+        ix = nullptr;
+        const int st = gj_index_build(kind, h_sorted_keys, d_sorted_keys, n, key_bytes, &p, stream, &ix);
+        if (st == GJ_ERR_HIP || st == GJ_ERR_NOMEM) throw error(st, gj_last_error());
+        if (st != GJ_OK) continue;  // a candidate this column does not admit
+        lookup_dispatch(*ix, d_probe_sample, m, d_rowids_scratch, lp, s);  // warm-up
+        GJ_HIP_CHECK(hipEventRecord(e0, s));
+        const int reps = 3;
+        for (int r = 0; r < reps; ++r) lookup_dispatch(*ix, d_probe_sample, m, d_rowids_scratch, lp, s);
+        GJ_HIP_CHECK(hipEventRecord(e1, s));
+        GJ_HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0;
+        GJ_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        gj_index_destroy(ix);
+        ix = nullptr;
+        const double per = ms / reps;
+        if (best_ms < 0 || per < best_ms) {
+          best_ms = per;
+          *best = p;
+        }
+      }
+    } catch (...) {
+      if (ix) gj_index_destroy(ix);
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    GJ_REQUIRE(best_ms > 0, GJ_ERR_INVALID, "no candidate index could be built over this column");
+    if (best_lookups_per_s) *best_lookups_per_s = static_cast<double>(m) / (best_ms * 1e-3);
+  });
+}

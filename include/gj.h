@@ -274,6 +274,25 @@ void gj_batch_plan_destroy(gj_batch_plan* plan);
 int gj_autotune_lookup(const gj_index* index, const void* d_probe_sample, uint64_t m, uint32_t* d_rowids_scratch,
                        gj_lookup_params* best, double* best_lookups_per_s, gj_stream_t stream);
 
+/* Index autotuner (no reference counterpart; the reference fixes these at
+ * compile time).  Builds `kind` over the column once per candidate of its own
+ * parameters, times the plain lookup of the probe sample on each, and writes
+ * the fastest parameter set into *best.  Candidates, starting from *base (or
+ * the defaults when base is NULL):
+ *   harmonia      harmonia_group_lanes 8 / 16 (/ 32 for 4-byte keys): threads per lookup
+ *   btree         btree_lds_bytes 4096 / 16384 / 32768: the LDS-resident upper levels
+ *   radix_spline  rs_radix_bits 16 / 18 / 20 x rs_max_error 16 / 32 / 64
+ *   learned_plr   learned_leaf_bits default - 1 / default / default + 1
+ *                 x learned_max_error 64 / 128 / 256
+ *   learned_mlp   learned_leaf_bits default - 1 / default / default + 1
+ *   other kinds   *base only
+ * Candidates the column does not admit are skipped.  The candidate indexes
+ * are destroyed; build the chosen one with gj_index_build.  Synchronises the stream. */
+int gj_autotune_index(gj_index_kind kind, const void* h_sorted_keys, const void* d_sorted_keys, uint64_t n,
+                      int key_bytes, const gj_index_params* base, const void* d_probe_sample, uint64_t m,
+                      uint32_t* d_rowids_scratch, gj_index_params* best, double* best_lookups_per_s,
+                      gj_stream_t stream);
+
 /* ---- Learned-model kernels exposed for gradient checks (no reference counterpart)
  * MLP root f(x) = b2 + sum_k w2_k * ramp(w1_k * x + b1_k), ramp(a) = min(max(a, 0), 1),
  * hidden width h (8, 16, 32 or 64), params laid out [w1(h), b1(h), w2(h), b2(1)] in

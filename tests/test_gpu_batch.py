@@ -158,3 +158,31 @@ def test_autotune_returns_exact_config(gj, orc, gpu):
     idx.lookup(dp, out, params=best)
     gj.synchronize()
     assert np.array_equal(out.numpy(), exp)
+
+
+@pytest.mark.parametrize("kind,kb,field,allowed", [
+    ("harmonia", 4, "harmonia_group_lanes", {8, 16, 32}),
+    ("harmonia", 8, "harmonia_group_lanes", {8, 16}),
+    ("btree", 8, "btree_lds_bytes", {4096, 16384, 32768}),
+    ("radix_spline", 8, "rs_max_error", {16, 32, 64}),
+    ("learned_plr", 8, "learned_leaf_bits", {15, 16, 17}),
+])
+def test_index_autotuner_picks_a_candidate(gj, orc, gpu, kind, kb, field, allowed):
+    # the index autotuner builds each candidate over the column, times it, and the
+    # chosen parameters build an index that answers exactly
+    print(f"[case] index autotuner {kind} kb={kb}", flush=True)
+    keys = gj.gen_build_keys(1 << 20, kb, _lib.KEYS_UNIFORM, 0, 0x63 + kb)
+    col = gj.DeviceArray.from_numpy(keys)
+    probes, exp = gj.gen_probes(keys, 1 << 19, _lib.PROBES_UNIFORM, 1.0, 0.0, 0, 0x64)
+    dp = gj.DeviceArray.from_numpy(probes)
+    scratch = gj.DeviceArray(probes.size, np.uint32)
+    best, rate = gj.autotune_index(kind, col, dp, scratch, host_column=keys)
+    print(f"[autotune_index] {kind}: {field}={getattr(best, field)} rate={rate / 1e9:.2f} G/s", flush=True)
+    assert rate > 0 and getattr(best, field) in allowed
+    idx = gj.Index(kind, col, host_column=keys, params=best)
+    out = gj.DeviceArray(probes.size, np.uint32)
+    idx.lookup(dp, out)
+    gj.synchronize()
+    assert np.array_equal(out.numpy(), exp)
+    assert np.array_equal(orc.lookup_lower_bound(keys, probes), exp)
+    idx.close()
