@@ -74,6 +74,11 @@ def load_traffic(kernel_tag: str):
     return None
 
 
+def _index_params_str(p) -> str:
+    return (f"lanes={p.harmonia_group_lanes} btree_lds={p.btree_lds_bytes} rs_bits={p.rs_radix_bits} "
+            f"rs_err={p.rs_max_error} leaf_bits={p.learned_leaf_bits} max_err={p.learned_max_error}")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,6 +92,8 @@ def main():
     ap.add_argument("--profile-index", default="", help="run only this index (for rocprofv3 runs)")
     ap.add_argument("--approach", default="auto", help="auto (plain + autotuned), plain, bws, partitioning or hj")
     ap.add_argument("--tune-log2", type=int, default=24, help="probe sample size of the autotuner")
+    ap.add_argument("--tune-index", action="store_true",
+                    help="pick each index's own parameters with the index autotuner before building it")
     ap.add_argument("--suite", action="store_true", help="measure the other workload configurations instead")
     ap.add_argument("--quick", action="store_true", help="--suite at reduced sizes")
     args = ap.parse_args()
@@ -174,7 +181,13 @@ def main():
     for name in names:
         t_build = time.time()
         try:
-            idx = gj.Index(name, d_keys, host_column=keys, stream=stream)
+            iparams = None
+            if args.tune_index:
+                iparams, irate = gj.autotune_index(name, d_keys, d_probes, d_out, host_column=keys, m=tune_m,
+                                                   stream=stream)
+                log(f"{name}: index autotune on 2^{args.tune_log2} probes -> {_index_params_str(iparams)} "
+                    f"({irate / 1e9:.2f} G/s)")
+            idx = gj.Index(name, d_keys, host_column=keys, params=iparams, stream=stream)
         except gj.GjError as e:
             log(f"{name}: not built ({e})")
             continue
@@ -191,6 +204,8 @@ def main():
             configs = [(args.approach, gj.lookup_params(args.approach))]
         entry = {"index_bytes": st["memory_bytes"], "depth": st["depth"], "lds_levels": st["lds_levels"],
                  "build_s": build_s}
+        if iparams is not None:
+            entry["index_params"] = _index_params_str(iparams)
         for k in ("spline_points", "leaves", "fallback_leaves", "fallback_keys", "max_window", "avg_window",
                   "train_loss"):
             if st[k]:
