@@ -15,6 +15,8 @@ Reported:
                    process (gj_gather), and lookups/s / G per index
   hbm_traffic      measured HBM bytes per lookup launch (rocprofv3 counters, committed under profiles/)
                    over the live average kernel time: how close the kernel runs to the 8 TB/s peak
+  l2               L2 hit rate and L2 misses per lookup of the headline kernel (rocprofv3 counters,
+                   committed under profiles/): lookups/s = the gather probe's miss rate / misses per lookup
   cpu_baseline     the oracle's OpenMP equality-checked lower_bound (port of binary_search_index) on a
                    bounded probe sample, on this host's cores
 
@@ -57,10 +59,9 @@ def time_steps(fn, steps, warmup, stream, gj):
     return per, ev[0].elapsed_ms(ev[steps])
 
 
-def load_traffic(kernel_tag: str):
-    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, copy-calibrated),
-    committed as profiles/traffic.txt lines `<tag> <bytes_per_launch>`."""
-    path = os.path.join(ROOT, "profiles", "traffic.txt")
+def _profile_values(file_name: str, kernel_tag: str):
+    """Numbers after `<tag>` on its line of a committed profiles/ table, or None."""
+    path = os.path.join(ROOT, "profiles", file_name)
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -68,10 +69,25 @@ def load_traffic(kernel_tag: str):
             parts = line.split()
             if len(parts) >= 2 and parts[0] == kernel_tag:
                 try:
-                    return float(parts[1])
+                    return [float(v) for v in parts[1:]]
                 except ValueError:
                     return None
     return None
+
+
+def load_traffic(kernel_tag: str):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, copy-calibrated),
+    committed as profiles/traffic.txt lines `<tag> <bytes_per_launch>`."""
+    v = _profile_values("traffic.txt", kernel_tag)
+    return v[0] if v else None
+
+
+def load_l2(kernel_tag: str):
+    """L2 hit rate and misses per lookup measured with rocprofv3 --pmc (TCC_HIT_sum, TCC_MISS_sum),
+    committed as profiles/l2.txt lines `<tag> <hit_rate> <misses_per_lookup>`."""
+    v = _profile_values("l2.txt", kernel_tag)
+    return {"hit_rate": v[0], "misses_per_lookup": v[1],
+            "source": "rocprofv3 TCC_HIT_sum / TCC_MISS_sum, profiles/l2.txt"} if v and len(v) >= 2 else None
 
 
 def _index_params_str(p) -> str:
@@ -291,6 +307,7 @@ def main():
                         "GBps": traffic / avg_s / 1e9 if (traffic and r) else None,
                         "frac_of_peak": traffic / avg_s / 1e9 / PEAK_HBM_GBPS if (traffic and r) else None,
                         "source": "rocprofv3 FETCH_SIZE x2 + WRITE_SIZE (copy-calibrated), profiles/traffic.txt"},
+        "l2": load_l2(f"{head}_u64_2^{args.log2_n}_2^{args.log2_m}") if head else None,
         "stream_copy_GBps": copy_gbps,
         "pointer_chase_ns": chase_ns,
         "host_gather_per_s": G_host,
