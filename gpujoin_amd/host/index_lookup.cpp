@@ -173,6 +173,33 @@ int main(int argc, char** argv) {
     approach->params.window = c.window;
     approach->initialize(qd);
 
+    // G: independent random 8 B gathers per second over the indexed column (its
+    // power-of-two prefix) with the same probe stream, the roofline every record's
+    // lookups/s is reported against
+    double gathers_per_s = 0;
+    {
+      uint64_t n2 = 1;
+      while (n2 * 2 <= qd.indexed.size()) n2 *= 2;
+      gj_event_t a = nullptr, b = nullptr;
+      check(gj_event_create(&a), "event");
+      check(gj_event_create(&b), "event");
+      auto gather = [&] {
+        check(gj_gather(qd.d_indexed.data(), n2, qd.d_lookup_keys.data(), c.num_lookups, qd.d_tids.data(), 256, 8,
+                        qd.stream),
+              "gather");
+      };
+      gather();
+      check(gj_event_record(a, qd.stream), "event");
+      for (int r = 0; r < 3; ++r) gather();
+      check(gj_event_record(b, qd.stream), "event");
+      check(gj_stream_synchronize(qd.stream), "sync");
+      float ms = 0;
+      check(gj_event_elapsed_ms(a, b, &ms), "elapsed");
+      gathers_per_s = ms > 0 ? 3.0 * static_cast<double>(c.num_lookups) / (ms * 1e-3) : 0;
+      gj_event_destroy(a);
+      gj_event_destroy(b);
+    }
+
     char dev[256] = {0};
     int cu = 0, major = 0;
     check(gj_device_info(dev, sizeof(dev), &cu, &major), "device");
@@ -205,7 +232,13 @@ int main(int argc, char** argv) {
         [&](const measuring::measurement& m) {
           std::vector<std::pair<std::string, std::string>> x;
           x.emplace_back("device_ms", std::to_string(m.device_ms));
-          x.emplace_back("lookups_per_s", std::to_string(c.num_lookups / (m.device_ms * 1e-3)));
+          const double lps = c.num_lookups / (m.device_ms * 1e-3);
+          x.emplace_back("lookups_per_s", std::to_string(lps));
+          x.emplace_back("gather_per_s", std::to_string(gathers_per_s));
+          x.emplace_back("gather_roofline_frac", std::to_string(gathers_per_s > 0 ? lps / gathers_per_s : 0.0));
+          // 8 B probe key + 8 B indexed key + 4 B row id per lookup; HBM traffic and the
+          // L2 hit rate need the profiler's counters
+          x.emplace_back("algorithmic_GBps", std::to_string(lps * 20.0 / 1e9));
           return x;
         });
     gj_stream_destroy(qd.stream);

@@ -37,5 +37,6 @@ def test_cli_records(gpu, approach, index, tmp_path):
     text = out.read_text()
     assert text.count("name: plain_lookup") == 3  # every repetition validated and recorded
     for field in ("approach: " + approach, "index_type: " + index, "num_lookups: !int64 1000000",
-                  "lookups_per_s:", "dt_prefix_sum:", "index_size: !int64"):
+                  "lookups_per_s:", "dt_prefix_sum:", "index_size: !int64", "gather_roofline_frac:",
+                  "algorithmic_GBps:"):
         assert field in text
