@@ -56,6 +56,9 @@ struct gj_index {
 
   // learned two-stage models (B-tree above is their fallback)
   gj::dbuf lm_leaves, lm_mlp, lm_knots, lm_seps, lm_subs;
+  gj::dbuf lm_groups;        // compact leaves: group bases (LeafGroup)
+  bool lm_compact = false;   // lm_leaves holds LeafCompact (8 B) instead of LeafModel (16 B)
+  uint32_t lm_gshift = 0;    // compact leaves per group = 2^lm_gshift
   uint32_t lm_splits = 0;  // learned leaves split into sub-leaves
   gj::RootModel lm_root{};
   uint32_t lm_num_leaves = 0, lm_fallback_any = 0;
@@ -128,6 +131,26 @@ struct gj_index {
     d.seps = lm_seps.as<const uint64_t>();
     d.subs = lm_subs.as<const gj::LeafModel>();
     d.btree = btree_dev<K>();
+    return d;
+  }
+
+  template <class K>
+  gj::LearnedCompactDev<K> learned_compact_dev() const {
+    gj::LearnedCompactDev<K> d{};
+    d.col = static_cast<const K*>(d_col);
+    d.n = n;
+    d.min_key = static_cast<K>(min_key);
+    d.max_key = static_cast<K>(max_key);
+    d.root = lm_root;
+    d.root.mlp = lm_mlp.as<const float>();
+    d.root.knots = lm_knots.as<const uint64_t>();
+    d.leaves = lm_leaves.as<const gj::LeafCompact>();
+    d.groups = lm_groups.as<const gj::LeafGroup>();
+    d.num_leaves = lm_num_leaves;
+    d.gshift = lm_gshift;
+    d.num_groups = static_cast<uint32_t>(lm_groups.bytes / sizeof(gj::LeafGroup));
+    d.err_lo = lm_err_lo;
+    d.err_hi = lm_err_hi;
     return d;
   }
 

@@ -398,6 +398,28 @@ struct LeafModel {  // 16 bytes
 };
 static_assert(sizeof(LeafModel) == 16, "leaf model is 16 bytes");
 
+// Compact leaves (8 bytes) for roots that spread the keys evenly (linear or MLP
+// root, every leaf within the error bound, no split leaf): twice the leaves in
+// the same table, so a smaller window and fewer column lines per lookup.  The
+// key offset is taken from a group base shared by 2^gshift leaves (at most 1024
+// groups, 16 bytes each, staged in LDS) instead of the leaf's own base key:
+//   pred = group.start + floor(fma(slope, f32(key - group.base_key), icpt))
+// The build measures every key's error under exactly this arithmetic, so a
+// leaf whose FP32 offsets lose precision only widens the bounds; if any leaf
+// then exceeds the bound the build keeps the 16-byte leaves.
+struct LeafCompact {  // 8 bytes
+  float slope;
+  float icpt;  // positions past group.start at key == group.base_key
+};
+static_assert(sizeof(LeafCompact) == 8, "compact leaf is 8 bytes");
+
+struct LeafGroup {  // 16 bytes
+  uint64_t base_key;  // key at start (the last key for a group past the column's end)
+  uint32_t start;     // first position routed to the group's first leaf or later
+  uint32_t pad;
+};
+static_assert(sizeof(LeafGroup) == 16, "leaf group is 16 bytes");
+
 constexpr int kMaxHidden = 64;
 
 struct RootModel {
@@ -470,6 +492,27 @@ __device__ __forceinline__ int64_t leaf_pred(const LeafModel& m, float dkey) {
   return static_cast<int64_t>(m.start) + static_cast<int64_t>(floorf(p));
 }
 
+__device__ __forceinline__ int64_t compact_pred(const LeafCompact& m, const LeafGroup& g, float dkey) {
+  float p = __builtin_fmaf(m.slope, dkey, m.icpt);
+  p = fminf(fmaxf(p, -2.0e9f), 2.0e9f);
+  return static_cast<int64_t>(g.start) + static_cast<int64_t>(floorf(p));
+}
+
+// The leaf a key is routed to by the root (whose parameters are staged at rp).
+template <class K>
+__device__ __forceinline__ uint32_t route_leaf(K x, const uint8_t* rp, const RootModel& root, K min_key,
+                                               uint32_t leaves) {
+  float cdf;
+  if (root.segments) {
+    cdf = spline_cdf(static_cast<uint64_t>(x), rp, root);
+  } else if (root.hidden) {
+    cdf = mlp_eval(reinterpret_cast<const float*>(rp), root.hidden, key_unit(x, min_key, root.inv_range));
+  } else {
+    cdf = __builtin_fmaf(root.a, key_unit(x, min_key, root.inv_range), root.b);
+  }
+  return leaf_of(cdf, leaves);
+}
+
 // How probes of leaves routed to the B-tree are answered:
 //   kLeafNoFallback  no such leaf exists; the kernel carries no B-tree code
 //   kLeafInline      the B-tree lookup runs inside the same kernel
@@ -514,16 +557,7 @@ struct LearnedDev {
   }
 
   __device__ __forceinline__ uint32_t route(K x, const uint8_t* lds) const {
-    const uint8_t* rp = lds + root_offset();
-    float cdf;
-    if (root.segments) {
-      cdf = spline_cdf(static_cast<uint64_t>(x), rp, root);
-    } else if (root.hidden) {
-      cdf = mlp_eval(reinterpret_cast<const float*>(rp), root.hidden, key_unit(x, min_key, root.inv_range));
-    } else {
-      cdf = __builtin_fmaf(root.a, key_unit(x, min_key, root.inv_range), root.b);
-    }
-    return leaf_of(cdf, num_leaves);
+    return route_leaf<K>(x, lds + root_offset(), root, min_key, num_leaves);
   }
 
   __device__ __forceinline__ uint32_t lookup(K x, bool active, const uint8_t* lds) const {
@@ -569,6 +603,49 @@ struct LearnedDev {
       if (use_tree) r = kFallbackMark;
     }
     return r;
+  }
+};
+
+// The learned index with compact leaves: no fallback and no split leaf exist.
+template <class K>
+struct LearnedCompactDev {
+  const K* col;
+  uint64_t n;
+  K min_key, max_key;
+  RootModel root;
+  const LeafCompact* leaves;
+  const LeafGroup* groups;  // num_groups entries, staged in LDS after the root
+  uint32_t num_leaves, gshift, num_groups;
+  int32_t err_lo, err_hi;  // window [pred + err_lo, pred + err_hi]
+
+  __host__ __device__ size_t group_offset() const { return (root.bytes() + 15) / 16 * 16; }
+  __host__ __device__ size_t lds_bytes() const {
+    return group_offset() + static_cast<size_t>(num_groups) * sizeof(LeafGroup);
+  }
+  __device__ void stage(uint8_t* lds) const {
+    if (root.bytes()) stage_to_lds(lds, root.params(), root.bytes());
+    stage_to_lds(lds + group_offset(), groups, static_cast<size_t>(num_groups) * sizeof(LeafGroup));
+  }
+
+  __device__ __forceinline__ uint32_t lookup(K x, bool active, const uint8_t* lds) const {
+    uint64_t lo = 0, hi = 0, pred = 0;
+    const bool live = active && x >= min_key && x <= max_key;
+    if (live) {
+      const uint32_t l = route_leaf<K>(x, lds, root, min_key, num_leaves);
+      const LeafCompact m = leaves[l];
+      const LeafGroup g = reinterpret_cast<const LeafGroup*>(lds + group_offset())[l >> gshift];
+      const int64_t p = compact_pred(m, g, leaf_dkey(x, g.base_key));
+      int64_t a = p + err_lo, b = p + err_hi + 1;
+      a = a < 0 ? 0 : a;
+      b = b > static_cast<int64_t>(n) ? static_cast<int64_t>(n) : b;
+      if (a < b) {
+        lo = static_cast<uint64_t>(a);
+        hi = static_cast<uint64_t>(b);
+        pred = static_cast<uint64_t>(p < a ? a : (p >= b ? b - 1 : p));
+      }
+    }
+    return group_window_search<K>(col, n, static_cast<uint32_t>(pred), static_cast<uint32_t>(lo),
+                                  static_cast<uint32_t>(hi), x, live);
   }
 };
 

@@ -407,6 +407,57 @@ __global__ void split_mark_kernel(LeafModel* __restrict__ lm, const uint32_t* __
   lm[leaf_id[j]] = m;
 }
 
+// Group g of the compact leaves starts at the first position routed to leaf
+// g << gshift or a later leaf (start[] of leaf_bounds_kernel).
+template <class K>
+__global__ void compact_groups_kernel(const K* __restrict__ col, uint64_t n, const uint32_t* __restrict__ start,
+                                      uint32_t groups, uint32_t gshift, LeafGroup* __restrict__ out) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= groups) return;
+  const uint32_t st = start[static_cast<uint64_t>(g) << gshift];
+  LeafGroup r;
+  r.start = st;
+  r.base_key = st < n ? static_cast<uint64_t>(col[st]) : static_cast<uint64_t>(col[n - 1]);
+  r.pad = 0;
+  out[g] = r;
+}
+
+// 16-byte leaf -> compact leaf over its group: start + slope * (key - base) equals
+// group.start + slope * (key - group.base) + icpt for icpt = start - group.start
+// + slope * (group.base - base).  Resets the leaf's error range for re-measuring.
+__global__ void compact_convert_kernel(const LeafModel* __restrict__ lm, const LeafGroup* __restrict__ groups,
+                                       uint32_t leaves, uint32_t gshift, LeafCompact* __restrict__ out,
+                                       LeafFit* __restrict__ fit) {
+  const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= leaves) return;
+  const LeafModel m = lm[l];
+  const LeafGroup g = groups[l >> gshift];
+  double icpt = static_cast<double>(static_cast<int64_t>(m.start) - static_cast<int64_t>(g.start));
+  if (m.slope != 0.0f)
+    icpt += static_cast<double>(m.slope) * static_cast<double>(static_cast<int64_t>(g.base_key - m.base_key));
+  out[l] = LeafCompact{m.slope, static_cast<float>(icpt)};
+  fit[l].err_lo = INT_MAX;
+  fit[l].err_hi = INT_MIN;
+}
+
+// err_lo/err_hi of every compact leaf under the exact lookup arithmetic.
+template <class K>
+__global__ void compact_error_kernel(const K* __restrict__ col, uint64_t n, Router<K> route,
+                                     const LeafCompact* __restrict__ lc, const LeafGroup* __restrict__ groups,
+                                     uint32_t gshift, LeafFit* __restrict__ fit) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const K key = col[i];
+    const uint32_t l = route(key);
+    const LeafGroup g = groups[l >> gshift];
+    const int64_t pred = compact_pred(lc[l], g, leaf_dkey(key, g.base_key));
+    int64_t e = static_cast<int64_t>(i) - pred;
+    e = e < INT_MIN / 2 ? INT_MIN / 2 : (e > INT_MAX / 2 ? INT_MAX / 2 : e);
+    atomicMin(&fit[l].err_lo, static_cast<int>(e));
+    atomicMax(&fit[l].err_hi, static_cast<int>(e));
+  }
+}
+
 }  // namespace
 
 template <class K>
@@ -417,7 +468,7 @@ void build_learned(gj_index& ix, hipStream_t s) {
   uint32_t bits = ix.params.learned_leaf_bits;
   if (bits == 0) bits = static_cast<uint32_t>(std::clamp(static_cast<int>(ilog2_ceil(n)) - 10, 16, 24));
   GJ_REQUIRE(bits >= 1 && bits <= 24, GJ_ERR_INVALID, "learned_leaf_bits must be in [1, 24]");
-  const uint32_t L = 1u << bits;
+  uint32_t L = 1u << bits;  // leaves of the build in progress
   const K kmin = static_cast<K>(ix.min_key);
   const uint64_t range = ix.max_key - ix.min_key;
   RootModel root{};
@@ -548,8 +599,11 @@ void build_learned(gj_index& ix, hipStream_t s) {
     root = spline_root(want);
   }
 
+  dbuf last_start, last_fit;  // leaf starts and fits of the last build_leaves
   // routing, leaf fits and calibration under a root; returns the build keys routed to the B-tree
   auto build_leaves = [&](const RootModel& root) -> uint64_t {
+    ix.lm_compact = false;
+    ix.lm_groups.release();
     Router<K> route{kmin, root, L};
     dbuf start((static_cast<uint64_t>(L) + 1 + 3) * sizeof(uint32_t));
     {
@@ -675,13 +729,73 @@ void build_learned(gj_index& ix, hipStream_t s) {
     ix.stats.avg_window = wkeys ? static_cast<double>(wsum) / static_cast<double>(wkeys) : 0.0;
     ix.stats.depth = 2;
     ix.stats.lds_levels = ix.bt.lds_levels;
+    last_start = std::move(start);
+    last_fit = std::move(fit);
     return fb_keys;
   };
-  const uint64_t fb = build_leaves(root);
-  if (ix.kind == GJ_LEARNED_PLR && ix.params.learned_root_segments == 0 && fb > 0 && n >= 4) {
-    // the linear root cannot spread these keys evenly over the leaves: route by a
-    // spline of the column's own CDF instead
-    build_leaves(spline_root(2048));
+
+  // compact leaves from the last build's 16-byte leaves; false (index unchanged) when a
+  // leaf's window under the compact arithmetic exceeds the bound
+  auto make_compact = [&](const RootModel& root) -> bool {
+    const uint32_t lbits = ilog2_ceil(L);
+    const uint32_t gshift = lbits > 10 ? lbits - 10 : 0;  // at most 1024 groups (16 KiB of LDS)
+    const uint32_t G = L >> gshift;
+    dbuf groups(sizeof(LeafGroup) * G), cl(sizeof(LeafCompact) * L);
+    hipLaunchKernelGGL(compact_groups_kernel<K>, dim3(grid_for(G, 256)), dim3(256), 0, s, col, n,
+                       last_start.as<const uint32_t>(), G, gshift, groups.as<LeafGroup>());
+    GJ_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(compact_convert_kernel, dim3(grid_for(L, 256)), dim3(256), 0, s,
+                       ix.lm_leaves.as<const LeafModel>(), groups.as<const LeafGroup>(), L, gshift,
+                       cl.as<LeafCompact>(), last_fit.as<LeafFit>());
+    GJ_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(compact_error_kernel<K>, dim3(grid_for(n, 256)), dim3(256), 0, s, col, n,
+                       Router<K>{kmin, root, L}, cl.as<const LeafCompact>(), groups.as<const LeafGroup>(), gshift,
+                       last_fit.as<LeafFit>());
+    GJ_HIP_CHECK(hipGetLastError());
+    std::vector<LeafFit> hf(L);
+    GJ_HIP_CHECK(hipMemcpyAsync(hf.data(), last_fit.p, last_fit.bytes, hipMemcpyDeviceToHost, s));
+    GJ_HIP_CHECK(hipStreamSynchronize(s));
+    const int64_t max_width = 2 * static_cast<int64_t>(ix.params.learned_max_error) + 2;
+    uint64_t wsum = 0, wkeys = 0;
+    uint32_t wmax = 0;
+    int32_t elo = 0, ehi = 0;
+    for (const LeafFit& f : hf) {
+      if (f.count == 0) continue;
+      const int64_t w = static_cast<int64_t>(f.err_hi) - f.err_lo + 1;
+      if (w > max_width) return false;
+      wmax = std::max(wmax, static_cast<uint32_t>(w));
+      wsum += static_cast<uint64_t>(w) * f.count;
+      wkeys += f.count;
+      elo = std::min(elo, f.err_lo);
+      ehi = std::max(ehi, f.err_hi);
+    }
+    ix.lm_leaves = std::move(cl);
+    ix.lm_groups = std::move(groups);
+    ix.lm_gshift = gshift;
+    ix.lm_compact = true;
+    ix.lm_err_lo = elo;
+    ix.lm_err_hi = ehi;
+    ix.stats.max_window = wmax;
+    ix.stats.avg_window = wkeys ? static_cast<double>(wsum) / static_cast<double>(wkeys) : 0.0;
+    return true;
+  };
+
+  // With the default leaf count, a root that spreads the keys evenly gets twice the
+  // leaves in compact form (the same table bytes); otherwise 16-byte leaves.
+  bool compact = false;
+  if (ix.params.learned_leaf_bits == 0 && bits < 24 && root.segments == 0) {
+    L = 2u << bits;
+    const uint64_t fb2 = build_leaves(root);
+    compact = fb2 == 0 && ix.lm_splits == 0 && make_compact(root);
+    if (!compact) L = 1u << bits;
+  }
+  if (!compact) {
+    const uint64_t fb = build_leaves(root);
+    if (ix.kind == GJ_LEARNED_PLR && ix.params.learned_root_segments == 0 && fb > 0 && n >= 4) {
+      // the linear root cannot spread these keys evenly over the leaves: route by a
+      // spline of the column's own CDF instead
+      build_leaves(spline_root(2048));
+    }
   }
 }
 template void build_learned<uint32_t>(gj_index&, hipStream_t);
