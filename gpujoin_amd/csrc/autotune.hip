@@ -118,6 +118,9 @@ std::vector<gj_index_params> index_candidates(gj_index_kind kind, uint64_t n, in
       break;
     case GJ_LEARNED_PLR:
     case GJ_LEARNED_MLP: {
+      // the default (compact leaves where the build admits them), then explicit counts
+      // of 16-byte leaves around the default
+      add([&](gj_index_params& p) { p.learned_leaf_bits = 0; });
       const int a = std::clamp(static_cast<int>(ilog2_ceil(n)) - 10, 16, 24);  // the build's default leaf bits
       for (int d : {-1, 0, 1})
         for (uint32_t e : {64u, 128u, 256u}) {

@@ -399,9 +399,10 @@ struct LeafModel {  // 16 bytes
 static_assert(sizeof(LeafModel) == 16, "leaf model is 16 bytes");
 
 // Compact leaves (8 bytes) for roots that spread the keys evenly (linear or MLP
-// root, every leaf within the error bound, no split leaf): twice the leaves in
+// root, every leaf within the error bound, no split leaf; 2^22 keys or more, the
+// default leaf count): twice the leaves in
 // the same table, so a smaller window and fewer column lines per lookup.  The
-// key offset is taken from a group base shared by 2^gshift leaves (at most 1024
+// key offset is taken from a group base shared by 2^gshift leaves (at most 512
 // groups, 16 bytes each, staged in LDS) instead of the leaf's own base key:
 //   pred = group.start + floor(fma(slope, f32(key - group.base_key), icpt))
 // The build measures every key's error under exactly this arithmetic, so a

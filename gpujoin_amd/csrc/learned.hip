@@ -738,7 +738,7 @@ void build_learned(gj_index& ix, hipStream_t s) {
   // leaf's window under the compact arithmetic exceeds the bound
   auto make_compact = [&](const RootModel& root) -> bool {
     const uint32_t lbits = ilog2_ceil(L);
-    const uint32_t gshift = lbits > 10 ? lbits - 10 : 0;  // at most 1024 groups (16 KiB of LDS)
+    const uint32_t gshift = lbits > 9 ? lbits - 9 : 0;  // at most 512 groups (8 KiB of LDS)
     const uint32_t G = L >> gshift;
     dbuf groups(sizeof(LeafGroup) * G), cl(sizeof(LeafCompact) * L);
     hipLaunchKernelGGL(compact_groups_kernel<K>, dim3(grid_for(G, 256)), dim3(256), 0, s, col, n,
@@ -781,9 +781,11 @@ void build_learned(gj_index& ix, hipStream_t s) {
   };
 
   // With the default leaf count, a root that spreads the keys evenly gets twice the
-  // leaves in compact form (the same table bytes); otherwise 16-byte leaves.
+  // leaves in compact form (the same table bytes); otherwise 16-byte leaves.  Below
+  // 2^22 keys the lookups are bound by L2 hits rather than misses and the compact
+  // form's LDS group table only costs occupancy, so small indexes keep 16-byte leaves.
   bool compact = false;
-  if (ix.params.learned_leaf_bits == 0 && bits < 24 && root.segments == 0) {
+  if (ix.params.learned_leaf_bits == 0 && bits < 24 && root.segments == 0 && n >= (1ull << 22)) {
     L = 2u << bits;
     const uint64_t fb2 = build_leaves(root);
     compact = fb2 == 0 && ix.lm_splits == 0 && make_compact(root);

@@ -82,7 +82,9 @@ typedef struct gj_index_params {
   uint32_t rs_max_error;         /* RadixSpline error bound, default 32 */
   uint32_t harmonia_group_lanes; /* lanes per Harmonia node search (8, 16, or 32 for u32 keys), default 8 */
   uint32_t btree_lds_bytes;      /* LDS budget for resident upper B-tree levels, default 16384 */
-  uint32_t learned_leaf_bits;    /* log2 of the number of linear leaves; 0 (default) = max(16, ceil(log2 n) - 10) */
+  uint32_t learned_leaf_bits;    /* log2 of the number of linear leaves; 0 (default) = max(16, ceil(log2 n) - 10)
+                                    16-byte leaves, or twice as many 8-byte leaves from 2^22 keys when the
+                                    root spreads the keys evenly (linear or MLP root, no fallback leaf) */
   uint32_t learned_max_error;    /* last-mile bound; leaves with a wider measured window use the B-tree, default 128 */
   uint32_t mlp_hidden;           /* hidden width of the MLP root (<= 64), default 32 */
   uint32_t mlp_steps;            /* training steps of the MLP root, default 400 */
@@ -282,9 +284,10 @@ int gj_autotune_lookup(const gj_index* index, const void* d_probe_sample, uint64
  *   harmonia      harmonia_group_lanes 8 / 16 (/ 32 for 4-byte keys): threads per lookup
  *   btree         btree_lds_bytes 4096 / 16384 / 32768: the LDS-resident upper levels
  *   radix_spline  rs_radix_bits 16 / 18 / 20 x rs_max_error 16 / 32 / 64
- *   learned_plr   learned_leaf_bits default - 1 / default / default + 1
- *                 x learned_max_error 64 / 128 / 256
- *   learned_mlp   learned_leaf_bits default - 1 / default / default + 1
+ *   learned_plr   learned_leaf_bits 0 (the default: compact 8-byte leaves where the build
+ *                 admits them), then 16-byte leaves at default - 1 / default / default + 1
+ *                 bits x learned_max_error 64 / 128 / 256
+ *   learned_mlp   learned_leaf_bits 0, then default - 1 / default / default + 1
  *   other kinds   *base only
  * Candidates the column does not admit are skipped.  The candidate indexes
  * are destroyed; build the chosen one with gj_index_build.  Synchronises the stream. */

@@ -165,7 +165,7 @@ def test_autotune_returns_exact_config(gj, orc, gpu):
     ("harmonia", 8, "harmonia_group_lanes", {8, 16}),
     ("btree", 8, "btree_lds_bytes", {4096, 16384, 32768}),
     ("radix_spline", 8, "rs_max_error", {16, 32, 64}),
-    ("learned_plr", 8, "learned_leaf_bits", {15, 16, 17}),
+    ("learned_plr", 8, "learned_leaf_bits", {0, 15, 16, 17}),
 ])
 def test_index_autotuner_picks_a_candidate(gj, orc, gpu, kind, kb, field, allowed):
     # the index autotuner builds each candidate over the column, times it, and the

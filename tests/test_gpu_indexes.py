@@ -252,3 +252,25 @@ def test_hash_matches_oracle(gj, orc, gpu, kb, n):
     assert np.array_equal(got, want)
     assert np.array_equal(orc.hash_join(keys, probes), want)
     assert st["memory_bytes"] >= orc.hash_capacity(n) * 16
+
+
+@pytest.mark.parametrize("kind", ["learned_plr", "learned_mlp"])
+@pytest.mark.parametrize("kb", [4, 8])
+def test_compact_leaves_exact(gj, orc, gpu, kind, kb):
+    # from 2^22 keys, the default leaf count gives twice the leaves in the 8-byte form
+    # (group bases in LDS) when every leaf stays within the bound; row ids stay exact,
+    # misses included
+    rng = np.random.default_rng(93 + kb)
+    dt = np.uint64 if kb == 8 else np.uint32
+    hi = 2 ** 62 if kb == 8 else 2 ** 32 - 2
+    keys = np.unique(rng.integers(0, hi, size=(1 << 22) + 40000, dtype=dt))[: (1 << 22) + 12345]
+    probes = np.concatenate([rng.choice(keys, 300000), rng.integers(0, hi, size=50000, dtype=dt),
+                             keys[:100], keys[-100:]]).astype(dt)
+    got, st = _lookup(gj, kind, keys, probes, params=gj.index_params(mlp_steps=100))
+    print(f"[case] {kind} kb={kb} leaves={st['leaves']} avg_window={st['avg_window']:.1f}", flush=True)
+    assert st["leaves"] == 1 << 17 and st["fallback_leaves"] == 0
+    assert np.array_equal(got, orc.lookup_lower_bound(keys, probes))
+    # an explicit leaf count keeps the 16-byte leaves and the same answers
+    got16, st16 = _lookup(gj, kind, keys, probes, params=gj.index_params(learned_leaf_bits=16, mlp_steps=100))
+    assert st16["leaves"] == 1 << 16
+    assert np.array_equal(got16, got)
