@@ -464,7 +464,8 @@ template <class K>
 void build_learned(gj_index& ix, hipStream_t s) {
   const K* col = static_cast<const K*>(ix.d_col);
   const uint64_t n = ix.n;
-  // default: 2^16 leaves (1 MiB, L2-resident) up to 2^26 keys, then about 1024 keys per leaf
+  // default: 2^16 16-byte leaves (1 MiB, L2-resident) up to 2^26 keys, then about 1024
+  // keys per leaf; twice as many compact leaves in the same bytes where the build admits them
   uint32_t bits = ix.params.learned_leaf_bits;
   if (bits == 0) bits = static_cast<uint32_t>(std::clamp(static_cast<int>(ilog2_ceil(n)) - 10, 16, 24));
   GJ_REQUIRE(bits >= 1 && bits <= 24, GJ_ERR_INVALID, "learned_leaf_bits must be in [1, 24]");
