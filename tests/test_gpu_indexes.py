@@ -274,3 +274,40 @@ def test_compact_leaves_exact(gj, orc, gpu, kind, kb):
     got16, st16 = _lookup(gj, kind, keys, probes, params=gj.index_params(learned_leaf_bits=16, mlp_steps=100))
     assert st16["leaves"] == 1 << 16
     assert np.array_equal(got16, got)
+
+
+@pytest.mark.parametrize("kind", ["learned_plr", "learned_mlp"])
+def test_compact_leaves_every_driver(gj, orc, gpu, kind):
+    # the compact-leaf descriptor through the blockwise-sorting, partitioned and
+    # hash-join drivers and the persistent and graph lookup plans
+    rng = np.random.default_rng(95)
+    keys = np.unique(rng.integers(0, 2 ** 62, size=(1 << 22) + 30000, dtype=np.uint64))[: (1 << 22) + 777]
+    probes = np.concatenate([rng.choice(keys, 400000), rng.integers(0, 2 ** 62, size=40000, dtype=np.uint64)])
+    want = orc.lookup_lower_bound(keys, probes)
+    col = gj.DeviceArray.from_numpy(keys)
+    idx = gj.Index(kind, col, host_column=keys, params=gj.index_params(mlp_steps=100))
+    assert idx.stats()["leaves"] == 1 << 17
+    dp = gj.DeviceArray.from_numpy(probes)
+    out = gj.DeviceArray(probes.size, np.uint32)
+    for approach, kw in (("bws", {}), ("partitioning", {"radix_bits": 10, "window": 1 << 17}),
+                         ("partitioning", {"radix_bits": 12, "passes": 2}), ("hj", {})):
+        print(f"[case] {kind} compact {approach} {kw}", flush=True)
+        out.fill(0x5A)
+        idx.lookup(dp, out, approach=approach, **kw)
+        gj.synchronize()
+        assert np.array_equal(out.numpy(), want), (approach, kw)
+    sizes = [1, 1000, 50000, 0, 250000, probes.size - 301001]
+    for mode in (gj.BATCH_PERSISTENT, gj.BATCH_GRAPH):
+        print(f"[case] {kind} compact plan mode={mode}", flush=True)
+        batches, off = [], 0
+        for sz in sizes:
+            batches.append((dp.offset(off), out.offset(off), sz))
+            off += sz
+        plan = gj.LookupPlan(idx, batches, mode)
+        for rep in range(2):
+            out.fill(0x33)
+            plan.run()
+            gj.synchronize()
+            assert np.array_equal(out.numpy(), want), (mode, rep)
+        plan.close()
+    idx.close()
