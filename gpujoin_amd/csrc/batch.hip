@@ -133,6 +133,12 @@ __device__ __forceinline__ void agg_rows(const Dev& d, const uint8_t* lds, AggQu
   }
 }
 
+// Per-workgroup sums of the one-batch kernel (no result atomics; see agg_sum_kernel).
+struct AggPartial {
+  int64_t num, den;
+  uint64_t matches, pad;
+};
+
 template <class Dev>
 __device__ __forceinline__ void agg_flush(const Dev& d, const uint8_t* lds, AggQueue& q,
                                           const uint8_t* __restrict__ flag, AggAcc& acc, gj_agg_result* out) {
@@ -160,21 +166,78 @@ __device__ __forceinline__ AggQueue wave_queue(uint8_t* lds_tail) {
 
 inline size_t agg_queue_bytes() { return (kAggBlock / kWave) * kAggQueue * (4 + 8); }
 
-// One batch, grid-stride over 64-row steps.
+// One batch, grid-stride over 64-row steps.  Each workgroup writes its sums to
+// partial[blockIdx.x]; agg_sum_kernel adds them to the result.  (One 64-bit atomic
+// per wave onto the three result words serialises at their L2 channel: at ~4000
+// waves that took about half of the kernel's time.)
 template <class Dev>
 __global__ __launch_bounds__(kAggBlock) void join_agg_kernel(Dev d, FactBatchDev b, int32_t lo, int32_t hi,
-                                                             const uint8_t* __restrict__ flag) {
+                                                             const uint8_t* __restrict__ flag,
+                                                             AggPartial* __restrict__ partial) {
   extern __shared__ __attribute__((aligned(16))) uint8_t lds[];
+  __shared__ int64_t red[kAggBlock / kWave][3];
   d.stage(lds);
   const size_t ilds = (d.lds_bytes() + 15) / 16 * 16;
   AggQueue q = wave_queue<Dev>(lds + ilds);
   __syncthreads();
   const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (blockDim.x / kWave);
-  const uint64_t wave = static_cast<uint64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
+  const int w = threadIdx.x / kWave;
+  const uint64_t wave = static_cast<uint64_t>(blockIdx.x) * (blockDim.x / kWave) + w;
   AggAcc acc;
   const uint64_t span = static_cast<uint64_t>(kWave) * kAggUnroll;
   agg_rows(d, lds, q, b, wave * span, b.rows, waves * span, lo, hi, flag, acc);
-  agg_flush(d, lds, q, flag, acc, b.result);
+  while (q.count > 0) agg_drain(d, lds, q, q.count < kWave ? q.count : kWave, flag, acc);
+  const int64_t num = wave_sum_i64(acc.num);
+  const int64_t den = wave_sum_i64(acc.den);
+  const int64_t mt = wave_sum_i64(static_cast<int64_t>(acc.matches));
+  if (lane() == 0) {
+    red[w][0] = num;
+    red[w][1] = den;
+    red[w][2] = mt;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    AggPartial p{0, 0, 0, 0};
+    for (int v = 0; v < kAggBlock / kWave; ++v) {
+      p.num += red[v][0];
+      p.den += red[v][1];
+      p.matches += static_cast<uint64_t>(red[v][2]);
+    }
+    partial[blockIdx.x] = p;
+  }
+}
+
+// Adds the workgroups' partial sums to the result: one workgroup, exact int64.
+__global__ __launch_bounds__(1024) void agg_sum_kernel(const AggPartial* __restrict__ partial, uint32_t blocks,
+                                                       gj_agg_result* __restrict__ out) {
+  __shared__ int64_t red[1024 / kWave][3];
+  int64_t num = 0, den = 0, mt = 0;
+  for (uint32_t i = threadIdx.x; i < blocks; i += blockDim.x) {
+    num += partial[i].num;
+    den += partial[i].den;
+    mt += static_cast<int64_t>(partial[i].matches);
+  }
+  num = wave_sum_i64(num);
+  den = wave_sum_i64(den);
+  mt = wave_sum_i64(mt);
+  const int w = threadIdx.x / kWave;
+  if (lane() == 0) {
+    red[w][0] = num;
+    red[w][1] = den;
+    red[w][2] = mt;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int v = 1; v < static_cast<int>(blockDim.x / kWave); ++v) {
+      num += red[v][0];
+      den += red[v][1];
+      mt += red[v][2];
+    }
+    // results are added (the caller zeroes them); a single writer per call
+    atomicAdd(reinterpret_cast<unsigned long long*>(&out->numerator), static_cast<unsigned long long>(num));
+    atomicAdd(reinterpret_cast<unsigned long long*>(&out->denominator), static_cast<unsigned long long>(den));
+    atomicAdd(reinterpret_cast<unsigned long long*>(&out->matches), static_cast<unsigned long long>(mt));
+  }
 }
 
 __device__ __forceinline__ uint32_t find_batch(const uint64_t* tile_start, uint32_t nb, uint64_t t) {
@@ -187,19 +250,22 @@ __device__ __forceinline__ uint32_t find_batch(const uint64_t* tile_start, uint3
   return lo;
 }
 
-// Persistent join-aggregate: waves pull (batch, tile) items until the counter passes total_tiles.
+// Persistent join-aggregate: waves pull (batch, tile) items until the counter
+// passes total_tiles.  A tile's sums go to partial[tile] (each tile has one
+// owner, so a plain store); agg_batch_sum_kernel then adds each batch's tiles to
+// its result, so the waves issue no result atomics.
 template <class Dev>
 __global__ __launch_bounds__(kAggBlock) void join_agg_persistent_kernel(Dev d, const FactBatchDev* __restrict__ batches,
                                                                         const uint64_t* __restrict__ tile_start,
                                                                         uint32_t nb, uint64_t total_tiles, int32_t lo,
                                                                         int32_t hi, const uint8_t* __restrict__ flag,
-                                                                        unsigned long long* counter) {
+                                                                        unsigned long long* counter,
+                                                                        AggPartial* __restrict__ partial) {
   extern __shared__ __attribute__((aligned(16))) uint8_t lds[];
   d.stage(lds);
   const size_t ilds = (d.lds_bytes() + 15) / 16 * 16;
   AggQueue q = wave_queue<Dev>(lds + ilds);
   __syncthreads();
-  AggAcc acc;
   for (;;) {
     unsigned long long t = 0;
     if (lane() == 0) t = atomicAdd(counter, 1ull);
@@ -209,8 +275,39 @@ __global__ __launch_bounds__(kAggBlock) void join_agg_persistent_kernel(Dev d, c
     const FactBatchDev b = batches[bi];
     const uint64_t r0 = (t - b.tile_start) * kAggTileRows;
     const uint64_t r1 = std::min<uint64_t>(b.rows, r0 + kAggTileRows);
+    AggAcc acc;
     agg_rows(d, lds, q, b, r0, r1, static_cast<uint64_t>(kWave) * kAggUnroll, lo, hi, flag, acc);
-    agg_flush(d, lds, q, flag, acc, b.result);
+    while (q.count > 0) agg_drain(d, lds, q, q.count < kWave ? q.count : kWave, flag, acc);
+    const int64_t num = wave_sum_i64(acc.num);
+    const int64_t den = wave_sum_i64(acc.den);
+    const int64_t mt = wave_sum_i64(static_cast<int64_t>(acc.matches));
+    if (lane() == 0) partial[t] = AggPartial{num, den, static_cast<uint64_t>(mt), 0};
+  }
+}
+
+// One wave per batch: the sum of its tiles' partials, added to its result.
+__global__ __launch_bounds__(256) void agg_batch_sum_kernel(const FactBatchDev* __restrict__ batches,
+                                                            const uint64_t* __restrict__ tile_start, uint32_t nb,
+                                                            uint64_t total_tiles,
+                                                            const AggPartial* __restrict__ partial) {
+  const uint64_t bw = static_cast<uint64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
+  if (bw >= nb) return;  // wave-uniform
+  const uint64_t t0 = tile_start[bw];
+  const uint64_t t1 = bw + 1 < nb ? tile_start[bw + 1] : total_tiles;
+  int64_t num = 0, den = 0, mt = 0;
+  for (uint64_t t = t0 + static_cast<uint64_t>(lane()); t < t1; t += kWave) {
+    num += partial[t].num;
+    den += partial[t].den;
+    mt += static_cast<int64_t>(partial[t].matches);
+  }
+  num = wave_sum_i64(num);
+  den = wave_sum_i64(den);
+  mt = wave_sum_i64(mt);
+  if (lane() == 0 && (num | den | mt) != 0) {
+    gj_agg_result* out = batches[bw].result;
+    atomicAdd(reinterpret_cast<unsigned long long*>(&out->numerator), static_cast<unsigned long long>(num));
+    atomicAdd(reinterpret_cast<unsigned long long*>(&out->denominator), static_cast<unsigned long long>(den));
+    atomicAdd(reinterpret_cast<unsigned long long*>(&out->matches), static_cast<unsigned long long>(mt));
   }
 }
 
@@ -244,17 +341,23 @@ __global__ __launch_bounds__(256) void lookup_persistent_kernel(Dev d, const Pro
 
 namespace {
 
+// Workgroup partial sums: at most 8 workgroups of 256 threads fit one CU.
+inline uint64_t agg_partial_cap(int cu) { return 8ull * static_cast<uint64_t>(cu); }
+
 template <class Dev>
 void launch_join_agg(const Dev& d, const FactBatchDev& b, int32_t lo, int32_t hi, const uint8_t* flag, int cu,
-                     hipStream_t s) {
+                     AggPartial* partial, hipStream_t s) {
   if (b.rows == 0) return;
   const size_t lds = (d.lds_bytes() + 15) / 16 * 16 + agg_queue_bytes();
   const void* fn = reinterpret_cast<const void*>(&join_agg_kernel<Dev>);
   uint64_t blocks = static_cast<uint64_t>(blocks_per_cu(fn, kAggBlock, lds)) * cu;
   blocks = std::min<uint64_t>(blocks, ceil_div(b.rows, static_cast<uint64_t>(kAggBlock) * kAggUnroll));
   blocks = std::max<uint64_t>(blocks, 1);
+  GJ_REQUIRE(blocks <= agg_partial_cap(cu), GJ_ERR_INTERNAL, "join-aggregate grid exceeds the partial buffer");
   hipLaunchKernelGGL((join_agg_kernel<Dev>), dim3(static_cast<unsigned>(blocks)), dim3(kAggBlock), lds, s, d, b, lo, hi,
-                     flag);
+                     flag, partial);
+  GJ_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(agg_sum_kernel, dim3(1), dim3(1024), 0, s, partial, static_cast<uint32_t>(blocks), b.result);
   GJ_HIP_CHECK(hipGetLastError());
 }
 
@@ -288,6 +391,7 @@ struct gj_batch_plan {
   gj_agg_result* results = nullptr;
   // device copies (persistent)
   dbuf d_batches, d_tile_start, d_counter;
+  dbuf d_partial;  // workgroup partial sums of the per-batch join-aggregate launches
   // graph
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -318,10 +422,16 @@ void enqueue_plan(gj_batch_plan& pl, hipStream_t s) {
         hipLaunchKernelGGL((join_agg_persistent_kernel<Dev>), dim3(static_cast<unsigned>(blocks)), dim3(kAggBlock),
                            lds, s, d, pl.d_batches.as<const FactBatchDev>(), pl.d_tile_start.as<const uint64_t>(),
                            static_cast<uint32_t>(pl.fact.size()), pl.total_tiles, pl.lo, pl.hi, pl.flag,
-                           pl.d_counter.as<unsigned long long>());
+                           pl.d_counter.as<unsigned long long>(), pl.d_partial.as<AggPartial>());
+        GJ_HIP_CHECK(hipGetLastError());
+        const uint32_t nbat = static_cast<uint32_t>(pl.fact.size());
+        hipLaunchKernelGGL(agg_batch_sum_kernel, dim3(static_cast<unsigned>(ceil_div(nbat, 256 / kWave))), dim3(256), 0,
+                           s, pl.d_batches.as<const FactBatchDev>(), pl.d_tile_start.as<const uint64_t>(), nbat,
+                           pl.total_tiles, pl.d_partial.as<const AggPartial>());
         GJ_HIP_CHECK(hipGetLastError());
       } else {
-        for (const auto& b : pl.fact) launch_join_agg(d, b, pl.lo, pl.hi, pl.flag, ix.cu, s);
+        for (const auto& b : pl.fact)
+          launch_join_agg(d, b, pl.lo, pl.hi, pl.flag, ix.cu, pl.d_partial.as<AggPartial>(), s);
       }
     });
     return;
@@ -404,7 +514,9 @@ int gj_join_aggregate(const gj_index* index, const gj_fact_batch* batch, const g
     GJ_REQUIRE(params->d_flag, GJ_ERR_INVALID, "null flag column");
     const FactBatchDev b = to_dev(*batch, d_result);
     with_index_dev<uint32_t>(*index, [&](auto d) {
-      launch_join_agg(d, b, params->date_lo, params->date_hi, params->d_flag, index->cu, S(stream));
+      index->scratch.reserve(agg_partial_cap(index->cu) * sizeof(AggPartial));
+      launch_join_agg(d, b, params->date_lo, params->date_hi, params->d_flag, index->cu,
+                      index->scratch.as<AggPartial>(), S(stream));
     });
   });
 }
@@ -467,6 +579,8 @@ int gj_aggregate_plan_create(const gj_index* index, const gj_fact_batch* h_batch
       tiles += ceil_div(b.rows, kAggTileRows);
     }
     pl->total_tiles = tiles;
+    // per-workgroup sums of the per-batch launches, or per-tile sums of the persistent kernel
+    pl->d_partial.alloc(std::max<uint64_t>(agg_partial_cap(index->cu), tiles) * sizeof(AggPartial));
     finish_plan(*pl, S(stream));
     *out = pl.release();
   });
