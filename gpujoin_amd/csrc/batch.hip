@@ -9,7 +9,9 @@
 // cooperative index lookup always runs on a full wave although only ~1 % of
 // the fact rows pass the filter.  Only the date column is read for every row;
 // fkey, price and discount are read for qualifying rows only.  Sums are exact
-// int64: wave shuffle reduction, then one 64-bit atomic per wave.
+// int64: wave shuffle and workgroup reductions write one partial per
+// workgroup, and a one-block kernel adds the partials to the result (64-bit
+// atomics on one address from every wave would serialise).
 //
 // Plans run a fixed list of uneven batches: one launch per batch, the same
 // launches captured once into a hipGraph, or one persistent kernel whose waves
@@ -25,8 +27,9 @@ namespace gj {
 constexpr int kAggBlock = 256;
 constexpr int kAggUnroll = 8;                          // 64-row slices per wave step
 constexpr int kAggQueue = kWave * (kAggUnroll + 1);    // queued qualifying rows per wave
-constexpr uint64_t kAggTileRows = 8192;     // fact rows per persistent work item
-constexpr uint64_t kLookupTileRows = 16384; // probes per persistent work item
+constexpr uint64_t kAggWaves = kAggBlock / kWave;  // partial sums per persistent work item
+constexpr uint64_t kAggTileRows = 65536;    // fact rows per persistent work item (one workgroup)
+constexpr uint64_t kLookupTileRows = 65536; // probes per persistent work item (one workgroup)
 
 struct FactBatchDev {
   const int32_t* fkey;
@@ -133,7 +136,8 @@ __device__ __forceinline__ void agg_rows(const Dev& d, const uint8_t* lds, AggQu
   }
 }
 
-// Per-workgroup sums of the one-batch kernel (no result atomics; see agg_sum_kernel).
+// Per-workgroup sums of the one-batch kernel and per-wave sums of the persistent
+// one (no result atomics; see agg_sum_kernel).
 struct AggPartial {
   int64_t num, den;
   uint64_t matches, pad;
@@ -169,7 +173,9 @@ inline size_t agg_queue_bytes() { return (kAggBlock / kWave) * kAggQueue * (4 + 
 // One batch, grid-stride over 64-row steps.  Each workgroup writes its sums to
 // partial[blockIdx.x]; agg_sum_kernel adds them to the result.  (One 64-bit atomic
 // per wave onto the three result words serialises at their L2 channel: at ~4000
-// waves that took about half of the kernel's time.)
+// waves that took about half of the kernel's time.  Letting the last workgroup
+// add the partials instead needs a device-scope fence in every workgroup, whose
+// L2 write-back cost more than the second launch.)
 template <class Dev>
 __global__ __launch_bounds__(kAggBlock) void join_agg_kernel(Dev d, FactBatchDev b, int32_t lo, int32_t hi,
                                                              const uint8_t* __restrict__ flag,
@@ -240,24 +246,14 @@ __global__ __launch_bounds__(1024) void agg_sum_kernel(const AggPartial* __restr
   }
 }
 
-__device__ __forceinline__ uint32_t find_batch(const uint64_t* tile_start, uint32_t nb, uint64_t t) {
-  uint32_t lo = 0, hi = nb;  // last b with tile_start[b] <= t
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) / 2;
-    if (tile_start[mid] <= t) lo = mid;
-    else hi = mid;
-  }
-  return lo;
-}
-
 // Persistent join-aggregate: waves pull (batch, tile) items until the counter
 // passes total_tiles.  A tile's sums go to partial[tile] (each tile has one
 // owner, so a plain store); agg_batch_sum_kernel then adds each batch's tiles to
 // its result, so the waves issue no result atomics.
 template <class Dev>
 __global__ __launch_bounds__(kAggBlock) void join_agg_persistent_kernel(Dev d, const FactBatchDev* __restrict__ batches,
-                                                                        const uint64_t* __restrict__ tile_start,
-                                                                        uint32_t nb, uint64_t total_tiles, int32_t lo,
+                                                                        const uint32_t* __restrict__ tile_batch,
+                                                                        uint64_t total_tiles, int32_t lo,
                                                                         int32_t hi, const uint8_t* __restrict__ flag,
                                                                         unsigned long long* counter,
                                                                         AggPartial* __restrict__ partial) {
@@ -266,22 +262,40 @@ __global__ __launch_bounds__(kAggBlock) void join_agg_persistent_kernel(Dev d, c
   const size_t ilds = (d.lds_bytes() + 15) / 16 * 16;
   AggQueue q = wave_queue<Dev>(lds + ilds);
   __syncthreads();
-  for (;;) {
-    unsigned long long t = 0;
-    if (lane() == 0) t = atomicAdd(counter, 1ull);
-    t = shfl64(t, 0);
-    if (t >= total_tiles) break;
-    const uint32_t bi = find_batch(tile_start, nb, t);
+  // A workgroup claims an item of kAggTileRows rows when it is free and its
+  // waves interleave over it in 64 x kAggUnroll-row steps (claiming ahead would
+  // leave the last items with busy workgroups; one stream per wave over its own
+  // rows reads HBM worse than adjacent waves on adjacent rows).  tile_batch maps
+  // an item to its batch.  Fewer than 64 queued rows stay queued into the next
+  // item when it is of the same batch (their sums count in that item's
+  // partials), so the part-empty lookup round that ends a queue is paid once
+  // per batch change.  Wave w of item t writes partial[t * waves + w].
+  constexpr int kWaves = kAggBlock / kWave;
+  const int w = threadIdx.x / kWave;
+  const uint64_t span = static_cast<uint64_t>(kWave) * kAggUnroll;
+  __shared__ uint64_t claimed;
+  auto claim = [&]() -> uint64_t {
+    __syncthreads();  // every wave has read the previous item
+    if (threadIdx.x == 0) claimed = atomicAdd(counter, 1ull);
+    __syncthreads();
+    return claimed;
+  };
+  uint64_t t = claim();
+  while (t < total_tiles) {  // workgroup-uniform
+    const uint32_t bi = tile_batch[t];
     const FactBatchDev b = batches[bi];
     const uint64_t r0 = (t - b.tile_start) * kAggTileRows;
     const uint64_t r1 = std::min<uint64_t>(b.rows, r0 + kAggTileRows);
     AggAcc acc;
-    agg_rows(d, lds, q, b, r0, r1, static_cast<uint64_t>(kWave) * kAggUnroll, lo, hi, flag, acc);
-    while (q.count > 0) agg_drain(d, lds, q, q.count < kWave ? q.count : kWave, flag, acc);
+    agg_rows(d, lds, q, b, r0 + static_cast<uint64_t>(w) * span, r1, span * kWaves, lo, hi, flag, acc);
+    const uint64_t tn = claim();
+    if (tn >= total_tiles || tile_batch[tn] != bi)
+      while (q.count > 0) agg_drain(d, lds, q, q.count < kWave ? q.count : kWave, flag, acc);
     const int64_t num = wave_sum_i64(acc.num);
     const int64_t den = wave_sum_i64(acc.den);
     const int64_t mt = wave_sum_i64(static_cast<int64_t>(acc.matches));
-    if (lane() == 0) partial[t] = AggPartial{num, den, static_cast<uint64_t>(mt), 0};
+    if (lane() == 0) partial[t * kWaves + w] = AggPartial{num, den, static_cast<uint64_t>(mt), 0};
+    t = tn;
   }
 }
 
@@ -295,7 +309,7 @@ __global__ __launch_bounds__(256) void agg_batch_sum_kernel(const FactBatchDev* 
   const uint64_t t0 = tile_start[bw];
   const uint64_t t1 = bw + 1 < nb ? tile_start[bw + 1] : total_tiles;
   int64_t num = 0, den = 0, mt = 0;
-  for (uint64_t t = t0 + static_cast<uint64_t>(lane()); t < t1; t += kWave) {
+  for (uint64_t t = t0 * kAggWaves + static_cast<uint64_t>(lane()); t < t1 * kAggWaves; t += kWave) {
     num += partial[t].num;
     den += partial[t].den;
     mt += static_cast<int64_t>(partial[t].matches);
@@ -314,22 +328,28 @@ __global__ __launch_bounds__(256) void agg_batch_sum_kernel(const FactBatchDev* 
 // Persistent lookups over many probe batches.
 template <class Dev, class K>
 __global__ __launch_bounds__(256) void lookup_persistent_kernel(Dev d, const ProbeBatchDev* __restrict__ batches,
-                                                                const uint64_t* __restrict__ tile_start, uint32_t nb,
+                                                                const uint32_t* __restrict__ tile_batch,
                                                                 uint64_t total_tiles, unsigned long long* counter) {
   extern __shared__ __attribute__((aligned(16))) uint8_t lds[];
   d.stage(lds);
   __syncthreads();
-  for (;;) {
-    unsigned long long t = 0;
-    if (lane() == 0) t = atomicAdd(counter, 1ull);
-    t = shfl64(t, 0);
-    if (t >= total_tiles) break;
-    const uint32_t bi = find_batch(tile_start, nb, t);
-    const ProbeBatchDev b = batches[bi];
+  // as in the join-aggregate: a free workgroup claims an item and its waves
+  // interleave over it 64 probes at a time
+  const uint64_t w = threadIdx.x / kWave;
+  const uint64_t step = blockDim.x;
+  __shared__ uint64_t claimed;
+  auto claim = [&]() -> uint64_t {
+    __syncthreads();  // every wave has read the previous item
+    if (threadIdx.x == 0) claimed = atomicAdd(counter, 1ull);
+    __syncthreads();
+    return claimed;
+  };
+  for (uint64_t t = claim(); t < total_tiles; t = claim()) {  // workgroup-uniform
+    const ProbeBatchDev b = batches[tile_batch[t]];
     const K* probes = static_cast<const K*>(b.probes);
     const uint64_t r0 = (t - b.tile_start) * kLookupTileRows;
     const uint64_t r1 = std::min<uint64_t>(b.m, r0 + kLookupTileRows);
-    for (uint64_t base = r0; base < r1; base += kWave) {
+    for (uint64_t base = r0 + w * kWave; base < r1; base += step) {
       const uint64_t i = base + static_cast<uint64_t>(lane());
       const bool active = i < r1;
       const K x = active ? load_stream(probes + i) : K(0);
@@ -391,6 +411,7 @@ struct gj_batch_plan {
   gj_agg_result* results = nullptr;
   // device copies (persistent)
   dbuf d_batches, d_tile_start, d_counter;
+  dbuf d_tile_batch;  // batch of each persistent work item
   dbuf d_partial;  // workgroup partial sums of the per-batch join-aggregate launches
   // graph
   hipGraph_t graph = nullptr;
@@ -417,11 +438,11 @@ void enqueue_plan(gj_batch_plan& pl, hipStream_t s) {
         const size_t lds = (d.lds_bytes() + 15) / 16 * 16 + agg_queue_bytes();
         const void* fn = reinterpret_cast<const void*>(&join_agg_persistent_kernel<Dev>);
         uint64_t blocks = static_cast<uint64_t>(blocks_per_cu(fn, kAggBlock, lds)) * ix.cu;
-        blocks = std::min<uint64_t>(blocks, ceil_div(pl.total_tiles, kAggBlock / kWave));
+        blocks = std::min<uint64_t>(blocks, pl.total_tiles);
         blocks = std::max<uint64_t>(blocks, 1);
         hipLaunchKernelGGL((join_agg_persistent_kernel<Dev>), dim3(static_cast<unsigned>(blocks)), dim3(kAggBlock),
-                           lds, s, d, pl.d_batches.as<const FactBatchDev>(), pl.d_tile_start.as<const uint64_t>(),
-                           static_cast<uint32_t>(pl.fact.size()), pl.total_tiles, pl.lo, pl.hi, pl.flag,
+                           lds, s, d, pl.d_batches.as<const FactBatchDev>(), pl.d_tile_batch.as<const uint32_t>(),
+                           pl.total_tiles, pl.lo, pl.hi, pl.flag,
                            pl.d_counter.as<unsigned long long>(), pl.d_partial.as<AggPartial>());
         GJ_HIP_CHECK(hipGetLastError());
         const uint32_t nbat = static_cast<uint32_t>(pl.fact.size());
@@ -445,11 +466,11 @@ void enqueue_plan(gj_batch_plan& pl, hipStream_t s) {
         const size_t lds = d.lds_bytes();
         const void* fn = reinterpret_cast<const void*>(&lookup_persistent_kernel<Dev, K>);
         uint64_t blocks = static_cast<uint64_t>(blocks_per_cu(fn, 256, lds)) * ix.cu;
-        blocks = std::min<uint64_t>(blocks, ceil_div(pl.total_tiles, 4));
+        blocks = std::min<uint64_t>(blocks, pl.total_tiles);
         blocks = std::max<uint64_t>(blocks, 1);
         hipLaunchKernelGGL((lookup_persistent_kernel<Dev, K>), dim3(static_cast<unsigned>(blocks)), dim3(256), lds, s,
-                           d, pl.d_batches.as<const ProbeBatchDev>(), pl.d_tile_start.as<const uint64_t>(),
-                           static_cast<uint32_t>(pl.probe.size()), pl.total_tiles,
+                           d, pl.d_batches.as<const ProbeBatchDev>(), pl.d_tile_batch.as<const uint32_t>(),
+                           pl.total_tiles,
                            pl.d_counter.as<unsigned long long>());
         GJ_HIP_CHECK(hipGetLastError());
       } else {
@@ -468,11 +489,19 @@ void finish_plan(gj_batch_plan& pl, hipStream_t s) {
     pl.d_batches.alloc(std::max<size_t>(nb * esz, 16));
     pl.d_tile_start.alloc(std::max<size_t>(nb * 8, 16));
     pl.d_counter.alloc(16);
+    std::vector<uint32_t> tile_batch(pl.total_tiles);
+    for (size_t b = 0; b < nb; ++b) {
+      const uint64_t end = b + 1 < nb ? pl.tile_start[b + 1] : pl.total_tiles;
+      for (uint64_t t = pl.tile_start[b]; t < end; ++t) tile_batch[t] = static_cast<uint32_t>(b);
+    }
+    pl.d_tile_batch.alloc(std::max<size_t>(tile_batch.size() * 4, 16));
     const void* src = pl.aggregate ? static_cast<const void*>(pl.fact.data()) : static_cast<const void*>(pl.probe.data());
     if (nb) {
       GJ_HIP_CHECK(hipMemcpyAsync(pl.d_batches.p, src, nb * esz, hipMemcpyHostToDevice, s));
       GJ_HIP_CHECK(hipMemcpyAsync(pl.d_tile_start.p, pl.tile_start.data(), nb * 8, hipMemcpyHostToDevice, s));
     }
+    if (!tile_batch.empty())
+      GJ_HIP_CHECK(hipMemcpyAsync(pl.d_tile_batch.p, tile_batch.data(), tile_batch.size() * 4, hipMemcpyHostToDevice, s));
     GJ_HIP_CHECK(hipStreamSynchronize(s));
   } else if (pl.mode == GJ_BATCH_GRAPH) {
     // capture needs a non-null stream
@@ -580,7 +609,7 @@ int gj_aggregate_plan_create(const gj_index* index, const gj_fact_batch* h_batch
     }
     pl->total_tiles = tiles;
     // per-workgroup sums of the per-batch launches, or per-tile sums of the persistent kernel
-    pl->d_partial.alloc(std::max<uint64_t>(agg_partial_cap(index->cu), tiles) * sizeof(AggPartial));
+    pl->d_partial.alloc(std::max<uint64_t>(agg_partial_cap(index->cu), tiles * kAggWaves) * sizeof(AggPartial));
     finish_plan(*pl, S(stream));
     *out = pl.release();
   });
