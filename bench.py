@@ -112,6 +112,8 @@ def main():
                     help="pick each index's own parameters with the index autotuner before building it")
     ap.add_argument("--suite", action="store_true", help="measure the other workload configurations instead")
     ap.add_argument("--quick", action="store_true", help="--suite at reduced sizes")
+    ap.add_argument("--join-aggregate", action="store_true",
+                    help="only the suite's join-aggregate measurements (for rocprofv3 runs)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -123,6 +125,11 @@ def main():
 
     info = gj.device_info()
     log("device", info)
+    if args.join_aggregate:
+        # the suite's join-aggregate measurements alone (one JSON object per line)
+        from gpujoin_amd import suite
+        suite.run_join_aggregate(gj, quick=args.quick, cpu=not args.no_cpu)
+        return
     n = 1 << args.log2_n
     m = 1 << args.log2_m
     threads = 0

@@ -236,8 +236,18 @@ def run(gj, G, quick=False, cpu=True, emit=None):
         del dk4, k4, p4, e4
     log("learned sweeps done")
     del dp, out
+    run_join_aggregate(gj, quick=quick, cpu=cpu, emit=emit, stream=stream)
 
-    # ---- join-aggregate -------------------------------------------------------
+
+def run_join_aggregate(gj, quick=False, cpu=True, emit=None, stream=None):
+    """The join-aggregate measurements alone: one batch, then log-uniform batches
+    through per-batch launches, a replayed hipGraph and the persistent kernel."""
+    import oracle
+
+    emit = emit or (lambda d: print(json.dumps(d), flush=True))
+    stream = stream or gj.Stream()
+    cores = os.cpu_count() or 1
+    log = lambda *a: print("[suite]", *a, file=sys.stderr, flush=True)
     dim_rows = 2_000_000
     fact_rows = 6_000_000 if quick else 60_000_000
     key, flag = gj.gen_dimension(dim_rows, 0x41)
