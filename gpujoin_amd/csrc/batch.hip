@@ -28,6 +28,7 @@ constexpr int kAggBlock = 256;
 constexpr int kAggUnroll = 8;                          // 64-row slices per wave step
 constexpr int kAggQueue = kWave * (kAggUnroll + 1);    // queued qualifying rows per wave
 constexpr uint64_t kAggWaves = kAggBlock / kWave;  // partial sums per persistent work item
+constexpr uint64_t kAggDirectBlocks = 128;         // grids up to this size skip the partial-sum kernel
 constexpr uint64_t kAggTileRows = 65536;    // fact rows per persistent work item (one workgroup)
 constexpr uint64_t kLookupTileRows = 65536; // probes per persistent work item (one workgroup)
 
@@ -179,7 +180,7 @@ inline size_t agg_queue_bytes() { return (kAggBlock / kWave) * kAggQueue * (4 + 
 template <class Dev>
 __global__ __launch_bounds__(kAggBlock) void join_agg_kernel(Dev d, FactBatchDev b, int32_t lo, int32_t hi,
                                                              const uint8_t* __restrict__ flag,
-                                                             AggPartial* __restrict__ partial) {
+                                                             AggPartial* __restrict__ partial, bool direct) {
   extern __shared__ __attribute__((aligned(16))) uint8_t lds[];
   __shared__ int64_t red[kAggBlock / kWave][3];
   d.stage(lds);
@@ -209,7 +210,13 @@ __global__ __launch_bounds__(kAggBlock) void join_agg_kernel(Dev d, FactBatchDev
       p.den += red[v][1];
       p.matches += static_cast<uint64_t>(red[v][2]);
     }
-    partial[blockIdx.x] = p;
+    if (!direct) {
+      partial[blockIdx.x] = p;
+    } else if ((p.num | p.den | static_cast<int64_t>(p.matches)) != 0) {
+      atomicAdd(reinterpret_cast<unsigned long long*>(&b.result->numerator), static_cast<unsigned long long>(p.num));
+      atomicAdd(reinterpret_cast<unsigned long long*>(&b.result->denominator), static_cast<unsigned long long>(p.den));
+      atomicAdd(reinterpret_cast<unsigned long long*>(&b.result->matches), static_cast<unsigned long long>(p.matches));
+    }
   }
 }
 
@@ -374,9 +381,13 @@ void launch_join_agg(const Dev& d, const FactBatchDev& b, int32_t lo, int32_t hi
   blocks = std::min<uint64_t>(blocks, ceil_div(b.rows, static_cast<uint64_t>(kAggBlock) * kAggUnroll));
   blocks = std::max<uint64_t>(blocks, 1);
   GJ_REQUIRE(blocks <= agg_partial_cap(cu), GJ_ERR_INTERNAL, "join-aggregate grid exceeds the partial buffer");
+  // a small grid adds its workgroup sums to the result itself (a few dozen atomics
+  // cost less than the second launch)
+  const bool direct = blocks <= kAggDirectBlocks;
   hipLaunchKernelGGL((join_agg_kernel<Dev>), dim3(static_cast<unsigned>(blocks)), dim3(kAggBlock), lds, s, d, b, lo, hi,
-                     flag, partial);
+                     flag, partial, direct);
   GJ_HIP_CHECK(hipGetLastError());
+  if (direct) return;
   hipLaunchKernelGGL(agg_sum_kernel, dim3(1), dim3(1024), 0, s, partial, static_cast<uint32_t>(blocks), b.result);
   GJ_HIP_CHECK(hipGetLastError());
 }

@@ -50,6 +50,27 @@ def test_join_aggregate_matches_oracle(gj, orc, gpu, kind):
     assert gj.promo_revenue(num, den) == orc.promo_revenue(onum, oden)
 
 
+@pytest.mark.parametrize("rows", [1, 2048, 5000, 128 * 2048, 128 * 2048 + 1, 140 * 2048])
+def test_join_aggregate_small_grids(gj, orc, gpu, rows):
+    # up to 128 workgroups of 2048 rows add their sums to the result themselves;
+    # larger grids go through the partial buffer and the sum kernel
+    dim_rows = 32 * 500
+    skey, sflag = _dimension(gj, dim_rows, seed=0x45)
+    fkey, date, price, disc = _fact(gj, rows, dim_rows + dim_rows // 4, seed=0x46 + rows)
+    lo, hi = 0, 400  # about one row in six qualifies
+    col, idx = _agg_index(gj, "btree", skey)
+    d = [gj.DeviceArray.from_numpy(a) for a in (fkey, date, price, disc)]
+    dflag = gj.DeviceArray.from_numpy(sflag)
+    res = gj.DeviceArray(4, np.int64)
+    for rep in range(2):  # the second call adds to the first call's result
+        if rep == 0:
+            res.fill(0)
+        gj.join_aggregate(idx, *d, dflag, lo, hi, res)
+    gj.synchronize()
+    onum, oden, omt = orc.join_aggregate(skey, sflag, fkey, date, price, disc, lo, hi)
+    assert tuple(int(v) for v in res.numpy()[:3]) == (2 * onum, 2 * oden, 2 * omt)
+
+
 def test_promo_revenue_closed_form(gj, orc):
     # process_results (src/tpch/tpch_14_common.cu:124-127): 100 * (num*1000) / (den/1000)
     for num, den in [(0, 0), (1, 999), (16380778, 100000000), (123456789012, 987654321098)]:
