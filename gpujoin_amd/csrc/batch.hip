@@ -25,8 +25,12 @@
 namespace gj {
 
 constexpr int kAggBlock = 256;
-constexpr int kAggUnroll = 8;                          // 64-row slices per wave step
-constexpr int kAggQueue = kWave * (kAggUnroll + 1);    // queued qualifying rows per wave
+// 64-row slices per wave step: the one-batch kernel runs more workgroups per CU
+// with the smaller queue of 4; the persistent kernel's few workgroups want 8
+constexpr int kAggUnrollBatch = 4;
+constexpr int kAggUnrollTile = 8;
+// queued qualifying rows per wave: a full step plus fewer than 64 left over
+constexpr int agg_queue_rows(int unroll) { return kWave * (unroll + 1); }
 constexpr uint64_t kAggWaves = kAggBlock / kWave;  // partial sums per persistent work item
 constexpr uint64_t kAggDirectBlocks = 128;         // grids up to this size skip the partial-sum kernel
 constexpr uint64_t kAggTileRows = 65536;    // fact rows per persistent work item (one workgroup)
@@ -84,35 +88,35 @@ __device__ __forceinline__ void agg_drain(const Dev& d, const uint8_t* lds, AggQ
   q.count -= take;
 }
 
-// Rows of one fact batch: the wave takes kAggUnroll x 64 rows per step (row
+// Rows of one fact batch: the wave takes U x 64 rows per step (row
 // base + k * 64 + lane), starting at r0 and advancing by `step`, below r1.  All
 // date loads of a step are issued together, then the fkey / price / discount
 // loads of every qualifying row, so a step costs two memory latencies however
 // many slices qualify; qualifying rows queue in LDS and are looked up 64 at a time.
-template <class Dev>
+template <int U, class Dev>
 __device__ __forceinline__ void agg_rows(const Dev& d, const uint8_t* lds, AggQueue& q, const FactBatchDev& b,
                                          uint64_t r0, uint64_t r1, uint64_t step, int32_t lo, int32_t hi,
                                          const uint8_t* __restrict__ flag, AggAcc& acc) {
   const int ln = lane();
   const uint64_t lt = (ln == 0) ? 0ull : (~0ull >> (64 - ln));
   for (uint64_t base = r0; base < r1; base += step) {
-    int32_t dt[kAggUnroll];
+    int32_t dt[U];
 #pragma unroll
-    for (int k = 0; k < kAggUnroll; ++k) {
+    for (int k = 0; k < U; ++k) {
       const uint64_t i = base + static_cast<uint64_t>(k * kWave + ln);
       dt[k] = i < r1 ? load_stream(b.date + i) : 0;
     }
     uint32_t qm = 0;  // bit k: row of slice k qualifies
 #pragma unroll
-    for (int k = 0; k < kAggUnroll; ++k) {
+    for (int k = 0; k < U; ++k) {
       const uint64_t i = base + static_cast<uint64_t>(k * kWave + ln);
       qm |= ((i < r1 && dt[k] >= lo && dt[k] < hi) ? 1u : 0u) << k;
     }
     if (ballot(qm != 0) == 0) continue;  // wave-uniform
-    uint32_t fk[kAggUnroll];
-    int64_t sv[kAggUnroll];
+    uint32_t fk[U];
+    int64_t sv[U];
 #pragma unroll
-    for (int k = 0; k < kAggUnroll; ++k) {
+    for (int k = 0; k < U; ++k) {
       fk[k] = 0;
       sv[k] = 0;
       if ((qm >> k) & 1u) {
@@ -122,7 +126,7 @@ __device__ __forceinline__ void agg_rows(const Dev& d, const uint8_t* lds, AggQu
       }
     }
 #pragma unroll
-    for (int k = 0; k < kAggUnroll; ++k) {
+    for (int k = 0; k < U; ++k) {
       const bool qual = (qm >> k) & 1u;
       const uint64_t mask = ballot(qual);
       if (qual) {
@@ -159,17 +163,18 @@ __device__ __forceinline__ void agg_flush(const Dev& d, const uint8_t* lds, AggQ
   acc = AggAcc{};
 }
 
-template <class Dev>
+template <int U>
 __device__ __forceinline__ AggQueue wave_queue(uint8_t* lds_tail) {
+  constexpr int rows = agg_queue_rows(U);
   const int w = threadIdx.x / kWave;
   AggQueue q;
-  q.key = reinterpret_cast<uint32_t*>(lds_tail) + w * kAggQueue;
-  q.val = reinterpret_cast<int64_t*>(lds_tail + (kAggBlock / kWave) * kAggQueue * 4) + w * kAggQueue;
+  q.key = reinterpret_cast<uint32_t*>(lds_tail) + w * rows;
+  q.val = reinterpret_cast<int64_t*>(lds_tail + (kAggBlock / kWave) * rows * 4) + w * rows;
   q.count = 0;
   return q;
 }
 
-inline size_t agg_queue_bytes() { return (kAggBlock / kWave) * kAggQueue * (4 + 8); }
+inline size_t agg_queue_bytes(int unroll) { return (kAggBlock / kWave) * agg_queue_rows(unroll) * (4 + 8); }
 
 // One batch, grid-stride over 64-row steps.  Each workgroup writes its sums to
 // partial[blockIdx.x]; agg_sum_kernel adds them to the result.  (One 64-bit atomic
@@ -185,14 +190,14 @@ __global__ __launch_bounds__(kAggBlock) void join_agg_kernel(Dev d, FactBatchDev
   __shared__ int64_t red[kAggBlock / kWave][3];
   d.stage(lds);
   const size_t ilds = (d.lds_bytes() + 15) / 16 * 16;
-  AggQueue q = wave_queue<Dev>(lds + ilds);
+  AggQueue q = wave_queue<kAggUnrollBatch>(lds + ilds);
   __syncthreads();
   const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (blockDim.x / kWave);
   const int w = threadIdx.x / kWave;
   const uint64_t wave = static_cast<uint64_t>(blockIdx.x) * (blockDim.x / kWave) + w;
   AggAcc acc;
-  const uint64_t span = static_cast<uint64_t>(kWave) * kAggUnroll;
-  agg_rows(d, lds, q, b, wave * span, b.rows, waves * span, lo, hi, flag, acc);
+  const uint64_t span = static_cast<uint64_t>(kWave) * kAggUnrollBatch;
+  agg_rows<kAggUnrollBatch>(d, lds, q, b, wave * span, b.rows, waves * span, lo, hi, flag, acc);
   while (q.count > 0) agg_drain(d, lds, q, q.count < kWave ? q.count : kWave, flag, acc);
   const int64_t num = wave_sum_i64(acc.num);
   const int64_t den = wave_sum_i64(acc.den);
@@ -267,10 +272,10 @@ __global__ __launch_bounds__(kAggBlock) void join_agg_persistent_kernel(Dev d, c
   extern __shared__ __attribute__((aligned(16))) uint8_t lds[];
   d.stage(lds);
   const size_t ilds = (d.lds_bytes() + 15) / 16 * 16;
-  AggQueue q = wave_queue<Dev>(lds + ilds);
+  AggQueue q = wave_queue<kAggUnrollTile>(lds + ilds);
   __syncthreads();
   // A workgroup claims an item of kAggTileRows rows when it is free and its
-  // waves interleave over it in 64 x kAggUnroll-row steps (claiming ahead would
+  // waves interleave over it in 64 x kAggUnrollTile-row steps (claiming ahead would
   // leave the last items with busy workgroups; one stream per wave over its own
   // rows reads HBM worse than adjacent waves on adjacent rows).  tile_batch maps
   // an item to its batch.  Fewer than 64 queued rows stay queued into the next
@@ -279,7 +284,7 @@ __global__ __launch_bounds__(kAggBlock) void join_agg_persistent_kernel(Dev d, c
   // per batch change.  Wave w of item t writes partial[t * waves + w].
   constexpr int kWaves = kAggBlock / kWave;
   const int w = threadIdx.x / kWave;
-  const uint64_t span = static_cast<uint64_t>(kWave) * kAggUnroll;
+  const uint64_t span = static_cast<uint64_t>(kWave) * kAggUnrollTile;
   __shared__ uint64_t claimed;
   auto claim = [&]() -> uint64_t {
     __syncthreads();  // every wave has read the previous item
@@ -294,7 +299,7 @@ __global__ __launch_bounds__(kAggBlock) void join_agg_persistent_kernel(Dev d, c
     const uint64_t r0 = (t - b.tile_start) * kAggTileRows;
     const uint64_t r1 = std::min<uint64_t>(b.rows, r0 + kAggTileRows);
     AggAcc acc;
-    agg_rows(d, lds, q, b, r0 + static_cast<uint64_t>(w) * span, r1, span * kWaves, lo, hi, flag, acc);
+    agg_rows<kAggUnrollTile>(d, lds, q, b, r0 + static_cast<uint64_t>(w) * span, r1, span * kWaves, lo, hi, flag, acc);
     const uint64_t tn = claim();
     if (tn >= total_tiles || tile_batch[tn] != bi)
       while (q.count > 0) agg_drain(d, lds, q, q.count < kWave ? q.count : kWave, flag, acc);
@@ -375,10 +380,10 @@ template <class Dev>
 void launch_join_agg(const Dev& d, const FactBatchDev& b, int32_t lo, int32_t hi, const uint8_t* flag, int cu,
                      AggPartial* partial, hipStream_t s) {
   if (b.rows == 0) return;
-  const size_t lds = (d.lds_bytes() + 15) / 16 * 16 + agg_queue_bytes();
+  const size_t lds = (d.lds_bytes() + 15) / 16 * 16 + agg_queue_bytes(kAggUnrollBatch);
   const void* fn = reinterpret_cast<const void*>(&join_agg_kernel<Dev>);
   uint64_t blocks = static_cast<uint64_t>(blocks_per_cu(fn, kAggBlock, lds)) * cu;
-  blocks = std::min<uint64_t>(blocks, ceil_div(b.rows, static_cast<uint64_t>(kAggBlock) * kAggUnroll));
+  blocks = std::min<uint64_t>(blocks, ceil_div(b.rows, static_cast<uint64_t>(kAggBlock) * kAggUnrollBatch));
   blocks = std::max<uint64_t>(blocks, 1);
   GJ_REQUIRE(blocks <= agg_partial_cap(cu), GJ_ERR_INTERNAL, "join-aggregate grid exceeds the partial buffer");
   // a small grid adds its workgroup sums to the result itself (a few dozen atomics
@@ -446,7 +451,7 @@ void enqueue_plan(gj_batch_plan& pl, hipStream_t s) {
       if (pl.mode == GJ_BATCH_PERSISTENT) {
         fill_bytes(pl.d_counter.p, 0, 16, s);
         if (pl.total_tiles == 0) return;
-        const size_t lds = (d.lds_bytes() + 15) / 16 * 16 + agg_queue_bytes();
+        const size_t lds = (d.lds_bytes() + 15) / 16 * 16 + agg_queue_bytes(kAggUnrollTile);
         const void* fn = reinterpret_cast<const void*>(&join_agg_persistent_kernel<Dev>);
         uint64_t blocks = static_cast<uint64_t>(blocks_per_cu(fn, kAggBlock, lds)) * ix.cu;
         blocks = std::min<uint64_t>(blocks, pl.total_tiles);

@@ -50,9 +50,10 @@ def test_join_aggregate_matches_oracle(gj, orc, gpu, kind):
     assert gj.promo_revenue(num, den) == orc.promo_revenue(onum, oden)
 
 
-@pytest.mark.parametrize("rows", [1, 2048, 5000, 128 * 2048, 128 * 2048 + 1, 140 * 2048])
+@pytest.mark.parametrize("rows", [1, 1024, 5000, 128 * 1024, 128 * 1024 + 1, 140 * 1024])
 def test_join_aggregate_small_grids(gj, orc, gpu, rows):
-    # up to 128 workgroups of 2048 rows add their sums to the result themselves;
+    # up to 128 workgroups (1024 rows per workgroup step) add their sums to the
+    # result themselves;
     # larger grids go through the partial buffer and the sum kernel
     dim_rows = 32 * 500
     skey, sflag = _dimension(gj, dim_rows, seed=0x45)
