@@ -567,11 +567,13 @@ int gj_join_aggregate(const gj_index* index, const gj_fact_batch* batch, const g
 }
 
 int64_t gj_promo_revenue(int64_t numerator, int64_t denominator) {
-  // process_results, src/tpch/tpch_14_common.cu:124-127
-  numerator *= 1000;
+  // process_results, src/tpch/tpch_14_common.cu:124-127.  Its int64 products
+  // overflow for large sums; they wrap here in two's complement (defined, through
+  // uint64), as the reference's build does in practice.
+  const int64_t n = static_cast<int64_t>(static_cast<uint64_t>(numerator) * 1000u);
   denominator /= 1000;
   if (denominator == 0) return 0;
-  return 100 * numerator / denominator;
+  return static_cast<int64_t>(static_cast<uint64_t>(n) * 100u) / denominator;
 }
 
 int gj_lookup_plan_create(const gj_index* index, const gj_probe_batch* h_batches, uint32_t num_batches,

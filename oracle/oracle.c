@@ -734,11 +734,13 @@ void orc_join_aggregate(const uint32_t* dim_sorted_key, const uint8_t* dim_flag,
   *matches = mt;
 }
 
+/* process_results (src/tpch/tpch_14_common.cu:124-127); its int64 products wrap in
+ * two's complement for large sums (computed through uint64, so defined here). */
 int64_t orc_promo_revenue(int64_t num, int64_t den) {
-  num *= 1000;
+  const int64_t n = (int64_t)((uint64_t)num * 1000u);
   den /= 1000;
   if (den == 0) return 0;
-  return 100 * num / den;
+  return (int64_t)((uint64_t)n * 100u) / den;
 }
 
 /* ------------------------------------------------------------------------

@@ -72,13 +72,6 @@ def test_join_aggregate_small_grids(gj, orc, gpu, rows):
     assert tuple(int(v) for v in res.numpy()[:3]) == (2 * onum, 2 * oden, 2 * omt)
 
 
-def test_promo_revenue_closed_form(gj, orc):
-    # process_results (src/tpch/tpch_14_common.cu:124-127): 100 * (num*1000) / (den/1000)
-    for num, den in [(0, 0), (1, 999), (16380778, 100000000), (123456789012, 987654321098)]:
-        assert gj.promo_revenue(num, den) == orc.promo_revenue(num, den)
-    assert gj.promo_revenue(1, 2) == 0
-
-
 def test_join_aggregate_all_rows_closed_form(gj, gpu):
     # every fact row qualifies and every dimension row is flagged: num == den == sum(price*(100-disc))
     dim_rows = 4096

@@ -89,6 +89,28 @@ def _host_line(out):
     return tuple(int(v) for v in m.groups())
 
 
+def _wrap64(v):
+    v &= (1 << 64) - 1
+    return v - (1 << 64) if v >> 63 else v
+
+
+def _c_div(a, b):  # C integer division truncates toward zero
+    q = abs(a) // abs(b)
+    return q if (a >= 0) == (b >= 0) else -q
+
+
+def test_promo_revenue_closed_form(gj, orc):
+    # process_results (src/tpch/tpch_14_common.cu:124-127): 100 * (num*1000) / (den/1000)
+    # in int64, whose products wrap for large sums (the last two cases)
+    cases = [(0, 0), (1, 999), (16380778, 100000000), (123456789012, 987654321098),
+             (70_000_000_000_000_000, 90_000_000_000_000_000), (-(1 << 62), 5_000_000)]
+    for num, den in cases:
+        d = _c_div(den, 1000)
+        want = 0 if d == 0 else _c_div(_wrap64(100 * _wrap64(num * 1000)), d)
+        assert gj.promo_revenue(num, den) == want == orc.promo_revenue(num, den), (num, den)
+    assert gj.promo_revenue(1, 2) == 0
+
+
 def test_julian_day_window():
     # the reference's constants (src/tpch/config.hpp:36-37) from the calendar
     assert _jdn(datetime.date(1995, 9, 1)) == 2449962
