@@ -97,14 +97,16 @@ def _log_uniform_sizes(rng, total_max, lo=1e3, hi=1e6):
 
 
 @pytest.mark.parametrize("mode", [_lib.BATCH_LAUNCH, _lib.BATCH_GRAPH, _lib.BATCH_PERSISTENT])
-def test_aggregate_plan_batches(gj, orc, gpu, mode):
+@pytest.mark.parametrize("kind", ["learned_plr", "hash", "btree"])
+def test_aggregate_plan_batches(gj, orc, gpu, mode, kind):
+    print(f"[case] aggregate plan mode={mode} kind={kind}", flush=True)
     dim_rows = 32 * 2000
     skey, sflag = _dimension(gj, dim_rows)
     rng = np.random.default_rng(mode)
     sizes = _log_uniform_sizes(rng, 3_000_000)
     rows = sum(sizes)
     fkey, date, price, disc = _fact(gj, rows, dim_rows, seed=0x43)
-    col, idx = _agg_index(gj, "learned_plr", skey)
+    col, idx = _agg_index(gj, kind, skey)
     d = [gj.DeviceArray.from_numpy(a) for a in (fkey, date, price, disc)]
     dflag = gj.DeviceArray.from_numpy(sflag)
     lo, hi = 500, 530
@@ -135,14 +137,16 @@ def test_aggregate_plan_batches(gj, orc, gpu, mode):
 
 @pytest.mark.parametrize("mode", [_lib.BATCH_LAUNCH, _lib.BATCH_GRAPH, _lib.BATCH_PERSISTENT])
 @pytest.mark.parametrize("kind", ["btree", "learned_plr", "harmonia"])
-def test_lookup_plan_batches(gj, orc, gpu, mode, kind):
-    keys = gj.gen_build_keys(32 * 8192, 8, _lib.KEYS_UNIFORM, 63, 0x51)
+@pytest.mark.parametrize("kb", [4, 8])
+def test_lookup_plan_batches(gj, orc, gpu, mode, kind, kb):
+    print(f"[case] lookup plan mode={mode} kind={kind} kb={kb}", flush=True)
+    keys = gj.gen_build_keys(32 * 8192, kb, _lib.KEYS_UNIFORM, 63 if kb == 8 else 0, 0x51)
     col = gj.DeviceArray.from_numpy(keys)
     idx = gj.Index(kind, col, host_column=keys, params=gj.index_params(learned_leaf_bits=10))
     rng = np.random.default_rng(7 + mode)
     sizes = _log_uniform_sizes(rng, 2_000_000) + [0, 1]
     m = sum(sizes)
-    probes, _ = gj.gen_probes(keys, m, _lib.PROBES_UNIFORM, 1.0, 0.1, 63, 0x52)
+    probes, _ = gj.gen_probes(keys, m, _lib.PROBES_UNIFORM, 1.0, 0.1, 63 if kb == 8 else 0, 0x52)
     dp = gj.DeviceArray.from_numpy(probes)
     out = gj.DeviceArray(m, np.uint32)
     batches, off = [], 0
