@@ -14,8 +14,9 @@
 // atomics on one address from every wave would serialise).
 //
 // Plans run a fixed list of uneven batches: one launch per batch, the same
-// launches captured once into a hipGraph, or one persistent kernel whose waves
-// pull (batch, tile) work items from a device counter until all are done.
+// launches captured once into a hipGraph, or one persistent kernel whose
+// workgroups pull (batch, rows) work items from a device counter until all are
+// done.
 #include <algorithm>
 #include <vector>
 
@@ -260,9 +261,9 @@ __global__ __launch_bounds__(1024) void agg_sum_kernel(const AggPartial* __restr
   }
 }
 
-// Persistent join-aggregate: waves pull (batch, tile) items until the counter
-// passes total_tiles.  A tile's sums go to partial[tile] (each tile has one
-// owner, so a plain store); agg_batch_sum_kernel then adds each batch's tiles to
+// Persistent join-aggregate: workgroups pull work items until the counter passes
+// total_tiles.  Each wave's sums over an item go to its own partial slot (one
+// owner, so a plain store); agg_batch_sum_kernel then adds each batch's slots to
 // its result, so the waves issue no result atomics.
 template <class Dev>
 __global__ __launch_bounds__(kAggBlock) void join_agg_persistent_kernel(Dev d, const FactBatchDev* __restrict__ batches,
@@ -313,7 +314,7 @@ __global__ __launch_bounds__(kAggBlock) void join_agg_persistent_kernel(Dev d, c
   }
 }
 
-// One wave per batch: the sum of its tiles' partials, added to its result.
+// One wave per batch: the sum of its items' per-wave partials, added to its result.
 __global__ __launch_bounds__(256) void agg_batch_sum_kernel(const FactBatchDev* __restrict__ batches,
                                                             const uint64_t* __restrict__ tile_start, uint32_t nb,
                                                             uint64_t total_tiles,
