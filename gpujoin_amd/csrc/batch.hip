@@ -34,6 +34,7 @@ constexpr int kAggUnrollTile = 8;
 constexpr int agg_queue_rows(int unroll) { return kWave * (unroll + 1); }
 constexpr uint64_t kAggWaves = kAggBlock / kWave;  // partial sums per persistent work item
 constexpr uint64_t kAggDirectBlocks = 128;         // grids up to this size skip the partial-sum kernel
+constexpr int kGraphLanes = 4;  // streams a graph plan's captured launches are spread over
 constexpr uint64_t kAggTileRows = 65536;    // fact rows per persistent work item (one workgroup)
 // probes per persistent work item (one workgroup): small items keep the last
 // round of items short (65536 left about 2.7 items per workgroup and lost 6 %)
@@ -444,8 +445,11 @@ struct gj_batch_plan {
 namespace gj {
 namespace {
 
-// Enqueues one execution of the plan (no graph replay) on s.
-void enqueue_plan(gj_batch_plan& pl, hipStream_t s) {
+// Enqueues one execution of the plan (no graph replay) on s.  With `lanes`
+// (graph capture), batch i goes to lanes[i % size] instead, so the captured
+// launches may overlap: lookup batches share no scratch, and the aggregate
+// launches of lane j use their own slice j of the partial buffer.
+void enqueue_plan(gj_batch_plan& pl, hipStream_t s, const std::vector<hipStream_t>* lanes = nullptr) {
   const gj_index& ix = *pl.ix;
   if (pl.aggregate) {
     fill_bytes(pl.results, 0, pl.fact.size() * sizeof(gj_agg_result), s);
@@ -470,8 +474,12 @@ void enqueue_plan(gj_batch_plan& pl, hipStream_t s) {
                            pl.total_tiles, pl.d_partial.as<const AggPartial>());
         GJ_HIP_CHECK(hipGetLastError());
       } else {
-        for (const auto& b : pl.fact)
-          launch_join_agg(d, b, pl.lo, pl.hi, pl.flag, ix.cu, pl.d_partial.as<AggPartial>(), s);
+        const uint64_t cap = agg_partial_cap(ix.cu);
+        for (size_t i = 0; i < pl.fact.size(); ++i) {
+          const size_t lane = lanes ? i % lanes->size() : 0;
+          launch_join_agg(d, pl.fact[i], pl.lo, pl.hi, pl.flag, ix.cu, pl.d_partial.as<AggPartial>() + lane * cap,
+                          lanes ? (*lanes)[lane] : s);
+        }
       }
     });
     return;
@@ -495,11 +503,60 @@ void enqueue_plan(gj_batch_plan& pl, hipStream_t s) {
       } else {
         gj_lookup_params lp;
         gj_lookup_params_default(&lp);
-        for (const auto& b : pl.probe) launch_plain(d, static_cast<const K*>(b.probes), b.m, b.out, lp, s, ix.cu);
+        for (size_t i = 0; i < pl.probe.size(); ++i) {
+          const ProbeBatchDev& b = pl.probe[i];
+          const hipStream_t ls = lanes ? (*lanes)[i % lanes->size()] : s;
+          launch_plain(d, static_cast<const K*>(b.probes), b.m, b.out, lp, ls, ix.cu);
+        }
       }
     });
   });
 }
+
+// Side streams and events of a forked graph capture: lanes[0] is the capturing
+// stream, lanes[1..] join the capture through the fork event.
+struct GraphLanes {
+  std::vector<hipStream_t> lanes;
+  std::vector<hipStream_t> own;
+  std::vector<hipEvent_t> events;  // [0] fork, [1..] one join per side stream
+  hipError_t create(hipStream_t cs) {
+    lanes.push_back(cs);
+    for (int i = 0; i < kGraphLanes; ++i) {
+      hipEvent_t ev = nullptr;
+      hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+      if (e != hipSuccess) return e;
+      events.push_back(ev);
+      if (i == 0) continue;
+      hipStream_t st = nullptr;
+      e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+      if (e != hipSuccess) return e;
+      own.push_back(st);
+      lanes.push_back(st);
+    }
+    return hipSuccess;
+  }
+  hipError_t fork(hipStream_t cs) {
+    hipError_t e = hipEventRecord(events[0], cs);
+    for (size_t i = 1; i < lanes.size() && e == hipSuccess; ++i) e = hipStreamWaitEvent(lanes[i], events[0], 0);
+    return e;
+  }
+  hipError_t join(hipStream_t cs) {
+    hipError_t e = hipSuccess;
+    for (size_t i = 1; i < lanes.size() && e == hipSuccess; ++i) {
+      e = hipEventRecord(events[i], lanes[i]);
+      if (e == hipSuccess) e = hipStreamWaitEvent(cs, events[i], 0);
+    }
+    return e;
+  }
+  void destroy() {
+    for (hipStream_t st : own) (void)hipStreamDestroy(st);
+    for (hipEvent_t ev : events) (void)hipEventDestroy(ev);
+    own.clear();
+    events.clear();
+    lanes.clear();
+  }
+  ~GraphLanes() { destroy(); }
+};
 
 void finish_plan(gj_batch_plan& pl, hipStream_t s) {
   if (pl.mode == GJ_BATCH_PERSISTENT) {
@@ -530,20 +587,29 @@ void finish_plan(gj_batch_plan& pl, hipStream_t s) {
       GJ_HIP_CHECK(hipStreamCreateWithFlags(&own, hipStreamNonBlocking));
       cs = own;
     }
-    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    // The batches fork over kGraphLanes captured streams (joined again before
+    // the graph ends), so small batches' launches overlap on replay.
+    GraphLanes gl;
+    hipError_t e = hipSuccess;
+    if ((pl.aggregate ? pl.fact.size() : pl.probe.size()) > 1) e = gl.create(cs);
+    if (e == hipSuccess) e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
       try {
-        enqueue_plan(pl, cs);
+        if (!gl.lanes.empty()) GJ_HIP_CHECK(gl.fork(cs));
+        enqueue_plan(pl, cs, gl.lanes.empty() ? nullptr : &gl.lanes);
+        if (!gl.lanes.empty()) GJ_HIP_CHECK(gl.join(cs));
       } catch (...) {
         hipGraph_t g = nullptr;
         (void)hipStreamEndCapture(cs, &g);
         if (g) (void)hipGraphDestroy(g);
+        gl.destroy();
         if (own) (void)hipStreamDestroy(own);
         throw;
       }
       e = hipStreamEndCapture(cs, &pl.graph);
     }
     if (e == hipSuccess) e = hipGraphInstantiate(&pl.exec, pl.graph, nullptr, nullptr, 0);
+    gl.destroy();
     if (own) (void)hipStreamDestroy(own);
     GJ_HIP_CHECK(e);
   }
@@ -630,7 +696,9 @@ int gj_aggregate_plan_create(const gj_index* index, const gj_fact_batch* h_batch
     }
     pl->total_tiles = tiles;
     // per-workgroup sums of the per-batch launches, or per-tile sums of the persistent kernel
-    pl->d_partial.alloc(std::max<uint64_t>(agg_partial_cap(index->cu), tiles * kAggWaves) * sizeof(AggPartial));
+    // (one slice per graph lane for the per-batch launches)
+    pl->d_partial.alloc(std::max<uint64_t>(agg_partial_cap(index->cu) * kGraphLanes, tiles * kAggWaves) *
+                        sizeof(AggPartial));
     finish_plan(*pl, S(stream));
     *out = pl.release();
   });
