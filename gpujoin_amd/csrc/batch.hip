@@ -445,76 +445,10 @@ struct gj_batch_plan {
 namespace gj {
 namespace {
 
-// Enqueues one execution of the plan (no graph replay) on s.  With `lanes`
-// (graph capture), batch i goes to lanes[i % size] instead, so the captured
-// launches may overlap: lookup batches share no scratch, and the aggregate
-// launches of lane j use their own slice j of the partial buffer.
-void enqueue_plan(gj_batch_plan& pl, hipStream_t s, const std::vector<hipStream_t>* lanes = nullptr) {
-  const gj_index& ix = *pl.ix;
-  if (pl.aggregate) {
-    fill_bytes(pl.results, 0, pl.fact.size() * sizeof(gj_agg_result), s);
-    with_index_dev<uint32_t>(ix, [&](auto d) {
-      using Dev = decltype(d);
-      if (pl.mode == GJ_BATCH_PERSISTENT) {
-        fill_bytes(pl.d_counter.p, 0, 16, s);
-        if (pl.total_tiles == 0) return;
-        const size_t lds = (d.lds_bytes() + 15) / 16 * 16 + agg_queue_bytes(kAggUnrollTile);
-        const void* fn = reinterpret_cast<const void*>(&join_agg_persistent_kernel<Dev>);
-        uint64_t blocks = static_cast<uint64_t>(blocks_per_cu(fn, kAggBlock, lds)) * ix.cu;
-        blocks = std::min<uint64_t>(blocks, pl.total_tiles);
-        blocks = std::max<uint64_t>(blocks, 1);
-        hipLaunchKernelGGL((join_agg_persistent_kernel<Dev>), dim3(static_cast<unsigned>(blocks)), dim3(kAggBlock),
-                           lds, s, d, pl.d_batches.as<const FactBatchDev>(), pl.d_tile_batch.as<const uint32_t>(),
-                           pl.total_tiles, pl.lo, pl.hi, pl.flag,
-                           pl.d_counter.as<unsigned long long>(), pl.d_partial.as<AggPartial>());
-        GJ_HIP_CHECK(hipGetLastError());
-        const uint32_t nbat = static_cast<uint32_t>(pl.fact.size());
-        hipLaunchKernelGGL(agg_batch_sum_kernel, dim3(static_cast<unsigned>(ceil_div(nbat, 256 / kWave))), dim3(256), 0,
-                           s, pl.d_batches.as<const FactBatchDev>(), pl.d_tile_start.as<const uint64_t>(), nbat,
-                           pl.total_tiles, pl.d_partial.as<const AggPartial>());
-        GJ_HIP_CHECK(hipGetLastError());
-      } else {
-        const uint64_t cap = agg_partial_cap(ix.cu);
-        for (size_t i = 0; i < pl.fact.size(); ++i) {
-          const size_t lane = lanes ? i % lanes->size() : 0;
-          launch_join_agg(d, pl.fact[i], pl.lo, pl.hi, pl.flag, ix.cu, pl.d_partial.as<AggPartial>() + lane * cap,
-                          lanes ? (*lanes)[lane] : s);
-        }
-      }
-    });
-    return;
-  }
-  dispatch_key(ix.key_bytes, [&]<class K>() {
-    with_index_dev<K>(ix, [&](auto d) {
-      using Dev = decltype(d);
-      if (pl.mode == GJ_BATCH_PERSISTENT) {
-        fill_bytes(pl.d_counter.p, 0, 16, s);
-        if (pl.total_tiles == 0) return;
-        const size_t lds = d.lds_bytes();
-        const void* fn = reinterpret_cast<const void*>(&lookup_persistent_kernel<Dev, K>);
-        uint64_t blocks = static_cast<uint64_t>(blocks_per_cu(fn, 256, lds)) * ix.cu;
-        blocks = std::min<uint64_t>(blocks, pl.total_tiles);
-        blocks = std::max<uint64_t>(blocks, 1);
-        hipLaunchKernelGGL((lookup_persistent_kernel<Dev, K>), dim3(static_cast<unsigned>(blocks)), dim3(256), lds, s,
-                           d, pl.d_batches.as<const ProbeBatchDev>(), pl.d_tile_batch.as<const uint32_t>(),
-                           pl.total_tiles,
-                           pl.d_counter.as<unsigned long long>());
-        GJ_HIP_CHECK(hipGetLastError());
-      } else {
-        gj_lookup_params lp;
-        gj_lookup_params_default(&lp);
-        for (size_t i = 0; i < pl.probe.size(); ++i) {
-          const ProbeBatchDev& b = pl.probe[i];
-          const hipStream_t ls = lanes ? (*lanes)[i % lanes->size()] : s;
-          launch_plain(d, static_cast<const K*>(b.probes), b.m, b.out, lp, ls, ix.cu);
-        }
-      }
-    });
-  });
-}
-
 // Side streams and events of a forked graph capture: lanes[0] is the capturing
-// stream, lanes[1..] join the capture through the fork event.
+// stream, lanes[1..] join the capture through the fork event, which is recorded
+// after everything the batches depend on (the results' fill) and joined back
+// into lanes[0] after the last launch.
 struct GraphLanes {
   std::vector<hipStream_t> lanes;
   std::vector<hipStream_t> own;
@@ -558,6 +492,80 @@ struct GraphLanes {
   ~GraphLanes() { destroy(); }
 };
 
+// Enqueues one execution of the plan (no graph replay) on s.  With `gl` (graph
+// capture), batch i goes to lane i % lanes instead, so the captured launches may
+// overlap: lookup batches share no scratch, and the aggregate launches of lane j
+// use their own slice j of the partial buffer.
+void enqueue_plan(gj_batch_plan& pl, hipStream_t s, GraphLanes* gl = nullptr) {
+  const std::vector<hipStream_t>* lanes = gl ? &gl->lanes : nullptr;
+  const gj_index& ix = *pl.ix;
+  if (pl.aggregate) {
+    fill_bytes(pl.results, 0, pl.fact.size() * sizeof(gj_agg_result), s);
+    with_index_dev<uint32_t>(ix, [&](auto d) {
+      using Dev = decltype(d);
+      if (pl.mode == GJ_BATCH_PERSISTENT) {
+        fill_bytes(pl.d_counter.p, 0, 16, s);
+        if (pl.total_tiles == 0) return;
+        const size_t lds = (d.lds_bytes() + 15) / 16 * 16 + agg_queue_bytes(kAggUnrollTile);
+        const void* fn = reinterpret_cast<const void*>(&join_agg_persistent_kernel<Dev>);
+        uint64_t blocks = static_cast<uint64_t>(blocks_per_cu(fn, kAggBlock, lds)) * ix.cu;
+        blocks = std::min<uint64_t>(blocks, pl.total_tiles);
+        blocks = std::max<uint64_t>(blocks, 1);
+        hipLaunchKernelGGL((join_agg_persistent_kernel<Dev>), dim3(static_cast<unsigned>(blocks)), dim3(kAggBlock),
+                           lds, s, d, pl.d_batches.as<const FactBatchDev>(), pl.d_tile_batch.as<const uint32_t>(),
+                           pl.total_tiles, pl.lo, pl.hi, pl.flag,
+                           pl.d_counter.as<unsigned long long>(), pl.d_partial.as<AggPartial>());
+        GJ_HIP_CHECK(hipGetLastError());
+        const uint32_t nbat = static_cast<uint32_t>(pl.fact.size());
+        hipLaunchKernelGGL(agg_batch_sum_kernel, dim3(static_cast<unsigned>(ceil_div(nbat, 256 / kWave))), dim3(256), 0,
+                           s, pl.d_batches.as<const FactBatchDev>(), pl.d_tile_start.as<const uint64_t>(), nbat,
+                           pl.total_tiles, pl.d_partial.as<const AggPartial>());
+        GJ_HIP_CHECK(hipGetLastError());
+      } else {
+        const uint64_t cap = agg_partial_cap(ix.cu);
+        if (gl) GJ_HIP_CHECK(gl->fork(s));  // after the results' fill
+        for (size_t i = 0; i < pl.fact.size(); ++i) {
+          const size_t lane = lanes ? i % lanes->size() : 0;
+          launch_join_agg(d, pl.fact[i], pl.lo, pl.hi, pl.flag, ix.cu, pl.d_partial.as<AggPartial>() + lane * cap,
+                          lanes ? (*lanes)[lane] : s);
+        }
+        if (gl) GJ_HIP_CHECK(gl->join(s));
+      }
+    });
+    return;
+  }
+  dispatch_key(ix.key_bytes, [&]<class K>() {
+    with_index_dev<K>(ix, [&](auto d) {
+      using Dev = decltype(d);
+      if (pl.mode == GJ_BATCH_PERSISTENT) {
+        fill_bytes(pl.d_counter.p, 0, 16, s);
+        if (pl.total_tiles == 0) return;
+        const size_t lds = d.lds_bytes();
+        const void* fn = reinterpret_cast<const void*>(&lookup_persistent_kernel<Dev, K>);
+        uint64_t blocks = static_cast<uint64_t>(blocks_per_cu(fn, 256, lds)) * ix.cu;
+        blocks = std::min<uint64_t>(blocks, pl.total_tiles);
+        blocks = std::max<uint64_t>(blocks, 1);
+        hipLaunchKernelGGL((lookup_persistent_kernel<Dev, K>), dim3(static_cast<unsigned>(blocks)), dim3(256), lds, s,
+                           d, pl.d_batches.as<const ProbeBatchDev>(), pl.d_tile_batch.as<const uint32_t>(),
+                           pl.total_tiles,
+                           pl.d_counter.as<unsigned long long>());
+        GJ_HIP_CHECK(hipGetLastError());
+      } else {
+        gj_lookup_params lp;
+        gj_lookup_params_default(&lp);
+        if (gl) GJ_HIP_CHECK(gl->fork(s));
+        for (size_t i = 0; i < pl.probe.size(); ++i) {
+          const ProbeBatchDev& b = pl.probe[i];
+          const hipStream_t ls = lanes ? (*lanes)[i % lanes->size()] : s;
+          launch_plain(d, static_cast<const K*>(b.probes), b.m, b.out, lp, ls, ix.cu);
+        }
+        if (gl) GJ_HIP_CHECK(gl->join(s));
+      }
+    });
+  });
+}
+
+
 void finish_plan(gj_batch_plan& pl, hipStream_t s) {
   if (pl.mode == GJ_BATCH_PERSISTENT) {
     const size_t nb = pl.aggregate ? pl.fact.size() : pl.probe.size();
@@ -595,9 +603,7 @@ void finish_plan(gj_batch_plan& pl, hipStream_t s) {
     if (e == hipSuccess) e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
       try {
-        if (!gl.lanes.empty()) GJ_HIP_CHECK(gl.fork(cs));
-        enqueue_plan(pl, cs, gl.lanes.empty() ? nullptr : &gl.lanes);
-        if (!gl.lanes.empty()) GJ_HIP_CHECK(gl.join(cs));
+        enqueue_plan(pl, cs, gl.lanes.empty() ? nullptr : &gl);
       } catch (...) {
         hipGraph_t g = nullptr;
         (void)hipStreamEndCapture(cs, &g);
