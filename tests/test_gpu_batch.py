@@ -123,6 +123,8 @@ def test_aggregate_plan_batches(gj, orc, gpu, mode, kind):
         off += sz
     want = np.array(want, dtype=np.int64)
     for rep in range(3):
+        res.fill(0x5A)  # the plan zeroes its results before any batch adds to them
+        gj.synchronize()
         plan.run()
         gj.synchronize()
         got = res.numpy().reshape(-1, 4)[:, :3]
