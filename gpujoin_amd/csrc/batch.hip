@@ -417,38 +417,13 @@ FactBatchDev to_dev(const gj_fact_batch& b, gj_agg_result* res) {
 
 using namespace gj;
 
-struct gj_batch_plan {
-  const gj_index* ix = nullptr;
-  bool aggregate = false;
-  gj_batch_mode mode = GJ_BATCH_LAUNCH;
-  // host copies
-  std::vector<FactBatchDev> fact;
-  std::vector<ProbeBatchDev> probe;
-  std::vector<uint64_t> tile_start;
-  uint64_t total_tiles = 0;
-  int32_t lo = 0, hi = 0;
-  const uint8_t* flag = nullptr;
-  gj_agg_result* results = nullptr;
-  // device copies (persistent)
-  dbuf d_batches, d_tile_start, d_counter;
-  dbuf d_tile_batch;  // batch of each persistent work item
-  dbuf d_partial;  // workgroup partial sums of the per-batch join-aggregate launches
-  // graph
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  ~gj_batch_plan() {
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
-  }
-};
-
 namespace gj {
-namespace {
 
-// Side streams and events of a forked graph capture: lanes[0] is the capturing
-// stream, lanes[1..] join the capture through the fork event, which is recorded
-// after everything the batches depend on (the results' fill) and joined back
-// into lanes[0] after the last launch.
+// Side streams and events over which a plan spreads its per-batch launches
+// (graph capture, or the launch mode at run time): lanes[0] is the caller's (or
+// capturing) stream, lanes[1..] wait on the fork event, which is recorded after
+// everything the batches depend on (the results' fill), and lanes[0] waits on
+// each side stream's join event after the last launch.
 struct GraphLanes {
   std::vector<hipStream_t> lanes;
   std::vector<hipStream_t> own;
@@ -491,6 +466,39 @@ struct GraphLanes {
   }
   ~GraphLanes() { destroy(); }
 };
+
+}  // namespace gj
+
+struct gj_batch_plan {
+  const gj_index* ix = nullptr;
+  bool aggregate = false;
+  gj_batch_mode mode = GJ_BATCH_LAUNCH;
+  // host copies
+  std::vector<FactBatchDev> fact;
+  std::vector<ProbeBatchDev> probe;
+  std::vector<uint64_t> tile_start;
+  uint64_t total_tiles = 0;
+  int32_t lo = 0, hi = 0;
+  const uint8_t* flag = nullptr;
+  gj_agg_result* results = nullptr;
+  // device copies (persistent)
+  dbuf d_batches, d_tile_start, d_counter;
+  dbuf d_tile_batch;  // batch of each persistent work item
+  dbuf d_partial;  // workgroup partial sums of the per-batch join-aggregate launches
+  // side streams of the launch mode
+  gj::GraphLanes lanes;
+  // graph
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~gj_batch_plan() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
+namespace gj {
+namespace {
+
 
 // Enqueues one execution of the plan (no graph replay) on s.  With `gl` (graph
 // capture), batch i goes to lane i % lanes instead, so the captured launches may
@@ -618,6 +626,9 @@ void finish_plan(gj_batch_plan& pl, hipStream_t s) {
     gl.destroy();
     if (own) (void)hipStreamDestroy(own);
     GJ_HIP_CHECK(e);
+  } else if ((pl.aggregate ? pl.fact.size() : pl.probe.size()) > 1) {
+    // launch mode: the side streams live with the plan; lanes[0] is set per run
+    GJ_HIP_CHECK(pl.lanes.create(nullptr));
   }
 }
 
@@ -715,6 +726,9 @@ int gj_batch_plan_run(gj_batch_plan* plan, gj_stream_t stream) {
     GJ_REQUIRE(plan, GJ_ERR_INVALID, "null plan");
     if (plan->mode == GJ_BATCH_GRAPH) {
       GJ_HIP_CHECK(hipGraphLaunch(plan->exec, S(stream)));
+    } else if (plan->mode == GJ_BATCH_LAUNCH && !plan->lanes.lanes.empty()) {
+      plan->lanes.lanes[0] = S(stream);
+      enqueue_plan(*plan, S(stream), &plan->lanes);
     } else {
       enqueue_plan(*plan, S(stream));
     }
