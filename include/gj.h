@@ -246,11 +246,15 @@ int64_t gj_promo_revenue(int64_t numerator, int64_t denominator);
  * No reference counterpart: the reference launches one kernel per run.
  * A plan holds a fixed list of batches; gj_batch_plan_run executes all of
  * them, by one of three modes:
- *   GJ_BATCH_LAUNCH      one kernel launch per batch (baseline)
- *   GJ_BATCH_GRAPH       the per-batch launches captured once into a hipGraph, replayed per run
- *   GJ_BATCH_PERSISTENT  one persistent kernel; workgroups pull (batch, tile) work items
+ *   GJ_BATCH_LAUNCH      one kernel launch per batch, spread over the caller's stream and
+ *                        3 side streams the plan owns (forked from and joined back into
+ *                        the caller's stream on every run)
+ *   GJ_BATCH_GRAPH       the same launches captured once into a hipGraph, replayed per run
+ *   GJ_BATCH_PERSISTENT  one persistent kernel; workgroups pull (batch, rows) work items
  *                        from a device counter until every batch is done
- * Lookup plans write rowids per batch; aggregate plans add one gj_agg_result per batch. */
+ * A run is ordered on the caller's stream as a whole; batches may overlap within it.
+ * Lookup plans write rowids per batch; aggregate plans set one gj_agg_result per batch
+ * (zeroed at the start of each run). */
 typedef enum gj_batch_mode { GJ_BATCH_LAUNCH = 0, GJ_BATCH_GRAPH = 1, GJ_BATCH_PERSISTENT = 2 } gj_batch_mode;
 
 typedef struct gj_probe_batch {
