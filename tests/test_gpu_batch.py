@@ -207,3 +207,58 @@ def test_index_autotuner_picks_a_candidate(gj, orc, gpu, kind, kb, field, allowe
     assert np.array_equal(out.numpy(), exp)
     assert np.array_equal(orc.lookup_lower_bound(keys, probes), exp)
     idx.close()
+
+
+@pytest.mark.parametrize("mode", [_lib.BATCH_LAUNCH, _lib.BATCH_GRAPH])
+def test_plans_on_a_created_stream(gj, orc, gpu, mode):
+    # launch and graph plans fork their batches over side streams from the caller's
+    # stream and join them back: a run on a created stream, followed by host reads after
+    # that stream's synchronisation, sees every batch's result
+    print(f"[case] plans on a created stream mode={mode}", flush=True)
+    stream = gj.Stream()
+    keys = gj.gen_build_keys(32 * 4096, 8, _lib.KEYS_UNIFORM, 63, 0x71)
+    col = gj.DeviceArray.from_numpy(keys, stream)
+    idx = gj.Index("learned_plr", col, host_column=keys, stream=stream)
+    sizes = _log_uniform_sizes(np.random.default_rng(0x72), 1_000_000)
+    m = sum(sizes)
+    probes, _ = gj.gen_probes(keys, m, _lib.PROBES_UNIFORM, 1.0, 0.1, 63, 0x73)
+    dp = gj.DeviceArray.from_numpy(probes, stream)
+    out = gj.DeviceArray(m, np.uint32)
+    batches, off = [], 0
+    for sz in sizes:
+        batches.append((dp.offset(off), out.offset(off), sz))
+        off += sz
+    plan = gj.LookupPlan(idx, batches, mode, stream=stream)
+    want = orc.lookup_lower_bound(keys, probes)
+    for rep in range(2):
+        out.fill(0x33, stream)
+        plan.run(stream)
+        stream.synchronize()
+        assert np.array_equal(out.numpy(), want), f"lookup run {rep}"
+    plan.close()
+
+    dim_rows = 32 * 1000
+    skey, sflag = _dimension(gj, dim_rows, seed=0x74)
+    fsizes = _log_uniform_sizes(np.random.default_rng(0x75), 1_500_000)
+    fkey, date, price, disc = _fact(gj, sum(fsizes), dim_rows, seed=0x76)
+    acol, aidx = _agg_index(gj, "btree", skey)
+    d = [gj.DeviceArray.from_numpy(a, stream) for a in (fkey, date, price, disc)]
+    dflag = gj.DeviceArray.from_numpy(sflag, stream)
+    fb, off = [], 0
+    for sz in fsizes:
+        fb.append((d[0].offset(off), d[1].offset(off), d[2].offset(off), d[3].offset(off), sz))
+        off += sz
+    res = gj.DeviceArray(4 * len(fb), np.int64)
+    aplan = gj.AggregatePlan(aidx, fb, dflag, 0, 400, res, mode, stream=stream)
+    want, off = [], 0
+    for sz in fsizes:
+        s = slice(off, off + sz)
+        want.append(orc.join_aggregate(skey, sflag, fkey[s], date[s], price[s], disc[s], 0, 400))
+        off += sz
+    want = np.array(want, dtype=np.int64)
+    for rep in range(2):
+        res.fill(0x5A, stream)
+        aplan.run(stream)
+        stream.synchronize()
+        assert np.array_equal(res.numpy().reshape(-1, 4)[:, :3], want), f"aggregate run {rep}"
+    aplan.close()
