@@ -425,6 +425,9 @@ namespace gj {
 // everything the batches depend on (the results' fill), and lanes[0] waits on
 // each side stream's join event after the last launch.
 struct GraphLanes {
+  GraphLanes() = default;
+  GraphLanes(const GraphLanes&) = delete;  // owns the side streams and events
+  GraphLanes& operator=(const GraphLanes&) = delete;
   std::vector<hipStream_t> lanes;
   std::vector<hipStream_t> own;
   std::vector<hipEvent_t> events;  // [0] fork, [1..] one join per side stream
