@@ -45,7 +45,7 @@ typedef struct ihipStream_t* gj_stream_t;
 typedef enum gj_status {
   GJ_OK = 0,
   GJ_ERR_INVALID = -1,     /* bad argument (null pointer, bad size, unsorted column, ...) */
-  GJ_ERR_UNSUPPORTED = -2, /* valid request this build does not implement */
+  GJ_ERR_UNSUPPORTED = -2, /* valid request the library does not implement */
   GJ_ERR_HIP = -3,         /* a HIP runtime call failed */
   GJ_ERR_NOMEM = -4,       /* device or host allocation failed */
   GJ_ERR_OVERFLOW = -5,    /* row ids do not fit in uint32 ("'value_t' does not suffice", src/indexes.cuh:219-221, 367-369) */
