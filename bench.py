@@ -104,7 +104,9 @@ def main():
     ap.add_argument("--log2-m", type=int, default=28)
     ap.add_argument("--indexes", default="binary_search,btree,harmonia,radix_spline,learned_plr,learned_mlp,hash")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-sample-log2", type=int, default=24)
+    ap.add_argument("--cpu-sample-log2", type=int, default=23)
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the other configurations' legs (u32, Zipf, learned with misses, join-aggregate)")
     ap.add_argument("--profile-index", default="", help="run only this index (for rocprofv3 runs)")
     ap.add_argument("--approach", default="auto", help="auto (plain + autotuned), plain, bws, partitioning or hj")
     ap.add_argument("--tune-log2", type=int, default=24, help="probe sample size of the autotuner")
@@ -127,7 +129,7 @@ def main():
     log("device", info)
     if args.join_aggregate:
         # the suite's join-aggregate measurements alone (one JSON object per line)
-        from gpujoin_amd import suite
+        import bench_suite as suite
         suite.run_join_aggregate(gj, quick=args.quick, cpu=not args.no_cpu)
         return
     n = 1 << args.log2_n
@@ -173,9 +175,13 @@ def main():
     nxt = np.empty(nxt_n, dtype=np.uint32)
     nxt[perm] = np.roll(perm, -1)  # one cycle through all slots
     d_next = gj.DeviceArray.from_numpy(nxt, stream)
-    chase_out = gj.DeviceArray(64, np.uint32)
-    steps_chase = 20000
-    per, _ = time_steps(lambda: gj.pointer_chase(d_next, 1, steps_chase, chase_out, stream), 2, 1, stream, gj)
+    chase_out = gj.DeviceArray(1, np.uint32)
+    steps_chase = 100000
+    gj.pointer_chase(d_next, 1, 1, chase_out, stream)
+    # every launch continues the chain where the previous one stopped, so no launch
+    # re-walks lines that L2 or the MALL may still hold (the cycle spans 2^27 slots)
+    per, _ = time_steps(lambda: gj.pointer_chase(d_next, 1, steps_chase, chase_out, stream, start=chase_out), 3, 1,
+                        stream, gj)
     chase_ns = statistics.median(per) * 1e6 / steps_chase
     del d_next, nxt, perm
     log(f"pointer chase: {chase_ns:.0f} ns per dependent access")
@@ -192,7 +198,7 @@ def main():
 
     if args.suite:
         # the other workload configurations (one JSON object per line; not the headline run)
-        from gpujoin_amd import suite
+        import bench_suite as suite
         del d_probes, d_out, d_keys
         suite.run(gj, G, quick=args.quick, cpu=not args.no_cpu)
         return
@@ -259,24 +265,38 @@ def main():
         idx.close()
 
     # ---- CPU baseline (oracle port, bounded sample) ---------------------------
+    # 3 warm-ups + 10 repetitions, median (the reference's measure(), src/measuring.hpp:12),
+    # for the equality-checked lower_bound and the reference's B-tree descent
     cpu = None
     if not args.no_cpu:
         import oracle
         ms_n = 1 << args.cpu_sample_log2
         sample = probes[:ms_n]
         cores = os.cpu_count() or 1
-        oracle.lookup_lower_bound(keys, sample[: 1 << 20], threads=cores)  # warm-up
-        times = []
-        for _ in range(3):
-            t = time.perf_counter()
-            out_cpu = oracle.lookup_lower_bound(keys, sample, threads=cores)
-            times.append(time.perf_counter() - t)
-        ok = bool(np.array_equal(out_cpu, expected[:ms_n]))
-        cpu = {"value": ms_n / statistics.median(times), "unit": "lookups/s", "cores": cores, "kind": "port",
+
+        def cpu_leg(fn):
+            for _ in range(3):
+                out_cpu = fn()
+            times = []
+            for _ in range(10):
+                t = time.perf_counter()
+                out_cpu = fn()
+                times.append(time.perf_counter() - t)
+            return ms_n / statistics.median(times), bool(np.array_equal(out_cpu, expected[:ms_n]))
+
+        lb_rate, lb_ok = cpu_leg(lambda: oracle.lookup_lower_bound(keys, sample, threads=cores))
+        tree = oracle.BTree(keys, 4)
+        bt_rate, bt_ok = cpu_leg(lambda: tree.lookup(sample, threads=cores))
+        del tree
+        cpu = {"value": lb_rate, "unit": "lookups/s", "cores": cores, "kind": "port",
                "sample": f"2^{args.cpu_sample_log2} of the 2^{args.log2_m} probes against the full 2^{args.log2_n}-key "
-                         f"column, equality-checked lower_bound, OpenMP static schedule, median of 3 after 1 warm-up; "
-                         f"row ids match: {ok}"}
-        log(f"cpu baseline: {cpu['value'] / 1e6:.1f} M lookups/s on {cores} threads")
+                         f"column, equality-checked lower_bound, OpenMP static schedule, median of 10 after 3 "
+                         f"warm-ups; row ids match: {lb_ok}",
+               "btree_descent": {"value": bt_rate, "unit": "lookups/s", "rowids_match": bt_ok,
+                                 "what": "the reference's clustered B+-tree (4 KiB pages) restated in the oracle, "
+                                         "same sample, threads and repetitions"}}
+        log(f"cpu baseline: lower_bound {lb_rate / 1e6:.1f} M lookups/s, B-tree {bt_rate / 1e6:.1f} M lookups/s "
+            f"on {cores} threads")
 
     head_order = ["learned_mlp", "learned_plr", "radix_spline", "btree", "harmonia", "binary_search"]
     head = next((h for h in head_order if h in results and results[h]["rowids_match"]), None)
@@ -319,6 +339,10 @@ def main():
         "pointer_chase_ns": chase_ns,
         "host_gather_per_s": G_host,
     }
+    if not args.no_extras:
+        import bench_suite as suite
+        del d_probes, probes, expected
+        line["configs"] = suite.extras(gj, G, stream, keys, d_keys, d_out, cpu=not args.no_cpu)
     print(json.dumps(line), flush=True)
 
 

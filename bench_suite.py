@@ -27,7 +27,7 @@ import time
 
 import numpy as np
 
-from . import _lib
+from gpujoin_amd import _lib
 
 
 def _time(gj, fn, stream, steps=5, warmup=2):
@@ -302,3 +302,211 @@ def run_join_aggregate(gj, quick=False, cpu=True, emit=None, stream=None):
         emit({"config": "join-aggregate CPU baseline (oracle restatement of the Q14 loop, OpenMP)", "cores": cores,
               "rows_per_s": fact_rows / cpu_s})
     log("join-aggregate done")
+
+
+# ---- the other configurations' legs of the default bench run ------------------------------
+
+def _timed(gj, fn, stream, steps, warmup):
+    """Average ms per call over `steps` calls after `warmup`, from events on `stream`."""
+    return _time(gj, fn, stream, steps=steps, warmup=warmup)
+
+
+def _cpu_median(fn, warmup=3, reps=10):
+    """The reference's measure() protocol (src/measuring.hpp:12): warm-ups, then the median of reps."""
+    out = None
+    for _ in range(warmup):
+        out = fn()
+    times = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        out = fn()
+        times.append(time.perf_counter() - t)
+    return statistics.median(times), out
+
+
+ALL_KINDS = ("binary_search", "btree", "harmonia", "radix_spline", "learned_plr", "learned_mlp", "hash")
+
+
+def extras(gj, G, stream, keys64, d_keys64, d_out, cpu=True, steps=5, warmup=2):
+    """C1, C2(b), C4 and C5 measured in the default run, every result checked."""
+    import oracle
+
+    log = lambda *a: print("[extras]", *a, file=sys.stderr, flush=True)
+    cores = os.cpu_count() or 1
+    res = {}
+
+    # C1: 2^20 unique u32 keys, 2^24 uniform probes (100 % hit): row ids + u64 checksum
+    keys, (probes, exp) = None, (None, None)
+    keys = gj.gen_build_keys(1 << 20, 4, _lib.KEYS_UNIFORM, 0, 0x1)
+    probes, exp = gj.gen_probes(keys, 1 << 24, _lib.PROBES_UNIFORM, 1.0, 0.0, 0, 0x2)
+    m = probes.size
+    dk, dp = gj.DeviceArray.from_numpy(keys, stream), gj.DeviceArray.from_numpy(probes, stream)
+    want_sum = int(exp.astype(np.uint64).sum())
+    c1 = {"workload": "2^20 unique u32 keys from [0, 2^32), 2^24 uniform u32 probes (100% hit), u32 row ids",
+          "checksum_expected": want_sum, "indexes": {}}
+    for kind in ALL_KINDS:
+        idx = gj.Index(kind, dk, host_column=keys, stream=stream)
+        ms = _timed(gj, lambda: idx.lookup(dp, d_out, m=m, params=gj.lookup_params(), stream=stream), stream, steps,
+                    warmup)
+        got = d_out.numpy(m)
+        c1["indexes"][kind] = {"lookups_per_s": m / ms * 1e3, "gather_roofline_frac": m / ms * 1e3 / G,
+                               "rowids_match": bool(np.array_equal(got, exp)),
+                               "checksum": int(got.astype(np.uint64).sum())}
+        idx.close()
+        log(f"C1 {kind}: {m / ms * 1e-6:.0f} M lookups/s match={c1['indexes'][kind]['rowids_match']}")
+    if cpu:
+        sec, ref = _cpu_median(lambda: oracle.lookup_lower_bound(keys, probes, threads=cores))
+        c1["cpu_baseline"] = {"lookups_per_s": m / sec, "cores": cores, "kind": "port",
+                              "what": "equality-checked lower_bound over all 2^24 probes, median of 10 after 3 "
+                                      "warm-ups", "checksum": oracle.checksum(ref)}
+    res["C1_u32"] = c1
+    del dk, dp, keys, probes, exp
+
+    # C2(b): the headline's 2^26 keys, 2^28 Zipf(1.0) probes over key rank
+    m = d_out.count
+    probes, exp = gj.gen_probes(keys64, m, _lib.PROBES_ZIPF, 1.0, 0.0, 63, 0x13)
+    dp = gj.DeviceArray.from_numpy(probes, stream)
+    del probes
+    c2 = {"workload": "2^26 unique u64 keys, 2^28 probes Zipf theta=1.0 over key rank (rank 0 never drawn, as in "
+                      "the reference), 100% hit", "indexes": {}}
+    for kind in ALL_KINDS:
+        idx = gj.Index(kind, d_keys64, host_column=keys64, stream=stream)
+        ms = _timed(gj, lambda: idx.lookup(dp, d_out, m=m, params=gj.lookup_params(), stream=stream), stream, 3, 1)
+        ok = bool(np.array_equal(d_out.numpy(m), exp))
+        c2["indexes"][kind] = {"lookups_per_s": m / ms * 1e3, "gather_roofline_frac": m / ms * 1e3 / G,
+                               "rowids_match": ok}
+        idx.close()
+        log(f"C2b {kind}: {m / ms * 1e-9:.2f} G lookups/s match={ok}")
+    res["C2b_zipf"] = c2
+    del dp, exp
+
+    # C4: learned indexes over three key distributions, 10 % misses, B-tree fallback counted
+    c4 = {"workload": "2^26 unique u64 keys (uniform, lognormal(0, 2), 64 Gaussian clusters), 2^28 probes with "
+                      "10% misses", "distributions": {}}
+    variants = [("btree", {}), ("radix_spline", {"rs_max_error": 8}), ("radix_spline", {"rs_max_error": 32}),
+                ("radix_spline", {"rs_max_error": 64}), ("learned_plr", {}), ("learned_mlp", {})]
+    for dist, dname in ((_lib.KEYS_UNIFORM, "uniform"), (_lib.KEYS_LOGNORMAL, "lognormal"),
+                        (_lib.KEYS_CLUSTERED, "clustered")):
+        k4 = gj.gen_build_keys(1 << 26, 8, dist, 63, 0x31 + dist)
+        p4, e4 = gj.gen_probes(k4, m, _lib.PROBES_UNIFORM, 1.0, 0.1, 63, 0x32 + dist)
+        dk4 = gj.DeviceArray.from_numpy(k4, stream)
+        dp4 = gj.DeviceArray.from_numpy(p4, stream)
+        del p4
+        rows = {}
+        for kind, prm in variants:
+            label = kind + (f"_err{prm['rs_max_error']}" if prm else "")
+            t = time.time()
+            idx = gj.Index(kind, dk4, host_column=k4, params=gj.index_params(**prm), stream=stream)
+            build_s = time.time() - t
+            st = idx.stats()
+            ms = _timed(gj, lambda: idx.lookup(dp4, d_out, m=m, params=gj.lookup_params(), stream=stream), stream, 3,
+                        1)
+            ok = bool(np.array_equal(d_out.numpy(m), e4))
+            rows[label] = {"lookups_per_s": m / ms * 1e3, "gather_roofline_frac": m / ms * 1e3 / G,
+                           "rowids_match": ok, "build_s": build_s, "fallback_keys": st["fallback_keys"],
+                           "fallback_leaves": st["fallback_leaves"], "avg_window": st["avg_window"],
+                           "max_window": st["max_window"], "train_loss": st["train_loss"],
+                           "index_bytes": st["memory_bytes"]}
+            idx.close()
+            log(f"C4 {dname} {label}: {m / ms * 1e-9:.2f} G lookups/s match={ok} fallback={st['fallback_keys']}")
+        bt = rows["btree"]["lookups_per_s"]
+        for label in ("learned_plr", "learned_mlp"):
+            rows[label]["vs_btree"] = rows[label]["lookups_per_s"] / bt
+        c4["distributions"][dname] = rows
+        del dk4, dp4, k4, e4
+    res["C4_learned_misses"] = c4
+
+    res["C5_join_aggregate"] = join_aggregate_leg(gj, stream, cpu=cpu)
+    return res
+
+
+def join_aggregate_leg(gj, stream, cpu=True, fact_rows=60_000_000, reps=6):
+    """C5: 2e6-row dimension indexed by the learned model, 6e7-row fact table, 30-day
+    window; one batch and log-uniform batches through launches / hipGraph / persistent
+    kernel.  Repetitions alternate between two independently drawn copies of the fact
+    table (2 x 1.02 GB), so no repetition finds its rows in the 256 MB MALL."""
+    import oracle
+
+    log = lambda *a: print("[extras]", *a, file=sys.stderr, flush=True)
+    cores = os.cpu_count() or 1
+    dim_rows = 2_000_000
+    key, flag = gj.gen_dimension(dim_rows, 0x41)
+    skey, sflag = gj.sort_dimension(key, flag)
+    lo, hi = 1200, 1230
+    copies = []
+    for seed in (0x42, 0x44):
+        cols = gj.gen_fact(fact_rows, dim_rows, 2557, 90000, 10500000, 10, seed)
+        want = oracle.join_aggregate(skey, sflag, *cols, lo, hi, threads=cores)
+        n_qual = int(np.count_nonzero((cols[1] >= lo) & (cols[1] < hi)))
+        copies.append({"host": cols, "dev": [gj.DeviceArray.from_numpy(a, stream) for a in cols], "want": want,
+                       "n_qual": n_qual})
+    dflag = gj.DeviceArray.from_numpy(sflag, stream)
+    dkey = gj.DeviceArray.from_numpy(skey, stream)
+    rng = np.random.default_rng(0x43)
+    sizes, total = [], 0
+    while total < fact_rows:
+        s = min(int(np.exp(rng.uniform(np.log(1e3), np.log(1e7)))), fact_rows - total)
+        sizes.append(s)
+        total += s
+    out = {"workload": f"dimension {dim_rows} rows (dense shuffled int32 keys, promo p=1/3), fact {fact_rows} rows, "
+                       f"30-day window (~1.2% selectivity), exact int64 sums; repetitions alternate two fact copies",
+           "batches": len(sizes), "one_batch": {}, "plans": {}}
+    res = gj.DeviceArray(4, np.int64)
+    for kind in ("learned_plr", "btree", "hash"):
+        idx = gj.Index(kind, dkey, host_column=skey, stream=stream)
+        ok = True
+        for c in copies:  # warm-up and check on both copies
+            res.fill(0, stream)
+            gj.join_aggregate(idx, *c["dev"], dflag, lo, hi, res, stream=stream)
+            ok &= tuple(int(v) for v in res.numpy()[:3]) == c["want"]
+        e = [gj.Event() for _ in range(reps + 1)]
+        stream.synchronize()
+        e[0].record(stream)
+        for r in range(reps):
+            res.fill(0, stream)
+            gj.join_aggregate(idx, *copies[r % 2]["dev"], dflag, lo, hi, res, stream=stream)
+            e[r + 1].record(stream)
+        stream.synchronize()
+        ms = e[0].elapsed_ms(e[reps]) / reps
+        alg = fact_rows * 4 + copies[0]["n_qual"] * 13 + copies[0]["want"][2]
+        out["one_batch"][kind] = {"rows_per_s": fact_rows / ms * 1e3, "ms": ms, "sums_match": ok,
+                                  "algorithmic_GBps": alg / (ms * 1e-3) / 1e9}
+        log(f"C5 one batch {kind}: {fact_rows / ms * 1e-9:.0f} G rows/s match={ok}")
+        if kind != "learned_plr":
+            idx.close()
+            continue
+        results = gj.DeviceArray(4 * len(sizes), np.int64)
+        for mode, mname in ((_lib.BATCH_LAUNCH, "launch"), (_lib.BATCH_GRAPH, "hipgraph"),
+                            (_lib.BATCH_PERSISTENT, "persistent")):
+            plans = []
+            for c in copies:
+                batches, off = [], 0
+                for s in sizes:
+                    batches.append(tuple(col.offset(off) for col in c["dev"]) + (s,))
+                    off += s
+                plans.append(gj.AggregatePlan(idx, batches, dflag, lo, hi, results, mode, stream=stream))
+            ok = True
+            for p, c in zip(plans, copies):
+                p.run(stream)
+                sums = results.numpy().reshape(-1, 4)[:, :3].sum(axis=0)
+                ok &= tuple(int(v) for v in sums) == c["want"]
+            stream.synchronize()
+            e[0].record(stream)
+            for r in range(reps):
+                plans[r % 2].run(stream)
+                e[r + 1].record(stream)
+            stream.synchronize()
+            ms = e[0].elapsed_ms(e[reps]) / reps
+            out["plans"][mname] = {"rows_per_s": fact_rows / ms * 1e3, "ms": ms, "sums_match": ok}
+            log(f"C5 plan {mname}: {fact_rows / ms * 1e-9:.0f} G rows/s match={ok}")
+            for p in plans:
+                p.close()
+        idx.close()
+    if cpu:
+        c = copies[0]["host"]
+        sec, got = _cpu_median(lambda: oracle.join_aggregate(skey, sflag, *c, lo, hi, threads=cores))
+        out["cpu_baseline"] = {"rows_per_s": fact_rows / sec, "cores": cores, "kind": "port",
+                               "what": "the oracle's restated Q14 loop (filter, lower_bound, int64 sums), OpenMP, "
+                                       "median of 10 after 3 warm-ups",
+                               "sums_match": tuple(got) == copies[0]["want"]}
+    return out

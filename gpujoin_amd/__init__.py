@@ -319,8 +319,12 @@ def gather(array: DeviceArray, probes: DeviceArray, out: DeviceArray, m: Optiona
     check(_lib.call("gj_gather", array.ptr, array.count, probes.ptr, mm, out.ptr, block, items, _s(stream)))
 
 
-def pointer_chase(nxt: DeviceArray, chains: int, steps: int, out: DeviceArray, stream: Optional[Stream] = None):
-    check(_lib.call("gj_pointer_chase", nxt.ptr, nxt.count, chains, steps, out.ptr, _s(stream)))
+def pointer_chase(nxt: DeviceArray, chains: int, steps: int, out: DeviceArray, stream: Optional[Stream] = None,
+                  start: Optional[DeviceArray] = None):
+    """cur = nxt[cur], `steps` times per chain; chain c starts at start[c] (start may be `out`
+    itself, to continue each chain where the last call stopped) or at a spread default."""
+    sp = start.ptr if start is not None else C.c_void_p()
+    check(_lib.call("gj_pointer_chase", nxt.ptr, nxt.count, chains, steps, sp, out.ptr, _s(stream)))
 
 
 def stream_copy(src: DeviceArray, dst: DeviceArray, nbytes: int, stream: Optional[Stream] = None):
@@ -330,8 +334,9 @@ def stream_copy(src: DeviceArray, dst: DeviceArray, nbytes: int, stream: Optiona
 # ---- partitioner / join-aggregate / plans / learned hooks ------------------
 
 def partition(keys: DeviceArray, m: int, radix_bits: int, ignore_bits: int, passes: int = 1,
-              stream: Optional[Stream] = None):
-    """Radix-partitions m keys; returns (keys, payloads, offsets, counts) as device arrays."""
+              stream: Optional[Stream] = None, dropped: Optional[DeviceArray] = None):
+    """Radix-partitions m keys; returns (keys, payloads, offsets, counts) as device arrays.
+    `dropped` (one uint64, optional) counts tuples that did not fit the output: 0 unless a fault."""
     kb = keys.dtype.itemsize
     pp = PartitionParams(radix_bits, ignore_bits, passes)
     cap = int(_lib.call("gj_partition_capacity", m, kb, radix_bits))
@@ -341,7 +346,7 @@ def partition(keys: DeviceArray, m: int, radix_bits: int, ignore_bits: int, pass
     off = DeviceArray(1 << radix_bits, np.uint64)
     cnt = DeviceArray(1 << radix_bits, np.uint64)
     scratch = DeviceArray(max(scratch_bytes, 16), np.uint8)
-    out = Partitioned(dk.ptr, dp.ptr, off.ptr, cnt.ptr, cap)
+    out = Partitioned(dk.ptr, dp.ptr, off.ptr, cnt.ptr, cap, dropped.ptr if dropped is not None else None)
     check(_lib.call("gj_partition", keys.ptr, m, kb, C.byref(pp), C.byref(out), scratch.ptr, scratch_bytes,
                     _s(stream)))
     return dk, dp, off, cnt

@@ -36,6 +36,8 @@ BATCH_LAUNCH, BATCH_GRAPH, BATCH_PERSISTENT = range(3)
 # generators
 KEYS_DENSE, KEYS_UNIFORM, KEYS_LOGNORMAL, KEYS_CLUSTERED = range(4)
 PROBES_UNIFORM, PROBES_ZIPF = range(2)
+# gj_index_params.learned_flags
+LEARNED_UNTRAINED = 0x1
 
 
 class IndexParams(C.Structure):
@@ -43,7 +45,7 @@ class IndexParams(C.Structure):
         ("rs_radix_bits", C.c_uint32), ("rs_max_error", C.c_uint32), ("harmonia_group_lanes", C.c_uint32),
         ("btree_lds_bytes", C.c_uint32), ("learned_leaf_bits", C.c_uint32), ("learned_max_error", C.c_uint32),
         ("mlp_hidden", C.c_uint32), ("mlp_steps", C.c_uint32), ("mlp_batch", C.c_uint32), ("mlp_lr", C.c_float),
-        ("seed", C.c_uint64), ("learned_root_segments", C.c_uint32),
+        ("seed", C.c_uint64), ("learned_root_segments", C.c_uint32), ("learned_flags", C.c_uint32),
     ]
 
 
@@ -60,7 +62,7 @@ class IndexStats(C.Structure):
         ("depth", C.c_uint32), ("lds_levels", C.c_uint32), ("spline_points", C.c_uint64), ("leaves", C.c_uint64),
         ("fallback_leaves", C.c_uint64), ("fallback_keys", C.c_uint64), ("max_window", C.c_uint32),
         ("avg_window", C.c_double), ("train_loss", C.c_float), ("root_segments", C.c_uint32),
-        ("split_leaves", C.c_uint32),
+        ("split_leaves", C.c_uint32), ("dropped_tuples", C.c_uint64),
     ]
 
 
@@ -71,7 +73,7 @@ class PartitionParams(C.Structure):
 class Partitioned(C.Structure):
     _fields_ = [
         ("d_keys", C.c_void_p), ("d_payloads", C.c_void_p), ("d_offsets", C.c_void_p), ("d_counts", C.c_void_p),
-        ("capacity", C.c_uint64),
+        ("capacity", C.c_uint64), ("d_dropped", C.c_void_p),
     ]
 
 
@@ -127,7 +129,7 @@ SIGNATURES = {
     "gj_mlp_loss_grad": (_I, [_P, _U32, _P, _P, _U64, _P, _P, _P]),
     "gj_mlp_forward": (_I, [_P, _U32, _P, _U64, _P, _P]),
     "gj_gather": (_I, [_P, _U64, _P, _U64, _P, _U32, _U32, _P]),
-    "gj_pointer_chase": (_I, [_P, _U64, _U32, _U32, _P, _P]),
+    "gj_pointer_chase": (_I, [_P, _U64, _U32, _U32, _P, _P, _P]),
     "gj_stream_copy": (_I, [_P, _P, _U64, _P]),
     "gj_last_error": (C.c_char_p, []),
     "gj_version": (C.c_char_p, []),
@@ -157,6 +159,7 @@ SIGNATURES = {
     "gj_sort_dimension": (_I, [_P, _P, _U64, _P, _P]),
     "gj_sort_keys": (_I, [_P, _U64, _I, _I]),
     "gj_rng_u64": (_U64, [_U64, _U64]),
+    "gj_zipf_trial": (_U64, [_U64, C.c_double, C.c_double]),
 }
 
 _lib = None

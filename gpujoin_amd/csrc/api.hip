@@ -121,6 +121,8 @@ int gj_index_build(gj_index_kind kind, const void* h_sorted_keys, const void* d_
       GJ_HIP_CHECK(hipStreamSynchronize(s));
     }
     build_index(*ix, s);
+    ix->dropped.alloc(16);
+    fill_bytes(ix->dropped.p, 0, ix->dropped.bytes, s);
     GJ_HIP_CHECK(hipStreamSynchronize(s));
     ix->h_col = nullptr;  // the host view is only valid during construct
     ix->stats.kind = kind;
@@ -151,6 +153,12 @@ int gj_index_get_stats(const gj_index* index, gj_index_stats* out) {
     GJ_REQUIRE(index && out, GJ_ERR_INVALID, "null argument");
     *out = index->stats;
     out->memory_bytes = memory_of(*index);
+    out->dropped_tuples = 0;
+    if (index->dropped.p) {
+      // the counter is written by kernels on any stream: wait for the device, then read it
+      GJ_HIP_CHECK(hipDeviceSynchronize());
+      GJ_HIP_CHECK(hipMemcpy(&out->dropped_tuples, index->dropped.p, sizeof(uint64_t), hipMemcpyDeviceToHost));
+    }
   });
 }
 

@@ -647,10 +647,11 @@ int gj_join_aggregate(const gj_index* index, const gj_fact_batch* batch, const g
     check_agg_index(*index);
     GJ_REQUIRE(params->d_flag, GJ_ERR_INVALID, "null flag column");
     const FactBatchDev b = to_dev(*batch, d_result);
+    dbuf& partial = index->scratch_for(S(stream)).a;
+    partial.reserve(agg_partial_cap(index->cu) * sizeof(AggPartial));
     with_index_dev<uint32_t>(*index, [&](auto d) {
-      index->scratch.reserve(agg_partial_cap(index->cu) * sizeof(AggPartial));
-      launch_join_agg(d, b, params->date_lo, params->date_hi, params->d_flag, index->cu,
-                      index->scratch.as<AggPartial>(), S(stream));
+      launch_join_agg(d, b, params->date_lo, params->date_hi, params->d_flag, index->cu, partial.as<AggPartial>(),
+                      S(stream));
     });
   });
 }

@@ -287,6 +287,11 @@ extern "C" {
 
 uint64_t gj_rng_u64(uint64_t seed, uint64_t index) { return rng(seed, index); }
 
+uint64_t gj_zipf_trial(uint64_t n, double theta, double u01) {
+  if (n == 0) return 0;
+  return Zipf(n, theta).trial(u01);
+}
+
 int gj_gen_build_keys(void* h_out, uint64_t n, int key_bytes, gj_key_dist dist, uint32_t upper_bits, uint64_t seed,
                       int threads) {
   return guarded([&] {

@@ -53,8 +53,9 @@ void lookup_partitioned(const gj_index& ix, const K* probes, uint64_t m, uint32_
   const uint64_t o_count = o_start + up((fan + 1) * 8);
   const uint64_t o_part = o_count + up(fan * 8);
   const uint64_t bytes = o_part + up(partition_scratch(window, sizeof(K), bits, passes));
-  ix.scratch.reserve(bytes);
-  uint8_t* base = ix.scratch.as<uint8_t>();
+  dbuf& scratch = ix.scratch_for(s).a;
+  scratch.reserve(bytes);
+  uint8_t* base = scratch.as<uint8_t>();
   K* tkeys = reinterpret_cast<K*>(base + o_keys);
   uint32_t* tpay = reinterpret_cast<uint32_t*>(base + o_pay);
   uint64_t* start = reinterpret_cast<uint64_t*>(base + o_start);
@@ -68,7 +69,7 @@ void lookup_partitioned(const gj_index& ix, const K* probes, uint64_t m, uint32_
     const uint64_t w = std::min<uint64_t>(window, m - w0);
     // unordered: the payload carries each probe's position back to `out`
     partition_keys<K>(probes + w0, w, static_cast<uint32_t>(w0), bits, ignore, passes, false, tkeys, tpay, start,
-                      count, base + o_part, ix.cu, s);
+                      count, base + o_part, ix.dropped.as<unsigned long long>(), ix.cu, s);
     with_index_dev<K>(ix, [&](auto d) {
       launch_tuples(d, tkeys, tpay, start + fan, partition_capacity(w, sizeof(K), bits), out, tp, s, ix.cu);
     });
@@ -83,9 +84,10 @@ template <class K>
 void lookup_hash_join(const gj_index& ix, const K* probes, uint64_t m, uint32_t* out, const gj_lookup_params& p,
                       hipStream_t s) {
   uint64_t cap = 0;
-  build_hash_table<K>(ix, ix.scratch2, cap, s);
+  dbuf& table = ix.scratch_for(s).b;
+  build_hash_table<K>(ix, table, cap, s);
   HashDev<K> d{};
-  d.table = ix.scratch2.as<const HashEntry>();
+  d.table = table.as<const HashEntry>();
   d.mask = cap - 1;
   launch_plain(d, probes, m, out, p, s, ix.cu);
 }

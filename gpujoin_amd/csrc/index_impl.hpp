@@ -1,7 +1,9 @@
 // index_impl.hpp - the gj_index object behind the opaque C handle.
 #pragma once
 
+#include <map>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "common.hpp"
@@ -68,9 +70,23 @@ struct gj_index {
   gj::dbuf ht;
   uint64_t ht_cap = 0;
 
-  // scratch reused by the bws / partitioning / hj drivers
-  mutable gj::dbuf scratch;
-  mutable gj::dbuf scratch2;
+  // Device scratch of the calls that need memory beyond their inputs and outputs
+  // (the partitioned driver's tuples, the hash-join driver's table, the
+  // join-aggregate's partial sums): one set per stream, grown on demand and kept
+  // until the index is destroyed, so calls on different streams never share it.
+  struct StreamScratch {
+    gj::dbuf a, b;
+  };
+  mutable std::mutex scratch_mu;
+  mutable std::map<hipStream_t, std::unique_ptr<StreamScratch>> scratch_by_stream;
+  StreamScratch& scratch_for(hipStream_t s) const {
+    std::lock_guard<std::mutex> g(scratch_mu);
+    auto& p = scratch_by_stream[s];
+    if (!p) p = std::make_unique<StreamScratch>();
+    return *p;
+  }
+  // u64 device counter of tuples the partitioned driver's scatter could not place
+  gj::dbuf dropped;
 
   gj_index_stats stats{};
 

@@ -531,13 +531,19 @@ void build_learned(gj_index& ix, hipStream_t s) {
     float* loss = grad + 768;
     SampleSource<K> src{nullptr, nullptr, col, n, kmin, root.inv_range, ix.params.seed};
     const uint64_t batch = std::min<uint64_t>(ix.params.mlp_batch, std::max<uint64_t>(n, 1));
-    for (uint32_t step = 1; step <= ix.params.mlp_steps; ++step) {
+    const uint32_t steps = (ix.params.learned_flags & GJ_LEARNED_UNTRAINED) ? 0u : ix.params.mlp_steps;
+    for (uint32_t step = 1; step <= steps; ++step) {
       src.seed = ix.params.seed + static_cast<uint64_t>(step) * 0x632BE59BD9B4E019ull;
-      if (step == ix.params.mlp_steps) fill_bytes(loss, 0, 16, s);
+      if (step == steps) fill_bytes(loss, 0, 16, s);
       launch_mlp_grad<K>(h, ix.lm_mlp.as<float>(), src, batch, grad, loss, s);
       hipLaunchKernelGGL(adam_kernel, dim3(1), dim3(256), 0, s, ix.lm_mlp.as<float>(), grad, m1, m2, h,
                          ix.params.mlp_lr, static_cast<int>(step));
       GJ_HIP_CHECK(hipGetLastError());
+    }
+    if (steps == 0) {
+      // untrained root: its loss on one minibatch of the same sampler, parameters untouched
+      src.seed = ix.params.seed + 0x632BE59BD9B4E019ull;
+      launch_mlp_grad<K>(h, ix.lm_mlp.as<float>(), src, batch, grad, loss, s);
     }
     float hl = 0;
     GJ_HIP_CHECK(hipMemcpyAsync(&hl, loss, sizeof(float), hipMemcpyDeviceToHost, s));

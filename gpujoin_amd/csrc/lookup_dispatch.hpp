@@ -136,7 +136,7 @@ uint64_t partition_scratch(uint64_t m, int key_bytes, uint32_t bits, uint32_t pa
 template <class K>
 void partition_keys(const K* keys, uint64_t m, uint32_t payload_base, uint32_t bits, uint32_t ignore, uint32_t passes,
                     bool stable, K* out_keys, uint32_t* out_payload, uint64_t* start, uint64_t* count, uint8_t* scratch,
-                    int cu, hipStream_t s);
+                    unsigned long long* dropped, int cu, hipStream_t s);
 
 // Bytes of index-private device memory held by the extended kinds.
 uint64_t extended_memory(const gj_index& ix);

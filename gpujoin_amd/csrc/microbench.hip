@@ -37,11 +37,12 @@ __global__ __launch_bounds__(256) void gather_kernel(const uint64_t* __restrict_
   }
 }
 
-__global__ void chase_kernel(const uint32_t* __restrict__ next, uint64_t n, uint32_t chains, uint32_t steps,
-                             uint32_t* __restrict__ out) {
+// start may alias out: each lane reads its start before it writes its end.
+__global__ void chase_kernel(const uint32_t* next, uint64_t n, uint32_t chains, uint32_t steps,
+                             const uint32_t* start, uint32_t* out) {
   const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= chains) return;
-  uint32_t cur = static_cast<uint32_t>((static_cast<uint64_t>(c) * 0x9E3779B97F4A7C15ull) % n);
+  uint32_t cur = start ? start[c] : static_cast<uint32_t>((static_cast<uint64_t>(c) * 0x9E3779B97F4A7C15ull) % n);
   for (uint32_t s = 0; s < steps; ++s) cur = next[cur];
   out[c] = cur;
 }
@@ -89,13 +90,13 @@ int gj_gather(const uint64_t* d_array, uint64_t n_pow2, const uint64_t* d_probes
   });
 }
 
-int gj_pointer_chase(const uint32_t* d_next, uint64_t n, uint32_t chains, uint32_t steps, uint32_t* d_out,
-                     gj_stream_t stream) {
+int gj_pointer_chase(const uint32_t* d_next, uint64_t n, uint32_t chains, uint32_t steps, const uint32_t* d_start,
+                     uint32_t* d_out, gj_stream_t stream) {
   return guarded([&] {
     GJ_REQUIRE(d_next && d_out && n > 0 && n <= 0xFFFFFFFFull, GJ_ERR_INVALID, "bad argument");
     if (chains == 0) return;
     hipLaunchKernelGGL(chase_kernel, dim3((chains + 63) / 64), dim3(64), 0, S(stream), d_next, n, chains, steps,
-                       d_out);
+                       d_start, d_out);
     GJ_HIP_CHECK(hipGetLastError());
   });
 }

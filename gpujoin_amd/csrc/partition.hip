@@ -42,6 +42,12 @@ __device__ __forceinline__ uint32_t digit_of(K k, uint32_t shift, uint32_t mask)
   return static_cast<uint32_t>(static_cast<uint64_t>(k) >> shift) & mask;
 }
 
+// A store that would land past the output's capacity is not made; it is counted
+// instead (never for a consistent histogram), so callers can report the loss.
+__device__ __forceinline__ void drop(unsigned long long* dropped) {
+  if (dropped) atomicAdd(dropped, 1ull);
+}
+
 // One block per chunk (256 or 1024 threads), four loads in flight per thread.
 template <class K>
 __global__ __launch_bounds__(1024) void part_hist_kernel(const K* __restrict__ keys, uint64_t m, uint64_t chunk,
@@ -150,7 +156,8 @@ template <class K>
 __global__ __launch_bounds__(kPartBlock) void part_scatter_kernel(
     const K* __restrict__ keys, const uint32_t* __restrict__ in_payload, uint32_t payload_base, uint64_t m,
     uint64_t chunk, uint32_t shift, uint32_t bits, const uint32_t* __restrict__ block_off,
-    const uint64_t* __restrict__ start, K* __restrict__ out_keys, uint32_t* __restrict__ out_payload, uint64_t cap) {
+    const uint64_t* __restrict__ start, K* __restrict__ out_keys, uint32_t* __restrict__ out_payload, uint64_t cap,
+    unsigned long long* __restrict__ dropped) {
   extern __shared__ uint32_t sm[];
   const uint32_t fan = 1u << bits;
   uint32_t* cnt = sm;                        // [kPartWaves][fan]: counts, then cursors
@@ -219,9 +226,11 @@ __global__ __launch_bounds__(kPartBlock) void part_scatter_kernel(
       if (peers) {
         const uint32_t at = mine[d];  // every lane of the group reads the cursor before the leader moves it
         const uint32_t pos = at + static_cast<uint32_t>(__popcll(peers & lt));
-        if (pos < cap) {  // always true for a consistent histogram; guards the buffers regardless
+        if (pos < cap) {  // always true for a consistent histogram
           out_keys[pos] = k[j];
           out_payload[pos] = pay[j];
+        } else {
+          drop(dropped);
         }
         if ((peers & lt) == 0) mine[d] = at + static_cast<uint32_t>(__popcll(peers));
       }
@@ -248,7 +257,8 @@ template <class K>
 __global__ __launch_bounds__(kPartBlock) void part_scatter_any_kernel(
     const K* __restrict__ keys, const uint32_t* __restrict__ in_payload, uint32_t payload_base, uint64_t m,
     uint64_t chunk, uint32_t shift, uint32_t bits, const uint32_t* __restrict__ block_off,
-    const uint64_t* __restrict__ start, K* __restrict__ out_keys, uint32_t* __restrict__ out_payload, uint64_t cap) {
+    const uint64_t* __restrict__ start, K* __restrict__ out_keys, uint32_t* __restrict__ out_payload, uint64_t cap,
+    unsigned long long* __restrict__ dropped) {
   extern __shared__ uint32_t brun[];  // [fan]: block cursor (absolute position)
   const uint32_t fan = 1u << bits;
   const uint64_t c = xcd_logical_block();
@@ -272,9 +282,11 @@ __global__ __launch_bounds__(kPartBlock) void part_scatter_any_kernel(
       const uint64_t i = tile + static_cast<uint64_t>(j * kPartBlock + threadIdx.x);
       if (i < hi) {
         const uint32_t pos = atomicAdd(&brun[digit_of(k[j], shift, fan - 1)], 1u);
-        if (pos < cap) {  // always true for a consistent histogram; guards the buffers regardless
+        if (pos < cap) {  // always true for a consistent histogram
           out_keys[pos] = k[j];
           out_payload[pos] = pay[j];
+        } else {
+          drop(dropped);
         }
       }
     }
@@ -326,7 +338,8 @@ template <class K, bool Stable>
 __global__ __launch_bounds__(wc_block(Stable)) void part_scatter_wc_kernel(
     const K* __restrict__ keys, const uint32_t* __restrict__ in_payload, uint32_t payload_base, uint64_t m,
     uint64_t chunk, uint32_t shift, uint32_t bits, uint32_t seg_bits, const uint32_t* __restrict__ block_off,
-    const uint64_t* __restrict__ start, K* __restrict__ out_keys, uint32_t* __restrict__ out_payload, uint64_t cap) {
+    const uint64_t* __restrict__ start, K* __restrict__ out_keys, uint32_t* __restrict__ out_payload, uint64_t cap,
+    unsigned long long* __restrict__ dropped) {
   constexpr int kBlock = wc_block(Stable);
   constexpr int kWaves = kBlock / kWave;
   constexpr int kItems = kWcTile / kBlock;  // items per lane per tile
@@ -476,6 +489,8 @@ __global__ __launch_bounds__(wc_block(Stable)) void part_scatter_wc_kernel(
         sk[slot] = k[j];
         sp[slot] = pay[j];
         pos[j] = cur[d] + slot - (off[d] & 0xFFFFu);
+      } else if (valid) {
+        drop(dropped);
       }
     }
     __syncthreads();
@@ -485,6 +500,7 @@ __global__ __launch_bounds__(wc_block(Stable)) void part_scatter_wc_kernel(
       const uint32_t o = off[p];
       const uint32_t cp0 = cur[p];
       const uint32_t at = (cp0 & ~lmask) + ((sg - (o >> 16)) << seg_bits) + q;
+      if (at >= run0[p] && at >= cap) drop(dropped);
       if (at >= run0[p] && at < cap) {
         K kk;
         uint32_t pp;
@@ -525,6 +541,7 @@ __global__ __launch_bounds__(wc_block(Stable)) void part_scatter_wc_kernel(
   // open segments: the slots of the block's run in [segment start, cursor)
   for (uint32_t p = g; p < fan; p += groups) {
     const uint32_t at = (cur[p] & ~lmask) + q;
+    if (at >= run0[p] && at < cur[p] && at >= cap) drop(dropped);
     if (at >= run0[p] && at < cur[p] && at < cap) {
       out_keys[at] = ck[p * L + q];
       out_payload[at] = cp[p * L + q];
@@ -592,7 +609,7 @@ uint64_t scratch_for(uint64_t m, int key_bytes, uint32_t bits, uint32_t passes, 
 template <class K>
 void partition_pass(const K* keys, const uint32_t* in_payload, uint32_t payload_base, uint64_t m, uint32_t shift,
                     uint32_t bits, uint32_t align, bool stable, K* out_keys, uint32_t* out_payload, uint64_t* start,
-                    uint64_t* count, uint32_t* hist, int cu, hipStream_t s) {
+                    uint64_t* count, uint32_t* hist, unsigned long long* dropped, int cu, hipStream_t s) {
   const uint32_t sb = wc_seg_bits(bits, sizeof(K), stable);
   size_t slds = 0;
   uint32_t per_cu = bits > 10 ? 2 : 4;  // the register-tile scatter holds 5 x 4 B per partition
@@ -622,7 +639,7 @@ void partition_pass(const K* keys, const uint32_t* in_payload, uint32_t payload_
         GJ_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
                                          hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(slds)));
       hipLaunchKernelGGL(kernel, dim3(p.blocks), dim3(wc_block(stable)), slds, s, keys, in_payload, payload_base, m,
-                         p.chunk, shift, bits, sb, hist, start, out_keys, out_payload, cap);
+                         p.chunk, shift, bits, sb, hist, start, out_keys, out_payload, cap, dropped);
       GJ_HIP_CHECK(hipGetLastError());
     };
     if (stable)
@@ -633,7 +650,7 @@ void partition_pass(const K* keys, const uint32_t* in_payload, uint32_t payload_
   }
   if (!stable) {
     hipLaunchKernelGGL(part_scatter_any_kernel<K>, dim3(p.blocks), dim3(kPartBlock), lds, s, keys, in_payload,
-                       payload_base, m, p.chunk, shift, bits, hist, start, out_keys, out_payload, cap);
+                       payload_base, m, p.chunk, shift, bits, hist, start, out_keys, out_payload, cap, dropped);
     GJ_HIP_CHECK(hipGetLastError());
     return;
   }
@@ -642,7 +659,7 @@ void partition_pass(const K* keys, const uint32_t* in_payload, uint32_t payload_
     GJ_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&part_scatter_kernel<K>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(rlds)));
   hipLaunchKernelGGL(part_scatter_kernel<K>, dim3(p.blocks), dim3(kPartBlock), rlds, s, keys, in_payload, payload_base,
-                     m, p.chunk, shift, bits, hist, start, out_keys, out_payload, cap);
+                     m, p.chunk, shift, bits, hist, start, out_keys, out_payload, cap, dropped);
   GJ_HIP_CHECK(hipGetLastError());
 }
 
@@ -659,12 +676,12 @@ uint64_t partition_scratch(uint64_t m, int key_bytes, uint32_t bits, uint32_t pa
 template <class K>
 void partition_keys(const K* keys, uint64_t m, uint32_t payload_base, uint32_t bits, uint32_t ignore, uint32_t passes,
                     bool stable, K* out_keys, uint32_t* out_payload, uint64_t* start, uint64_t* count, uint8_t* scratch,
-                    int cu, hipStream_t s) {
+                    unsigned long long* dropped, int cu, hipStream_t s) {
   uint32_t* hist = reinterpret_cast<uint32_t*>(scratch);
   const uint32_t align = part_align(sizeof(K));
   if (passes == 1 || bits < 2) {
     partition_pass<K>(keys, nullptr, payload_base, m, ignore, bits, align, stable, out_keys, out_payload, start, count,
-                      hist, cu, s);
+                      hist, dropped, cu, s);
     return;
   }
   // pass 1: the high half of the digit, unpadded, into scratch; pass 2: the full digit.
@@ -673,13 +690,16 @@ void partition_keys(const K* keys, uint64_t m, uint32_t payload_base, uint32_t b
   K* tk = reinterpret_cast<K*>(tmp);
   uint32_t* tp = reinterpret_cast<uint32_t*>(tmp + ceil_div((m + 32) * sizeof(K), 256) * 256);
   partition_pass<K>(keys, nullptr, payload_base, m, ignore + (bits - b_hi), b_hi, 1, stable, tk, tp, start, count,
-                    hist, cu, s);
-  partition_pass<K>(tk, tp, 0, m, ignore, bits, align, stable, out_keys, out_payload, start, count, hist, cu, s);
+                    hist, dropped, cu, s);
+  partition_pass<K>(tk, tp, 0, m, ignore, bits, align, stable, out_keys, out_payload, start, count, hist, dropped, cu,
+                    s);
 }
 template void partition_keys<uint32_t>(const uint32_t*, uint64_t, uint32_t, uint32_t, uint32_t, uint32_t, bool,
-                                       uint32_t*, uint32_t*, uint64_t*, uint64_t*, uint8_t*, int, hipStream_t);
+                                       uint32_t*, uint32_t*, uint64_t*, uint64_t*, uint8_t*, unsigned long long*, int,
+                                       hipStream_t);
 template void partition_keys<uint64_t>(const uint64_t*, uint64_t, uint32_t, uint32_t, uint32_t, uint32_t, bool,
-                                       uint64_t*, uint32_t*, uint64_t*, uint64_t*, uint8_t*, int, hipStream_t);
+                                       uint64_t*, uint32_t*, uint64_t*, uint64_t*, uint8_t*, unsigned long long*, int,
+                                       hipStream_t);
 
 }  // namespace gj
 
@@ -730,7 +750,8 @@ int gj_partition(const void* d_keys, uint64_t m, int key_bytes, const gj_partiti
     start = reinterpret_cast<uint64_t*>(reinterpret_cast<uintptr_t>(start) & ~static_cast<uintptr_t>(7));
     dispatch_key(key_bytes, [&]<class K>() {
       partition_keys<K>(static_cast<const K*>(d_keys), m, 0, bits, params->ignore_bits, params->passes, true,
-                        static_cast<K*>(out->d_keys), out->d_payloads, start, out->d_counts, sc, cu, s);
+                        static_cast<K*>(out->d_keys), out->d_payloads, start, out->d_counts, sc,
+                        reinterpret_cast<unsigned long long*>(out->d_dropped), cu, s);
     });
     GJ_HIP_CHECK(hipMemcpyAsync(out->d_offsets, start, (1ull << bits) * 8, hipMemcpyDeviceToDevice, s));
   });

@@ -94,7 +94,13 @@ typedef struct gj_index_params {
   uint32_t learned_root_segments; /* learned_plr root: 0 = automatic (linear; a CDF spline of <= 2048
                                      segments when the linear root routes keys to the B-tree), 1 = linear,
                                      otherwise the most spline segments (a power of two, <= 4096) */
+  uint32_t learned_flags;         /* GJ_LEARNED_* bits, default 0 */
 } gj_index_params;
+
+/* learned_flags: keep the MLP root at its initialisation (the piecewise-linear CDF through
+ * mlp_hidden + 1 quantiles of the column) and run no training step; the root's loss is
+ * still measured and reported in gj_index_stats.train_loss. */
+#define GJ_LEARNED_UNTRAINED 0x1u
 
 void gj_index_params_default(gj_index_params* p);
 
@@ -135,9 +141,10 @@ int gj_index_build(gj_index_kind kind, const void* h_sorted_keys, const void* d_
  * (src/index_lookup_new.cu:227-249), partitioning_approach::run
  * (src/index_lookup_partitioning.cu:493-507) and hj_approach::run
  * (src/index_lookup_new.cu:262-301).  Probe keys have the index's key width.
- * m may be 0.  Stream-ordered; scratch memory for the bws / partitioning /
- * hj drivers is owned by the index and reused, so concurrent lookups on one
- * index from different streams race. */
+ * m may be 0.  Stream-ordered.  The partitioning and hj drivers use device
+ * scratch that the index owns: one buffer set per stream, grown on first use
+ * and kept until gj_index_destroy, so lookups on one index from different
+ * streams may run concurrently (calls on one stream are ordered anyway). */
 int gj_index_lookup(const gj_index* index, const void* d_probe_keys, uint64_t m, uint32_t* d_rowids,
                     const gj_lookup_params* params, gj_stream_t stream);
 
@@ -162,8 +169,12 @@ typedef struct gj_index_stats {
   float train_loss;           /* final MLP training loss (learned_mlp) */
   uint32_t root_segments;     /* learned_plr: segments of the spline root (0: linear root) */
   uint32_t split_leaves;      /* learned: leaves too wide for one line, split into 16 sub-leaves */
+  uint64_t dropped_tuples;    /* probes the partitioned driver's scatter could not place (0 unless an
+                                 internal fault occurred; their row ids were not written) */
 } gj_index_stats;
 
+/* Fills *out.  dropped_tuples is read from the device: the call waits for the
+ * work already queued on the index's lookups. */
 int gj_index_get_stats(const gj_index* index, gj_index_stats* out);
 
 void gj_index_destroy(gj_index* index);
@@ -196,6 +207,9 @@ typedef struct gj_partitioned {
   uint64_t* d_offsets;   /* 2^radix_bits partition starts */
   uint64_t* d_counts;    /* 2^radix_bits partition sizes */
   uint64_t capacity;     /* tuples d_keys / d_payloads can hold: >= gj_partition_capacity(m, ...) */
+  uint64_t* d_dropped;   /* optional device counter: tuples whose store would fall past `capacity` are not
+                            stored but added here (0 for any consistent partitioning; a non-zero value
+                            reports an internal fault instead of a silently missing tuple) */
 } gj_partitioned;
 
 uint64_t gj_partition_capacity(uint64_t m, int key_bytes, uint32_t radix_bits);
@@ -213,7 +227,10 @@ int gj_partition(const void* d_keys, uint64_t m, int key_bytes, const gj_partiti
  * index over the dimension keys: s = price * (100 - discount) (int64);
  * denominator += s; numerator += s when d_flag[rowid] != 0.
  * The index must have key_bytes 4 (fkeys are read as uint32).  Results are
- * ADDED to d_result (the caller zeroes it), so several batches accumulate. */
+ * ADDED to d_result (the caller zeroes it), so several batches accumulate.
+ * Workgroup partial sums go to scratch that the index owns per stream (64 KiB
+ * at 256 CUs, allocated on a stream's first call), so calls from different
+ * streams may overlap. */
 typedef struct gj_fact_batch {
   const int32_t* d_fkey;
   const int32_t* d_date;
@@ -319,9 +336,13 @@ int gj_mlp_forward(const float* d_params, uint32_t hidden, const float* d_x, uin
 int gj_gather(const uint64_t* d_array, uint64_t n_pow2, const uint64_t* d_probes, uint64_t m, uint32_t* d_out,
               uint32_t block_size, uint32_t items_per_thread, gj_stream_t stream);
 /* Dependent chain: cur = d_next[cur] for steps steps from `chains` independent
- * starts; writes the final positions to d_out (chains entries). */
-int gj_pointer_chase(const uint32_t* d_next, uint64_t n, uint32_t chains, uint32_t steps, uint32_t* d_out,
-                     gj_stream_t stream);
+ * starts; writes the final positions to d_out (chains entries).  Chain c starts
+ * at d_start[c] when d_start is given (it may be d_out itself, so that each
+ * launch continues where the previous one stopped and never re-walks lines a
+ * cache may still hold), else at (c * 0x9E3779B97F4A7C15) mod n.  Every d_next
+ * entry and start must be < n. */
+int gj_pointer_chase(const uint32_t* d_next, uint64_t n, uint32_t chains, uint32_t steps, const uint32_t* d_start,
+                     uint32_t* d_out, gj_stream_t stream);
 /* Streaming copy of bytes (multiple of 16) from d_src to d_dst. */
 int gj_stream_copy(const void* d_src, void* d_dst, uint64_t bytes, gj_stream_t stream);
 

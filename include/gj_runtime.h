@@ -98,6 +98,11 @@ int gj_sort_keys(void* h_keys, uint64_t n, int key_bytes, int threads);
 /* splitmix64 finaliser of (seed, index); the counter-based stream every generator draws from. */
 uint64_t gj_rng_u64(uint64_t seed, uint64_t index);
 
+/* One rejection-inversion trial of the Zipf(theta) sampler on {1..n} that GJ_PROBES_ZIPF draws
+ * from, fed the uniform u01 in [0, 1): the rank k, or 0 when the trial is rejected
+ * (zipf_distribution::operator(), src/zipf.hpp:30-42). */
+uint64_t gj_zipf_trial(uint64_t n, double theta, double u01);
+
 #ifdef __cplusplus
 }
 #endif

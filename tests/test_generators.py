@@ -75,3 +75,19 @@ def test_dimension_and_fact(gj):
     # a 30-day window selects ~30/2557 of the rows
     sel = ((date >= 1000) & (date < 1030)).mean()
     assert abs(sel - 30 / 2557) < 0.002
+
+
+@pytest.mark.parametrize("n,q", [(99_999, 1.0), ((1 << 26) - 1, 1.0), (1000, 0.5), (1_000_000, 1.5), (2, 1.0)])
+def test_zipf_trial_equals_oracle_on_equal_uniforms(gj, orc, n, q):
+    # libgj's rejection-inversion trial (what GJ_PROBES_ZIPF draws ranks with) against the
+    # oracle's, which is pinned bit for bit to the reference's zipf_distribution (src/zipf.hpp)
+    # by the committed draws of tests/golden/zipf_ref.npz: equal uniforms, equal ranks
+    lib = gj._lib.load()
+    rng = np.random.default_rng(n)
+    us = np.concatenate([np.linspace(0.0, 1.0, 4001, endpoint=False), rng.random(4000),
+                         [0.0, 1e-12, 0.5, 1.0 - 2.0 ** -53]])
+    got = [int(lib.gj_zipf_trial(n, q, float(u))) for u in us]
+    want = [orc.zipf_trial(n, q, float(u)) for u in us]
+    assert got == want
+    accepted = [k for k in got if k]
+    assert accepted and min(accepted) >= 1 and max(accepted) <= n

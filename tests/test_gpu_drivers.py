@@ -26,8 +26,10 @@ def test_partition_matches_oracle(gj, orc, gpu, kb, bits, passes, m):
     keys = rng.integers(0, np.iinfo(dt).max, size=m, dtype=dt)
     ignore = 8 * kb - bits - 3
     d = gj.DeviceArray.from_numpy(keys) if m else gj.DeviceArray(1, dt)
-    dk, dp, off, cnt = gj.partition(d, m, bits, ignore, passes)
+    dropped = gj.DeviceArray.from_numpy(np.zeros(1, np.uint64))
+    dk, dp, off, cnt = gj.partition(d, m, bits, ignore, passes, dropped=dropped)
     gj.synchronize()
+    assert int(dropped.numpy()[0]) == 0
     counts, ok, op = orc.partition(keys, bits, ignore)
     got_cnt = cnt.numpy()
     assert np.array_equal(got_cnt, counts)
@@ -57,8 +59,10 @@ def test_partition_many_tiles_per_block(gj, orc, gpu, bits, passes):
     rng = np.random.default_rng(bits + 17 * passes)
     keys = rng.integers(0, 2 ** 64 - 1, size=m, dtype=np.uint64)
     ignore = 64 - bits
-    dk, dp, off, cnt = gj.partition(gj.DeviceArray.from_numpy(keys), m, bits, ignore, passes)
+    dropped = gj.DeviceArray.from_numpy(np.zeros(1, np.uint64))
+    dk, dp, off, cnt = gj.partition(gj.DeviceArray.from_numpy(keys), m, bits, ignore, passes, dropped=dropped)
     gj.synchronize()
+    assert int(dropped.numpy()[0]) == 0, "scatter dropped tuples"
     counts, ok, op = orc.partition(keys, bits, ignore)
     assert np.array_equal(cnt.numpy(), counts)
     offs = off.numpy().astype(np.int64)
@@ -104,6 +108,7 @@ def test_approaches_match_oracle(gj, orc, gpu, kind, kb, approach, extra):
     out.fill(0x5A)
     idx.lookup(dp, out, approach=approach, **extra)
     gj.synchronize()
+    assert idx.stats()["dropped_tuples"] == 0
     want = orc.lookup_lower_bound(keys, probes)
     if kind == "no_op" and approach != "hj":
         want = np.full_like(want, gj.NOT_FOUND)
@@ -123,6 +128,7 @@ def test_partitioned_lookup_many_tiles(gj, orc, gpu, kb, bits):
     out.fill(0x5A)
     idx.lookup(dp, out, approach="partitioning", radix_bits=bits, window=1 << 22)
     gj.synchronize()
+    assert idx.stats()["dropped_tuples"] == 0
     assert np.array_equal(out.numpy(), orc.lookup_lower_bound(keys, probes))
 
 
