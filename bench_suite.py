@@ -353,7 +353,7 @@ def extras(gj, G, stream, keys64, d_keys64, d_out, cpu=True, steps=5, warmup=2):
                                "rowids_match": bool(np.array_equal(got, exp)),
                                "checksum": int(got.astype(np.uint64).sum())}
         idx.close()
-        log(f"C1 {kind}: {m / ms * 1e-6:.0f} M lookups/s match={c1['indexes'][kind]['rowids_match']}")
+        log(f"C1 {kind}: {m / ms * 1e-6:.2f} G lookups/s match={c1['indexes'][kind]['rowids_match']}")
     if cpu:
         sec, ref = _cpu_median(lambda: oracle.lookup_lower_bound(keys, probes, threads=cores))
         c1["cpu_baseline"] = {"lookups_per_s": m / sec, "cores": cores, "kind": "port",
@@ -376,7 +376,7 @@ def extras(gj, G, stream, keys64, d_keys64, d_out, cpu=True, steps=5, warmup=2):
         c2["indexes"][kind] = {"lookups_per_s": m / ms * 1e3, "gather_roofline_frac": m / ms * 1e3 / G,
                                "rowids_match": ok}
         idx.close()
-        log(f"C2b {kind}: {m / ms * 1e-9:.2f} G lookups/s match={ok}")
+        log(f"C2b {kind}: {m / ms * 1e-6:.2f} G lookups/s match={ok}")
     res["C2b_zipf"] = c2
     del dp, exp
 
@@ -408,7 +408,7 @@ def extras(gj, G, stream, keys64, d_keys64, d_out, cpu=True, steps=5, warmup=2):
                            "max_window": st["max_window"], "train_loss": st["train_loss"],
                            "index_bytes": st["memory_bytes"]}
             idx.close()
-            log(f"C4 {dname} {label}: {m / ms * 1e-9:.2f} G lookups/s match={ok} fallback={st['fallback_keys']}")
+            log(f"C4 {dname} {label}: {m / ms * 1e-6:.2f} G lookups/s match={ok} fallback={st['fallback_keys']}")
         bt = rows["btree"]["lookups_per_s"]
         for label in ("learned_plr", "learned_mlp"):
             rows[label]["vs_btree"] = rows[label]["lookups_per_s"] / bt
@@ -458,6 +458,7 @@ def join_aggregate_leg(gj, stream, cpu=True, fact_rows=60_000_000, reps=6):
         for c in copies:  # warm-up and check on both copies
             res.fill(0, stream)
             gj.join_aggregate(idx, *c["dev"], dflag, lo, hi, res, stream=stream)
+            stream.synchronize()
             ok &= tuple(int(v) for v in res.numpy()[:3]) == c["want"]
         e = [gj.Event() for _ in range(reps + 1)]
         stream.synchronize()
@@ -471,7 +472,7 @@ def join_aggregate_leg(gj, stream, cpu=True, fact_rows=60_000_000, reps=6):
         alg = fact_rows * 4 + copies[0]["n_qual"] * 13 + copies[0]["want"][2]
         out["one_batch"][kind] = {"rows_per_s": fact_rows / ms * 1e3, "ms": ms, "sums_match": ok,
                                   "algorithmic_GBps": alg / (ms * 1e-3) / 1e9}
-        log(f"C5 one batch {kind}: {fact_rows / ms * 1e-9:.0f} G rows/s match={ok}")
+        log(f"C5 one batch {kind}: {fact_rows / ms * 1e-6:.0f} G rows/s match={ok}")
         if kind != "learned_plr":
             idx.close()
             continue
@@ -488,6 +489,7 @@ def join_aggregate_leg(gj, stream, cpu=True, fact_rows=60_000_000, reps=6):
             ok = True
             for p, c in zip(plans, copies):
                 p.run(stream)
+                stream.synchronize()
                 sums = results.numpy().reshape(-1, 4)[:, :3].sum(axis=0)
                 ok &= tuple(int(v) for v in sums) == c["want"]
             stream.synchronize()
@@ -498,7 +500,7 @@ def join_aggregate_leg(gj, stream, cpu=True, fact_rows=60_000_000, reps=6):
             stream.synchronize()
             ms = e[0].elapsed_ms(e[reps]) / reps
             out["plans"][mname] = {"rows_per_s": fact_rows / ms * 1e3, "ms": ms, "sums_match": ok}
-            log(f"C5 plan {mname}: {fact_rows / ms * 1e-9:.0f} G rows/s match={ok}")
+            log(f"C5 plan {mname}: {fact_rows / ms * 1e-6:.0f} G rows/s match={ok}")
             for p in plans:
                 p.close()
         idx.close()
