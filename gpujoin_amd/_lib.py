@@ -117,6 +117,7 @@ SIGNATURES = {
     "gj_partition_scratch_bytes": (_U64, [_U64, _I, C.POINTER(PartitionParams)]),
     "gj_partition": (_I, [_P, _U64, _I, C.POINTER(PartitionParams), C.POINTER(Partitioned), _P, _U64, _P]),
     "gj_join_aggregate": (_I, [_P, C.POINTER(FactBatch), C.POINTER(AggParams), _P, _P]),
+    "gj_join_aggregate_noalloc": (_I, [_P, C.POINTER(FactBatch), C.POINTER(AggParams), _P, _P]),
     "gj_promo_revenue": (C.c_int64, [C.c_int64, C.c_int64]),
     "gj_lookup_plan_create": (_I, [_P, C.POINTER(ProbeBatch), _U32, _I, _P, C.POINTER(_P)]),
     "gj_aggregate_plan_create": (_I, [_P, C.POINTER(FactBatch), _U32, C.POINTER(AggParams), _P, _I, _P,

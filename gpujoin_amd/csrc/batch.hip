@@ -18,6 +18,7 @@
 // workgroups pull (batch, rows) work items from a device counter until all are
 // done.
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "common.hpp"
@@ -380,16 +381,22 @@ namespace {
 // Workgroup partial sums: at most 8 workgroups of 256 threads fit one CU.
 inline uint64_t agg_partial_cap(int cu) { return 8ull * static_cast<uint64_t>(cu); }
 
+// partial holds partial_cap slots (0: agg_partial_cap(cu), what the callers' buffers are sized for).
 template <class Dev>
 void launch_join_agg(const Dev& d, const FactBatchDev& b, int32_t lo, int32_t hi, const uint8_t* flag, int cu,
-                     AggPartial* partial, hipStream_t s) {
+                     AggPartial* partial, hipStream_t s, uint64_t partial_cap = 0) {
   if (b.rows == 0) return;
   const size_t lds = (d.lds_bytes() + 15) / 16 * 16 + agg_queue_bytes(kAggUnrollBatch);
   const void* fn = reinterpret_cast<const void*>(&join_agg_kernel<Dev>);
   uint64_t blocks = static_cast<uint64_t>(blocks_per_cu(fn, kAggBlock, lds)) * cu;
   blocks = std::min<uint64_t>(blocks, ceil_div(b.rows, static_cast<uint64_t>(kAggBlock) * kAggUnrollBatch));
   blocks = std::max<uint64_t>(blocks, 1);
-  GJ_REQUIRE(blocks <= agg_partial_cap(cu), GJ_ERR_INTERNAL, "join-aggregate grid exceeds the partial buffer");
+  if (partial_cap == 0) {
+    GJ_REQUIRE(blocks <= agg_partial_cap(cu), GJ_ERR_INTERNAL, "join-aggregate grid exceeds the partial buffer");
+  } else {
+    GJ_REQUIRE(blocks <= kAggDirectBlocks || blocks <= partial_cap, GJ_ERR_UNSUPPORTED,
+               "join-aggregate grid needs more workgroup partial slots than the static array holds");
+  }
   // a small grid adds its workgroup sums to the result itself (a few dozen atomics
   // cost less than the second launch)
   const bool direct = blocks <= kAggDirectBlocks;
@@ -399,6 +406,32 @@ void launch_join_agg(const Dev& d, const FactBatchDev& b, int32_t lo, int32_t hi
   if (direct) return;
   hipLaunchKernelGGL(agg_sum_kernel, dim3(1), dim3(1024), 0, s, partial, static_cast<uint32_t>(blocks), b.result);
   GJ_HIP_CHECK(hipGetLastError());
+}
+
+// Workgroup partial sums of gj_join_aggregate_noalloc, which must not allocate: a
+// file-scope device array (each GPU holds its own copy until the process ends)
+// instead of index-owned scratch.  Host code never passes `scratch` itself (on
+// the host that name is only a stand-in): noalloc_partials() resolves each
+// device's address once, with the device current, and every use goes through it.
+// Each workgroup writes its own slot before agg_sum_kernel reads the slots of
+// that launch, so nothing carried over from an earlier call is ever read.
+constexpr uint64_t kNoAllocPartials = 4096;  // 128 KiB: 8 workgroups per CU on up to 512 CUs
+static __device__ __align__(256) AggPartial scratch[kNoAllocPartials];
+
+AggPartial* noalloc_partials() {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static AggPartial* ptr[kMaxDevices] = {};
+  int dev = 0;
+  GJ_HIP_CHECK(hipGetDevice(&dev));
+  GJ_REQUIRE(dev >= 0 && dev < kMaxDevices, GJ_ERR_UNSUPPORTED, "device ordinal beyond 63");
+  std::lock_guard<std::mutex> g(mu);
+  if (!ptr[dev]) {
+    void* p = nullptr;
+    GJ_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    ptr[dev] = static_cast<AggPartial*>(p);
+  }
+  return ptr[dev];
 }
 
 void check_agg_index(const gj_index& ix) {
@@ -652,6 +685,21 @@ int gj_join_aggregate(const gj_index* index, const gj_fact_batch* batch, const g
     with_index_dev<uint32_t>(*index, [&](auto d) {
       launch_join_agg(d, b, params->date_lo, params->date_hi, params->d_flag, index->cu, partial.as<AggPartial>(),
                       S(stream));
+    });
+  });
+}
+
+int gj_join_aggregate_noalloc(const gj_index* index, const gj_fact_batch* batch, const gj_agg_params* params,
+                              gj_agg_result* d_result, gj_stream_t stream) {
+  return guarded([&] {
+    GJ_REQUIRE(index && batch && params && d_result, GJ_ERR_INVALID, "null argument");
+    check_agg_index(*index);
+    GJ_REQUIRE(params->d_flag, GJ_ERR_INVALID, "null flag column");
+    const FactBatchDev b = to_dev(*batch, d_result);
+    AggPartial* partial = noalloc_partials();
+    with_index_dev<uint32_t>(*index, [&](auto d) {
+      launch_join_agg(d, b, params->date_lo, params->date_hi, params->d_flag, index->cu, partial, S(stream),
+                      kNoAllocPartials);
     });
   });
 }

@@ -255,6 +255,18 @@ typedef struct gj_agg_result {
 int gj_join_aggregate(const gj_index* index, const gj_fact_batch* batch, const gj_agg_params* params,
                       gj_agg_result* d_result, gj_stream_t stream);
 
+/* gj_join_aggregate for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  Same arguments, semantics and results as
+ * gj_join_aggregate, but the call allocates nothing: the workgroup partial sums
+ * live in a static device array of 4096 slots (128 KiB per GPU, held until the
+ * process ends) instead of index-owned scratch.  A launch uses at most 8
+ * workgroups per CU, so devices with more than 512 CUs are rejected with
+ * GJ_ERR_UNSUPPORTED.  Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (on different streams) race and give wrong sums. */
+int gj_join_aggregate_noalloc(const gj_index* index, const gj_fact_batch* batch, const gj_agg_params* params,
+                              gj_agg_result* d_result, gj_stream_t stream);
+
 /* Final fixed-point value 100 * (num * 1000) / (den / 1000), i.e. percent x 1e6
  * (process_results, src/tpch/tpch_14_common.cu:124-127).  Returns 0 when den / 1000 == 0. */
 int64_t gj_promo_revenue(int64_t numerator, int64_t denominator);
