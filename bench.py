@@ -236,7 +236,7 @@ def main():
         if iparams is not None:
             entry["index_params"] = _index_params_str(iparams)
         for k in ("spline_points", "leaves", "fallback_leaves", "fallback_keys", "max_window", "avg_window",
-                  "train_loss"):
+                  "train_loss", "layout", "lines", "avg_lines", "max_lines"):
             if st[k]:
                 entry[k] = st[k]
         best_rate = 0.0
@@ -298,8 +298,11 @@ def main():
         log(f"cpu baseline: lower_bound {lb_rate / 1e6:.1f} M lookups/s, B-tree {bt_rate / 1e6:.1f} M lookups/s "
             f"on {cores} threads")
 
-    head_order = ["learned_mlp", "learned_plr", "radix_spline", "btree", "harmonia", "binary_search"]
-    head = next((h for h in head_order if h in results and results[h]["rowids_match"]), None)
+    # the headline is the faster learned index (both roots are learned models), else the next in order
+    learned = [h for h in ("learned_plr", "learned_mlp") if h in results and results[h]["rowids_match"]]
+    head_order = ["radix_spline", "btree", "harmonia", "binary_search"]
+    head = max(learned, key=lambda h: results[h]["lookups_per_s"]) if learned else \
+        next((h for h in head_order if h in results and results[h]["rowids_match"]), None)
     if head is None and results:
         head = next(iter(results))
     r = results.get(head, {})

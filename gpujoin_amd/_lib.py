@@ -36,8 +36,9 @@ BATCH_LAUNCH, BATCH_GRAPH, BATCH_PERSISTENT = range(3)
 # generators
 KEYS_DENSE, KEYS_UNIFORM, KEYS_LOGNORMAL, KEYS_CLUSTERED = range(4)
 PROBES_UNIFORM, PROBES_ZIPF = range(2)
-# gj_index_params.learned_flags
+# gj_index_params.learned_flags / learned_layout
 LEARNED_UNTRAINED = 0x1
+LAYOUT_AUTO, LAYOUT_COLUMN, LAYOUT_LINES = range(3)
 
 
 class IndexParams(C.Structure):
@@ -46,6 +47,7 @@ class IndexParams(C.Structure):
         ("btree_lds_bytes", C.c_uint32), ("learned_leaf_bits", C.c_uint32), ("learned_max_error", C.c_uint32),
         ("mlp_hidden", C.c_uint32), ("mlp_steps", C.c_uint32), ("mlp_batch", C.c_uint32), ("mlp_lr", C.c_float),
         ("seed", C.c_uint64), ("learned_root_segments", C.c_uint32), ("learned_flags", C.c_uint32),
+        ("learned_layout", C.c_uint32), ("learned_line_keys", C.c_uint32),
     ]
 
 
@@ -62,7 +64,8 @@ class IndexStats(C.Structure):
         ("depth", C.c_uint32), ("lds_levels", C.c_uint32), ("spline_points", C.c_uint64), ("leaves", C.c_uint64),
         ("fallback_leaves", C.c_uint64), ("fallback_keys", C.c_uint64), ("max_window", C.c_uint32),
         ("avg_window", C.c_double), ("train_loss", C.c_float), ("root_segments", C.c_uint32),
-        ("split_leaves", C.c_uint32), ("dropped_tuples", C.c_uint64),
+        ("split_leaves", C.c_uint32), ("dropped_tuples", C.c_uint64), ("layout", C.c_uint32),
+        ("max_lines", C.c_uint32), ("lines", C.c_uint64), ("avg_lines", C.c_double),
     ]
 
 

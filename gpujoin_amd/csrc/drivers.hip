@@ -100,7 +100,8 @@ void lookup_dispatch(const gj_index& ix, const void* probes, uint64_t m, uint32_
     const K* pk = static_cast<const K*>(probes);
     switch (p.approach) {
       case GJ_PLAIN:
-        if ((ix.kind == GJ_LEARNED_PLR || ix.kind == GJ_LEARNED_MLP) && ix.lm_fallback_any &&
+        if ((ix.kind == GJ_LEARNED_PLR || ix.kind == GJ_LEARNED_MLP) && ix.lm_layout == GJ_LAYOUT_COLUMN &&
+            ix.lm_fallback_any &&
             ix.stats.fallback_keys * kMarkFallbackRatio <= ix.n) {
           // few keys in leaves routed to the B-tree: the learned kernel marks their
           // probes, then one B-tree pass over the marks (the hot kernel stays free of

@@ -252,7 +252,7 @@ template void build_hash_table<uint64_t>(const gj_index&, dbuf&, uint64_t&, hipS
 uint64_t extended_memory(const gj_index& ix) {
   return ix.hm_keys.bytes + ix.hm_children.bytes + ix.rs_radix.bytes + ix.rs_knots.bytes +
          ix.lm_leaves.bytes + ix.lm_mlp.bytes + ix.lm_knots.bytes + ix.lm_seps.bytes + ix.lm_subs.bytes +
-         ix.lm_groups.bytes + ix.ht.bytes;
+         ix.lm_groups.bytes + ix.lm_lines.bytes + ix.lm_line_leaves.bytes + ix.ht.bytes;
 }
 
 template <class K>

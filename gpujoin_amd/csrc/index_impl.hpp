@@ -65,6 +65,11 @@ struct gj_index {
   gj::RootModel lm_root{};
   uint32_t lm_num_leaves = 0, lm_fallback_any = 0;
   int32_t lm_err_lo = 0, lm_err_hi = 0;  // learned window bounds over all leaves
+  // learned lines layout: the key lines and the two-stage leaves (none: root only)
+  uint32_t lm_layout = GJ_LAYOUT_COLUMN;
+  gj::dbuf lm_lines, lm_line_leaves;
+  uint64_t lm_num_lines = 0, lm_mult = 0;
+  uint32_t lm_num_line_leaves = 0;
 
   // hash table
   gj::dbuf ht;
@@ -167,6 +172,22 @@ struct gj_index {
     d.num_groups = static_cast<uint32_t>(lm_groups.bytes / sizeof(gj::LeafGroup));
     d.err_lo = lm_err_lo;
     d.err_hi = lm_err_hi;
+    return d;
+  }
+
+  template <class K>
+  gj::LinesDev<K> lines_dev() const {
+    gj::LinesDev<K> d{};
+    d.lines = lm_lines.as<const K>();
+    d.num_lines = lm_num_lines;
+    d.min_key = static_cast<K>(min_key);
+    d.max_key = static_cast<K>(max_key);
+    d.mult = lm_mult;
+    d.root = lm_root;
+    d.root.mlp = lm_mlp.as<const float>();
+    d.root.knots = lm_knots.as<const uint64_t>();
+    d.leaves = lm_line_leaves.as<const gj::LineLeaf>();
+    d.num_leaves = lm_num_line_leaves;
     return d;
   }
 

@@ -189,7 +189,11 @@ struct LineProbe {
   static constexpr int KPL = LK / kWinG;                         // keys per lane (32 bytes)
   K k[KPL];
   __device__ __forceinline__ void load(const K* col, uint64_t n, uint32_t line, int gl) {
-    const uint64_t start = static_cast<uint64_t>(line) * LK + static_cast<uint64_t>(gl * KPL);
+    load(col, n, 0u, gl, static_cast<uint64_t>(line));
+  }
+  // line given as 64 bits (the line copies of the lines layout can exceed 2^32 slots)
+  __device__ __forceinline__ void load(const K* col, uint64_t n, uint32_t, int gl, uint64_t line) {
+    const uint64_t start = line * LK + static_cast<uint64_t>(gl * KPL);
     if (start + KPL <= n) {
       const u32x4 a = *reinterpret_cast<const u32x4*>(col + start);
       const u32x4 b = *reinterpret_cast<const u32x4*>(col + start + KPL / 2);
@@ -647,6 +651,140 @@ struct LearnedCompactDev {
     }
     return group_window_search<K>(col, n, static_cast<uint32_t>(pred), static_cast<uint32_t>(lo),
                                   static_cast<uint32_t>(hi), x, live);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Learned index, lines layout (no reference counterpart).  The index keeps its
+// own copy of the keys in 128-byte lines.  The model assigns every key a target
+// line (monotone in the key); line t holds, in key order, the keys assigned to
+// it, after any keys an earlier full line pushed on to it, up to CAP keys; what
+// does not fit moves on to line t + 1.  So a key sits in its target line or,
+// only behind a run of full lines, a few lines later, and the layout stays in
+// global key order.  Each line starts with an 8-byte header {row id of its first
+// key, key count}; the row id of a key is header.base + its rank in the line.
+//
+// Lookup: read the target line; the key is there, or absent, unless every key of
+// the line is below the probe and the line is full, in which case the next line
+// is read.  A probe below its target line's keys is absent (every key assigned
+// to an earlier line is smaller, and no key moves backwards).
+//
+// Targets:
+//   root only   target = umulhi(key - min_key, mult): the keys' range spread
+//               linearly over the lines in fixed point (exact integer arithmetic)
+//   two-stage   a root model (linear, MLP or CDF spline, staged in LDS) picks a
+//               leaf; the leaf maps its key span linearly onto its own run of
+//               lines: target = line0 + floor(f32(key - key0) * scale).  Leaves
+//               own disjoint, ordered key ranges and line runs, so the target is
+//               monotone; the build evaluates the same device code.
+// ---------------------------------------------------------------------------
+struct LineLeaf {  // 16 bytes
+  uint64_t key0;   // first build key routed to the leaf (0 for an empty leaf)
+  uint32_t line0;  // first target line of the leaf
+  float scale;     // target lines per key unit over the leaf's key span
+};
+static_assert(sizeof(LineLeaf) == 16, "line leaf is 16 bytes");
+
+template <class K>
+struct LinesDev {
+  static constexpr uint32_t SLOTS = 128 / sizeof(K);  // K-sized slots per line
+  static constexpr uint32_t HDR = 8 / sizeof(K);      // slots of the {base, count} header
+  static constexpr uint32_t CAP = SLOTS - HDR;        // keys per line: 15 (u64), 30 (u32)
+  const K* lines;  // num_lines x SLOTS
+  uint64_t num_lines;
+  K min_key, max_key;
+  uint64_t mult;            // root only: target = umulhi(key - min_key, mult)
+  RootModel root;           // two-stage: routes to a leaf
+  const LineLeaf* leaves;   // nullptr: root only
+  uint32_t num_leaves;
+
+  __host__ __device__ size_t lds_bytes() const { return leaves ? root.bytes() : 0; }
+  __device__ void stage(uint8_t* lds) const {
+    if (leaves && root.bytes()) stage_to_lds(lds, root.params(), root.bytes());
+  }
+
+  // rp: the root's parameters (LDS during lookups, global memory at build)
+  __device__ __forceinline__ uint64_t target(K x, const uint8_t* rp) const {
+    const uint64_t off = static_cast<uint64_t>(x - min_key);
+    if (!leaves) return __umul64hi(off, mult);
+    const LineLeaf f = leaves[route_leaf<K>(x, rp, root, min_key, num_leaves)];
+    float p = leaf_dkey(x, f.key0) * f.scale;
+    p = fminf(fmaxf(p, 0.0f), 2.0e9f);
+    return static_cast<uint64_t>(f.line0) + static_cast<uint64_t>(floorf(p));
+  }
+
+  __device__ __forceinline__ uint32_t lookup(K x, bool active, const uint8_t* lds) const {
+    const bool live = active && x >= min_key && x <= max_key;
+    const uint64_t t = live ? target(x, lds) : 0;
+    return group_line_walk(t, x, live);
+  }
+
+  // Groups of kWinG lanes read whole lines (32 bytes per lane, lane 0 holding the
+  // header) and serve their probes in kWinG rounds; every open round requests its
+  // line before any line is examined, so a wave pays one memory latency per step.
+  __device__ __forceinline__ uint32_t group_line_walk(uint64_t my_line, K my_x, bool my_live) const {
+    using LP = LineProbe<K>;
+    constexpr int G = kWinG;
+    constexpr uint32_t KPL = LP::KPL;
+    static_assert(LP::LK == SLOTS, "a line probe covers one line");
+    const int ln = lane();
+    const int gbase = ln & ~(G - 1);
+    const int gl = ln & (G - 1);
+    const uint64_t all = num_lines * SLOTS;
+    bool open = my_live;
+    uint64_t line = my_line < num_lines ? my_line : num_lines - 1;
+    uint32_t found = kNotFound;
+    for (;;) {
+      const uint64_t open_mask = ballot(open);
+      if (open_mask == 0) break;  // wave-uniform exit
+      LP pc[G];
+#pragma unroll
+      for (int r = 0; r < G; ++r) {
+        const uint64_t l = shfl64(line, gbase + r);
+        if ((open_mask >> (gbase + r)) & 1ull) pc[r].load(lines, all, static_cast<uint32_t>(0), gl, l);
+      }
+      uint32_t below = 0, count = 0, base = 0;
+      bool eq = false;
+#pragma unroll
+      for (int r = 0; r < G; ++r) {
+        if (!((open_mask >> (gbase + r)) & 1ull)) continue;
+        uint64_t w0;
+        if constexpr (sizeof(K) == 8) {
+          w0 = static_cast<uint64_t>(pc[r].k[0]);
+        } else {
+          w0 = (static_cast<uint64_t>(pc[r].k[1]) << 32) | static_cast<uint64_t>(pc[r].k[0]);
+        }
+        const uint64_t hdr = shfl64(w0, gbase);  // group lane 0 holds the header
+        const uint32_t c = static_cast<uint32_t>(hdr >> 32);
+        const K xr = shfl(my_x, gbase + r);
+        uint32_t b = 0;
+        bool e = false;
+#pragma unroll
+        for (uint32_t j = 0; j < KPL; ++j) {
+          const uint32_t slot = static_cast<uint32_t>(gl) * KPL + j;
+          const bool valid = slot >= HDR && slot < HDR + c;
+          b += static_cast<uint32_t>(group_popc<G>(ballot(valid && pc[r].k[j] < xr), gbase));
+          e |= group_bits<G>(ballot(valid && pc[r].k[j] == xr), gbase) != 0;
+        }
+        if (gl == r) {
+          below = b;
+          eq = e;
+          count = c;
+          base = static_cast<uint32_t>(hdr);
+        }
+      }
+      if (open) {
+        if (eq) {
+          found = base + below;
+          open = false;
+        } else if (below == count && count == CAP && line + 1 < num_lines) {
+          ++line;  // a full line below the probe: keys pushed on from it sit in the next line
+        } else {
+          open = false;
+        }
+      }
+    }
+    return found;
   }
 };
 

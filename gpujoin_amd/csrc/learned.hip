@@ -458,6 +458,259 @@ __global__ void compact_error_kernel(const K* __restrict__ col, uint64_t n, Rout
   }
 }
 
+// ---- lines layout ------------------------------------------------------------
+
+// first[l] = first position whose target line is >= l (first[] pre-filled with n).
+// Counts keys whose target falls below their predecessor's or at or past
+// `targets` (neither may occur: the target is monotone and bounded).
+template <class K>
+__global__ void line_first_kernel(const K* __restrict__ col, uint64_t n, LinesDev<K> d, const uint8_t* rp,
+                                  uint64_t targets, uint32_t* __restrict__ first, unsigned long long* bad) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint64_t t = d.target(col[i], rp);
+    const int64_t tp = (i == 0) ? -1 : static_cast<int64_t>(d.target(col[i - 1], rp));
+    if (t >= targets || static_cast<int64_t>(t) < tp) {
+      atomicAdd(bad, 1ull);
+      continue;
+    }
+    for (int64_t l = tp + 1; l <= static_cast<int64_t>(t); ++l) first[l] = static_cast<uint32_t>(i);
+  }
+}
+
+// Line l holds the keys at positions [base[l], base[l + 1]), after its header.
+template <class K>
+__global__ void line_fill_kernel(const K* __restrict__ col, const uint32_t* __restrict__ base, uint64_t lines,
+                                 K* __restrict__ out) {
+  using D = LinesDev<K>;
+  const uint64_t slots = lines * D::SLOTS;
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t g = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; g < slots; g += stride) {
+    const uint64_t l = g / D::SLOTS;
+    const uint32_t sl = static_cast<uint32_t>(g % D::SLOTS);
+    const uint32_t b = base[l];
+    const uint32_t cnt = base[l + 1] - b;
+    K v;
+    if (sl < D::HDR) {
+      const uint64_t hdr = (static_cast<uint64_t>(cnt) << 32) | b;
+      v = static_cast<K>(sizeof(K) == 8 ? hdr : (sl == 0 ? (hdr & 0xFFFFFFFFull) : (hdr >> 32)));
+    } else {
+      const uint32_t k = sl - D::HDR;
+      v = k < cnt ? col[static_cast<uint64_t>(b) + k] : key_max<K>();
+    }
+    out[g] = v;
+  }
+}
+
+// Two-stage leaves: leaf l maps its keys [start[l], start[l + 1]) linearly onto its
+// lines [line0[l], line0[l + 1]).
+template <class K>
+__global__ void line_leaf_kernel(const K* __restrict__ col, const uint32_t* __restrict__ start,
+                                 const uint32_t* __restrict__ line0, uint32_t leaves, LineLeaf* __restrict__ out) {
+  const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= leaves) return;
+  const uint32_t a = start[l], b = start[l + 1];
+  LineLeaf f;
+  f.line0 = line0[l];
+  if (b > a) {
+    const uint64_t k0 = static_cast<uint64_t>(col[a]);
+    const uint64_t span = static_cast<uint64_t>(col[b - 1]) - k0;
+    const uint32_t nl = line0[l + 1] - line0[l];
+    f.key0 = k0;
+    f.scale = static_cast<float>(static_cast<double>(nl) / (static_cast<double>(span) + 1.0));
+  } else {
+    f.key0 = 0;
+    f.scale = 0.0f;
+  }
+  out[l] = f;
+}
+
+// A lines-layout build under one target model; the candidate holds its buffers
+// until it is adopted by the index or dropped.
+struct LinesBuild {
+  dbuf lines, leaves;
+  uint64_t num_lines = 0, mult = 0;
+  uint32_t num_leaves = 0, max_lines = 0;
+  double avg_lines = 0.0;
+  RootModel root{};
+  bool ok = false;
+};
+
+// Places the keys under target model d (d.leaves / d.mult already set; d.lines unused)
+// over `targets` target lines plus whatever tail lines the last full lines push keys into.
+template <class K>
+void place_lines(const gj_index& ix, LinesDev<K> d, const uint8_t* rp, uint64_t targets, LinesBuild& out,
+                 hipStream_t s) {
+  using D = LinesDev<K>;
+  const K* col = static_cast<const K*>(ix.d_col);
+  const uint64_t n = ix.n;
+  dbuf first((targets + 1) * sizeof(uint32_t));
+  {
+    std::vector<uint32_t> init(targets + 1, static_cast<uint32_t>(n));
+    GJ_HIP_CHECK(hipMemcpyAsync(first.p, init.data(), init.size() * 4, hipMemcpyHostToDevice, s));
+    dbuf bad(sizeof(unsigned long long));
+    fill_bytes(bad.p, 0, bad.bytes, s);
+    hipLaunchKernelGGL(line_first_kernel<K>, dim3(grid_for(n, 256)), dim3(256), 0, s, col, n, d, rp, targets,
+                       first.as<uint32_t>(), bad.as<unsigned long long>());
+    GJ_HIP_CHECK(hipGetLastError());
+    unsigned long long hbad = 0;
+    GJ_HIP_CHECK(hipMemcpyAsync(&hbad, bad.p, sizeof(hbad), hipMemcpyDeviceToHost, s));
+    GJ_HIP_CHECK(hipMemcpyAsync(init.data(), first.p, init.size() * 4, hipMemcpyDeviceToHost, s));
+    GJ_HIP_CHECK(hipStreamSynchronize(s));
+    GJ_REQUIRE(hbad == 0, GJ_ERR_INTERNAL, "learned lines: target model is not monotone");
+    // Greedy placement in key order: line l takes its carried-over keys and its own,
+    // up to CAP; the rest moves on.  base[l] = row id of line l's first key.
+    std::vector<uint32_t> base;
+    base.reserve(targets + 16);
+    uint64_t carry = 0, pos = 0;
+    double sum_line = 0.0, sum_target = 0.0;
+    for (uint64_t l = 0; l < targets; ++l) {
+      const uint64_t c = static_cast<uint64_t>(init[l + 1]) - init[l];
+      const uint64_t avail = carry + c;
+      const uint64_t placed = avail < D::CAP ? avail : D::CAP;
+      carry = avail - placed;
+      base.push_back(static_cast<uint32_t>(pos));
+      pos += placed;
+      sum_line += static_cast<double>(l) * static_cast<double>(placed);
+      sum_target += static_cast<double>(l) * static_cast<double>(c);
+    }
+    for (uint64_t l = targets; carry > 0; ++l) {  // tail lines for what the last full lines push on
+      const uint64_t placed = carry < D::CAP ? carry : D::CAP;
+      base.push_back(static_cast<uint32_t>(pos));
+      pos += placed;
+      carry -= placed;
+      sum_line += static_cast<double>(l) * static_cast<double>(placed);
+    }
+    GJ_REQUIRE(pos == n, GJ_ERR_INTERNAL, "learned lines: placement lost keys");
+    const uint64_t total = base.size();
+    base.push_back(static_cast<uint32_t>(n));
+    // largest spill: the first key of a line has the smallest target in it
+    uint64_t t = 0, max_spill = 0;
+    for (uint64_t l = 0; l < total; ++l) {
+      if (base[l + 1] == base[l]) continue;
+      while (t + 1 < targets && init[t + 1] <= base[l]) ++t;
+      max_spill = std::max<uint64_t>(max_spill, l - t);
+    }
+    out.num_lines = total;
+    out.avg_lines = 1.0 + (sum_line - sum_target) / static_cast<double>(n);
+    out.max_lines = static_cast<uint32_t>(std::min<uint64_t>(max_spill + 1, 0xFFFFFFFFull));
+    first.release();
+    dbuf d_base(base.size() * sizeof(uint32_t));
+    GJ_HIP_CHECK(hipMemcpyAsync(d_base.p, base.data(), base.size() * 4, hipMemcpyHostToDevice, s));
+    out.lines.alloc(total * 128);
+    hipLaunchKernelGGL(line_fill_kernel<K>, dim3(grid_for(total * D::SLOTS, 256)), dim3(256), 0, s, col,
+                       d_base.as<const uint32_t>(), total, out.lines.as<K>());
+    GJ_HIP_CHECK(hipGetLastError());
+    GJ_HIP_CHECK(hipStreamSynchronize(s));
+  }
+  out.ok = true;
+}
+
+// Root-only lines: the key range spread linearly over ceil(n / line_keys) lines.
+template <class K>
+void lines_root_only(const gj_index& ix, uint32_t line_keys, LinesBuild& out, hipStream_t s) {
+  const uint64_t targets = std::max<uint64_t>(1, ceil_div(ix.n, line_keys));
+  const unsigned __int128 den = static_cast<unsigned __int128>(ix.max_key - ix.min_key) + 1;
+  // mult = floor(targets * 2^64 / (range + 1)), so umulhi(range, mult) < targets; capped at
+  // 2^64 - 1 for a range narrower than the line count (then target <= key - min_key)
+  const unsigned __int128 q = (static_cast<unsigned __int128>(targets) << 64) / den;
+  out.mult = q > static_cast<unsigned __int128>(~0ull) ? ~0ull : static_cast<uint64_t>(q);
+  LinesDev<K> d{};
+  d.min_key = static_cast<K>(ix.min_key);
+  d.max_key = static_cast<K>(ix.max_key);
+  d.mult = out.mult;
+  out.num_leaves = 0;
+  out.root = RootModel{};
+  place_lines<K>(ix, d, nullptr, targets, out, s);
+}
+
+// Two-stage lines: `root` routes keys to 2^bits leaves; each leaf gets lines for its
+// keys at line_keys per line and maps its key span linearly onto them.
+template <class K>
+void lines_two_stage(const gj_index& ix, const RootModel& root, uint32_t bits, uint32_t line_keys, LinesBuild& out,
+                     hipStream_t s) {
+  const K* col = static_cast<const K*>(ix.d_col);
+  const uint64_t n = ix.n;
+  const uint32_t L = 1u << bits;
+  const K kmin = static_cast<K>(ix.min_key);
+  Router<K> route{kmin, root, L};
+  dbuf start((static_cast<uint64_t>(L) + 1) * sizeof(uint32_t));
+  std::vector<uint32_t> hs(L + 1, static_cast<uint32_t>(n));
+  {
+    GJ_HIP_CHECK(hipMemcpyAsync(start.p, hs.data(), hs.size() * 4, hipMemcpyHostToDevice, s));
+    dbuf inv(sizeof(unsigned long long));
+    fill_bytes(inv.p, 0, inv.bytes, s);
+    hipLaunchKernelGGL(leaf_bounds_kernel<K>, dim3(grid_for(n, 256)), dim3(256), 0, s, col, n, route,
+                       start.as<uint32_t>(), inv.as<unsigned long long>());
+    GJ_HIP_CHECK(hipGetLastError());
+    unsigned long long hinv = 0;
+    GJ_HIP_CHECK(hipMemcpyAsync(&hinv, inv.p, sizeof(hinv), hipMemcpyDeviceToHost, s));
+    GJ_HIP_CHECK(hipMemcpyAsync(hs.data(), start.p, hs.size() * 4, hipMemcpyDeviceToHost, s));
+    GJ_HIP_CHECK(hipStreamSynchronize(s));
+    GJ_REQUIRE(hinv == 0, GJ_ERR_INTERNAL, "learned lines: root model is not monotone");
+  }
+  std::vector<uint32_t> line0(L + 1, 0);
+  uint64_t lines = 0;
+  for (uint32_t l = 0; l < L; ++l) {
+    line0[l] = static_cast<uint32_t>(lines);
+    const uint64_t c = static_cast<uint64_t>(hs[l + 1]) - hs[l];
+    lines += ceil_div(c, line_keys);
+  }
+  GJ_REQUIRE(lines < 0xFFFFFFFFull, GJ_ERR_OVERFLOW, "learned lines: too many lines");
+  line0[L] = static_cast<uint32_t>(lines);
+  dbuf d_line0(line0.size() * sizeof(uint32_t));
+  GJ_HIP_CHECK(hipMemcpyAsync(d_line0.p, line0.data(), line0.size() * 4, hipMemcpyHostToDevice, s));
+  out.leaves.alloc(sizeof(LineLeaf) * L);
+  hipLaunchKernelGGL(line_leaf_kernel<K>, dim3(grid_for(L, 256)), dim3(256), 0, s, col, start.as<const uint32_t>(),
+                     d_line0.as<const uint32_t>(), L, out.leaves.as<LineLeaf>());
+  GJ_HIP_CHECK(hipGetLastError());
+  out.num_leaves = L;
+  out.root = root;
+  LinesDev<K> d{};
+  d.min_key = kmin;
+  d.max_key = static_cast<K>(ix.max_key);
+  d.root = root;
+  d.leaves = out.leaves.as<const LineLeaf>();
+  d.num_leaves = L;
+  // FP32 rounding can carry a leaf's last keys to floor(p) = its line count, the first line
+  // of the next leaf (still monotone); for the last leaf that is target line `lines`
+  place_lines<K>(ix, d, reinterpret_cast<const uint8_t*>(root.params()), lines + 1, out, s);
+}
+
+// Adopts a lines build into the index: the column layout's leaves and the B-tree
+// fallback are not needed.
+void adopt_lines(gj_index& ix, LinesBuild& b) {
+  ix.lm_layout = GJ_LAYOUT_LINES;
+  ix.lm_lines = std::move(b.lines);
+  ix.lm_line_leaves = std::move(b.leaves);
+  ix.lm_num_lines = b.num_lines;
+  ix.lm_mult = b.mult;
+  ix.lm_num_line_leaves = b.num_leaves;
+  ix.lm_root = b.root;
+  ix.lm_leaves.release();
+  ix.lm_groups.release();
+  ix.lm_seps.release();
+  ix.lm_subs.release();
+  ix.lm_compact = false;
+  ix.lm_splits = 0;
+  ix.lm_fallback_any = 0;
+  ix.btree_nodes.release();
+  ix.bt = BTreeLayout{};
+  ix.stats.layout = GJ_LAYOUT_LINES;
+  ix.stats.lines = b.num_lines;
+  ix.stats.avg_lines = b.avg_lines;
+  ix.stats.max_lines = b.max_lines;
+  ix.stats.leaves = b.num_leaves;
+  ix.stats.root_segments = b.root.segments;
+  ix.stats.fallback_leaves = 0;
+  ix.stats.fallback_keys = 0;
+  ix.stats.split_leaves = 0;
+  ix.stats.max_window = 0;
+  ix.stats.avg_window = 0.0;
+  ix.stats.depth = b.num_leaves ? 2 : 1;
+  ix.stats.lds_levels = 0;
+}
+
 }  // namespace
 
 template <class K>
@@ -605,6 +858,58 @@ void build_learned(gj_index& ix, hipStream_t s) {
                "learned_root_segments must be 0, 1 or a power of two <= 4096");
     root = spline_root(want);
   }
+
+  // Lines layout: the model places the keys itself, so a present key is read from the
+  // line the model names (or, behind a run of full lines, the next ones).  Candidates,
+  // cheapest first: the linear root alone (no table to read), then the root routing to
+  // leaves that each spread their keys over their own lines, then (learned_plr) a
+  // spline root of the column's CDF routing to leaves.  The one with the fewest lines
+  // per present key wins; the column layout takes over in auto mode when even that
+  // reads more than 1.25 lines per key, or when the line copy does not fit in memory.
+  const uint32_t layout = ix.params.learned_layout;
+  GJ_REQUIRE(layout <= GJ_LAYOUT_LINES, GJ_ERR_INVALID, "learned_layout must be 0, 1 or 2");
+  if (layout != GJ_LAYOUT_COLUMN) {
+    using D = LinesDev<K>;
+    const uint32_t lk = ix.params.learned_line_keys ? ix.params.learned_line_keys : (sizeof(K) == 8 ? 10u : 20u);
+    GJ_REQUIRE(lk >= 1 && lk <= D::CAP, GJ_ERR_INVALID, "learned_line_keys must be in [1, keys per line]");
+    const uint32_t lbits = ix.params.learned_leaf_bits
+                               ? ix.params.learned_leaf_bits
+                               : static_cast<uint32_t>(std::clamp(static_cast<int>(ilog2_ceil(n)) - 10, 8, 20));
+    LinesBuild best;
+    auto consider = [&](auto&& make) {
+      LinesBuild c;
+      try {
+        make(c);
+      } catch (const error& e) {
+        if (e.status != GJ_ERR_NOMEM || layout == GJ_LAYOUT_LINES) throw;
+        (void)hipGetLastError();
+        return;  // auto: the line copy does not fit; try the next candidate or the column layout
+      }
+      if (!best.ok || c.avg_lines < best.avg_lines) best = std::move(c);
+    };
+    constexpr double kGoodLines = 1.05;  // below this, a costlier candidate cannot gain much
+    const bool linear_root = root.hidden == 0 && root.segments == 0;
+    if (linear_root && ix.params.learned_leaf_bits == 0)
+      consider([&](LinesBuild& c) { lines_root_only<K>(ix, lk, c, s); });
+    if (!best.ok || best.avg_lines > kGoodLines)
+      consider([&](LinesBuild& c) { lines_two_stage<K>(ix, root, lbits, lk, c, s); });
+    if (ix.kind == GJ_LEARNED_PLR && linear_root && ix.params.learned_root_segments == 0 &&
+        (!best.ok || best.avg_lines > kGoodLines) && n >= 4)
+      consider([&](LinesBuild& c) { lines_two_stage<K>(ix, spline_root(2048), lbits, lk, c, s); });
+    if (best.ok && (layout == GJ_LAYOUT_LINES || best.avg_lines <= 1.25)) {
+      if (best.root.segments == 0) {
+        ix.lm_knots.release();
+        ix.stats.spline_points = 0;
+      }
+      adopt_lines(ix, best);
+      return;
+    }
+    GJ_REQUIRE(layout != GJ_LAYOUT_LINES, GJ_ERR_NOMEM, "learned lines: the line copy does not fit");
+    ix.lm_knots.release();
+    ix.stats.spline_points = 0;
+  }
+  ix.lm_layout = GJ_LAYOUT_COLUMN;
+  ix.stats.layout = GJ_LAYOUT_COLUMN;
 
   dbuf last_start, last_fit;  // leaf starts and fits of the last build_leaves
   // routing, leaf fits and calibration under a root; returns the build keys routed to the B-tree

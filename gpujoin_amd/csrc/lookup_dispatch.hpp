@@ -112,7 +112,9 @@ void with_index_dev(const gj_index& ix, F&& f) {
     case GJ_RADIX_SPLINE: f(ix.rs_dev<K>()); return;
     case GJ_LEARNED_PLR:
     case GJ_LEARNED_MLP:
-      if (ix.lm_compact)
+      if (ix.lm_layout == GJ_LAYOUT_LINES)
+        f(ix.lines_dev<K>());
+      else if (ix.lm_compact)
         f(ix.learned_compact_dev<K>());
       else if (ix.lm_fallback_any)
         f(ix.learned_dev<K, kLeafInline>());

@@ -95,7 +95,26 @@ typedef struct gj_index_params {
                                      segments when the linear root routes keys to the B-tree), 1 = linear,
                                      otherwise the most spline segments (a power of two, <= 4096) */
   uint32_t learned_flags;         /* GJ_LEARNED_* bits, default 0 */
+  uint32_t learned_layout;        /* GJ_LAYOUT_AUTO (default), GJ_LAYOUT_COLUMN or GJ_LAYOUT_LINES */
+  uint32_t learned_line_keys;     /* lines layout: build keys per 128-byte line on average; default 10 of the
+                                     15 a line holds for 8-byte keys, 20 of 30 for 4-byte keys */
 } gj_index_params;
+
+/* Layouts of the learned kinds.
+ *   GJ_LAYOUT_COLUMN  the model predicts a position in the borrowed column and a bounded window
+ *                     search closes it (leaves too wide for the bound fall back to the B-tree)
+ *   GJ_LAYOUT_LINES   the index keeps its own copy of the keys in 128-byte lines, each key placed
+ *                     in the line the model assigns it (a line holds an 8-byte header {first row id,
+ *                     key count} and up to 15 8-byte or 30 4-byte keys, in key order; a full line
+ *                     spills its surplus into the next line).  A lookup reads the assigned line and,
+ *                     only for the keys a full line pushed on, the lines after it, at most the
+ *                     build's measured max_lines.  About 12.8 bytes per 8-byte key at the default
+ *                     load, against the column's 8.
+ *   GJ_LAYOUT_AUTO    lines when the build's mean lines per present key stays below 1.25 and the
+ *                     line copy fits, else column */
+#define GJ_LAYOUT_AUTO 0u
+#define GJ_LAYOUT_COLUMN 1u
+#define GJ_LAYOUT_LINES 2u
 
 /* learned_flags: keep the MLP root at its initialisation (the piecewise-linear CDF through
  * mlp_hidden + 1 quantiles of the column) and run no training step; the root's loss is
@@ -171,6 +190,10 @@ typedef struct gj_index_stats {
   uint32_t split_leaves;      /* learned: leaves too wide for one line, split into 16 sub-leaves */
   uint64_t dropped_tuples;    /* probes the partitioned driver's scatter could not place (0 unless an
                                  internal fault occurred; their row ids were not written) */
+  uint32_t layout;            /* learned: GJ_LAYOUT_COLUMN or GJ_LAYOUT_LINES */
+  uint32_t max_lines;         /* lines layout: most lines a present key's lookup reads (1 + the largest spill) */
+  uint64_t lines;             /* lines layout: 128-byte lines held */
+  double avg_lines;           /* lines layout: mean lines a present key's lookup reads */
 } gj_index_stats;
 
 /* Fills *out.  dropped_tuples is read from the device: the call waits for the

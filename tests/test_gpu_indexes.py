@@ -105,7 +105,7 @@ def test_learned_matches_oracle(gj, orc, gpu, kind, dist, kb):
     keys = gj.gen_build_keys(n, kb, dist, 0, 500 + dist)
     probes = _probes(gj, keys, 40000, 600 + dist)
     want = orc.lookup_lower_bound(keys, probes)
-    params = gj.index_params(learned_leaf_bits=10, learned_max_error=16, mlp_steps=150)
+    params = gj.index_params(learned_layout=_lib.LAYOUT_COLUMN, learned_leaf_bits=10, learned_max_error=16, mlp_steps=150)
     got, st = _lookup(gj, kind, keys, probes, params=params)
     assert np.array_equal(got, want)
     assert st["leaves"] == 1 << 10
@@ -117,7 +117,7 @@ def test_learned_fallback_is_exact(gj, orc, gpu):
     keys = gj.gen_build_keys(1 << 17, 8, _lib.KEYS_LOGNORMAL, 0, 77)
     probes = _probes(gj, keys, 30000, 78)
     got, st = _lookup(gj, "learned_plr", keys, probes,
-                      params=gj.index_params(learned_leaf_bits=6, learned_max_error=1))
+                      params=gj.index_params(learned_layout=_lib.LAYOUT_COLUMN, learned_leaf_bits=6, learned_max_error=1))
     assert st["fallback_leaves"] > 0
     assert np.array_equal(got, orc.lookup_lower_bound(keys, probes))
 
@@ -132,7 +132,8 @@ def test_learned_few_fallback_leaves_exact(gj, orc, gpu, kind):
     run = np.arange(2 ** 61, 2 ** 61 + 600, dtype=np.uint64)
     keys = np.unique(np.concatenate([base, run]))
     probes = np.concatenate([_probes(gj, keys, 200000, 92), run, run + 600])
-    params = gj.index_params(learned_leaf_bits=14, learned_max_error=16, learned_root_segments=1, mlp_steps=100)
+    params = gj.index_params(learned_layout=_lib.LAYOUT_COLUMN, learned_leaf_bits=14, learned_max_error=16, learned_root_segments=1,
+                             mlp_steps=100)
     got, st = _lookup(gj, kind, keys, probes, params=params)
     print(f"[case] {kind} fallback_keys={st['fallback_keys']} of {keys.size}", flush=True)
     assert 0 < st["fallback_keys"] and st["fallback_keys"] * 400 <= keys.size
@@ -147,7 +148,8 @@ def test_split_leaves_exact(gj, orc, gpu, kind, kb):
     # sub-leaves that are still too wide go to the B-tree; every answer stays exact
     keys = gj.gen_build_keys(1 << 18, kb, _lib.KEYS_CLUSTERED, 0, 0x51 + kb)
     probes = _probes(gj, keys, 60000, 0x52 + kb)
-    params = gj.index_params(learned_leaf_bits=12, learned_max_error=32, learned_root_segments=1, mlp_steps=60)
+    params = gj.index_params(learned_layout=_lib.LAYOUT_COLUMN, learned_leaf_bits=12, learned_max_error=32, learned_root_segments=1,
+                             mlp_steps=60)
     got, st = _lookup(gj, kind, keys, probes, params=params)
     print(f"[case] {kind} kb={kb} split_leaves={st['split_leaves']} fallback_keys={st['fallback_keys']}", flush=True)
     assert st["split_leaves"] > 0
@@ -163,9 +165,9 @@ def test_mlp_root_beats_linear_root_on_skewed_keys(gj, gpu):
     # keys into a few leaves; the trained monotone MLP root spreads them, so far fewer
     # keys need the B-tree fallback and the mean last-mile window shrinks
     keys = gj.gen_build_keys(1 << 20, 8, _lib.KEYS_LOGNORMAL, 63, 0x31)
-    params = gj.index_params(learned_leaf_bits=12, learned_max_error=32)
+    params = gj.index_params(learned_layout=_lib.LAYOUT_COLUMN, learned_leaf_bits=12, learned_max_error=32)
     col = gj.DeviceArray.from_numpy(keys)
-    linear = gj.index_params(learned_leaf_bits=12, learned_max_error=32, learned_root_segments=1)
+    linear = gj.index_params(learned_layout=_lib.LAYOUT_COLUMN, learned_leaf_bits=12, learned_max_error=32, learned_root_segments=1)
     plr = gj.Index("learned_plr", col, host_column=keys, params=linear).stats()
     assert plr["root_segments"] == 0
     mlp = gj.Index("learned_mlp", col, host_column=keys, params=params).stats()
@@ -183,7 +185,8 @@ def test_learned_spline_root_matches_oracle(gj, orc, gpu, segments, dist, kb):
     print(f"[case] spline root segments={segments} dist={dist} kb={kb}", flush=True)
     keys = gj.gen_build_keys(n, kb, dist, 0, 700 + dist)
     probes = _probes(gj, keys, 40000, 800 + dist)
-    params = gj.index_params(learned_leaf_bits=10, learned_max_error=16, learned_root_segments=segments)
+    params = gj.index_params(learned_layout=_lib.LAYOUT_COLUMN, learned_leaf_bits=10, learned_max_error=16,
+                             learned_root_segments=segments)
     got, st = _lookup(gj, "learned_plr", keys, probes, params=params)
     assert np.array_equal(got, orc.lookup_lower_bound(keys, probes))
     if segments >= 2:
@@ -195,7 +198,7 @@ def test_learned_spline_root_tiny_columns(gj, orc, gpu):
     for n in (1, 2, 3, 5, 64, 100):
         keys = np.unique(np.random.default_rng(n).integers(1, 2 ** 40, size=n, dtype=np.uint64))
         probes = np.concatenate([keys, keys + 1, keys - 1])
-        params = gj.index_params(learned_leaf_bits=4, learned_root_segments=4096)
+        params = gj.index_params(learned_layout=_lib.LAYOUT_COLUMN, learned_leaf_bits=4, learned_root_segments=4096)
         got, st = _lookup(gj, "learned_plr", keys, probes, params=params)
         assert np.array_equal(got, orc.lookup_lower_bound(keys, probes)), n
         assert st["root_segments"] <= max(2, keys.size)
@@ -208,9 +211,10 @@ def test_spline_root_spreads_clustered_keys(gj, gpu):
     keys = gj.gen_build_keys(1 << 20, 8, _lib.KEYS_CLUSTERED, 63, 0x33)
     col = gj.DeviceArray.from_numpy(keys)
     lin = gj.Index("learned_plr", col, host_column=keys,
-                   params=gj.index_params(learned_leaf_bits=12, learned_max_error=32, learned_root_segments=1)).stats()
+                   params=gj.index_params(learned_layout=_lib.LAYOUT_COLUMN, learned_leaf_bits=12, learned_max_error=32,
+                                         learned_root_segments=1)).stats()
     auto = gj.Index("learned_plr", col, host_column=keys,
-                    params=gj.index_params(learned_leaf_bits=12, learned_max_error=32)).stats()
+                    params=gj.index_params(learned_layout=_lib.LAYOUT_COLUMN, learned_leaf_bits=12, learned_max_error=32)).stats()
     print(f"[gain] linear root fallback_keys={lin['fallback_keys']}; auto root segments={auto['root_segments']} "
           f"fallback_keys={auto['fallback_keys']} avg_window={auto['avg_window']:.1f}", flush=True)
     assert lin["root_segments"] == 0 and lin["fallback_keys"] > 0 and lin["split_leaves"] > 0
@@ -266,12 +270,13 @@ def test_compact_leaves_exact(gj, orc, gpu, kind, kb):
     keys = np.unique(rng.integers(0, hi, size=(1 << 22) + 40000, dtype=dt))[: (1 << 22) + 12345]
     probes = np.concatenate([rng.choice(keys, 300000), rng.integers(0, hi, size=50000, dtype=dt),
                              keys[:100], keys[-100:]]).astype(dt)
-    got, st = _lookup(gj, kind, keys, probes, params=gj.index_params(mlp_steps=100))
+    got, st = _lookup(gj, kind, keys, probes, params=gj.index_params(learned_layout=_lib.LAYOUT_COLUMN, mlp_steps=100))
     print(f"[case] {kind} kb={kb} leaves={st['leaves']} avg_window={st['avg_window']:.1f}", flush=True)
     assert st["leaves"] == 1 << 17 and st["fallback_leaves"] == 0
     assert np.array_equal(got, orc.lookup_lower_bound(keys, probes))
     # an explicit leaf count keeps the 16-byte leaves and the same answers
-    got16, st16 = _lookup(gj, kind, keys, probes, params=gj.index_params(learned_leaf_bits=16, mlp_steps=100))
+    got16, st16 = _lookup(gj, kind, keys, probes,
+                          params=gj.index_params(learned_layout=_lib.LAYOUT_COLUMN, learned_leaf_bits=16, mlp_steps=100))
     assert st16["leaves"] == 1 << 16
     assert np.array_equal(got16, got)
 
@@ -285,7 +290,7 @@ def test_compact_leaves_every_driver(gj, orc, gpu, kind):
     probes = np.concatenate([rng.choice(keys, 400000), rng.integers(0, 2 ** 62, size=40000, dtype=np.uint64)])
     want = orc.lookup_lower_bound(keys, probes)
     col = gj.DeviceArray.from_numpy(keys)
-    idx = gj.Index(kind, col, host_column=keys, params=gj.index_params(mlp_steps=100))
+    idx = gj.Index(kind, col, host_column=keys, params=gj.index_params(learned_layout=_lib.LAYOUT_COLUMN, mlp_steps=100))
     assert idx.stats()["leaves"] == 1 << 17
     dp = gj.DeviceArray.from_numpy(probes)
     out = gj.DeviceArray(probes.size, np.uint32)
