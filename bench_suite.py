@@ -406,9 +406,11 @@ def extras(gj, G, stream, keys64, d_keys64, d_out, cpu=True, steps=5, warmup=2):
                            "rowids_match": ok, "build_s": build_s, "fallback_keys": st["fallback_keys"],
                            "fallback_leaves": st["fallback_leaves"], "avg_window": st["avg_window"],
                            "max_window": st["max_window"], "train_loss": st["train_loss"],
-                           "index_bytes": st["memory_bytes"]}
+                           "index_bytes": st["memory_bytes"], "layout": st["layout"], "leaves": st["leaves"],
+                           "avg_lines": st["avg_lines"], "max_lines": st["max_lines"]}
             idx.close()
-            log(f"C4 {dname} {label}: {m / ms * 1e-6:.2f} G lookups/s match={ok} fallback={st['fallback_keys']}")
+            log(f"C4 {dname} {label}: {m / ms * 1e-6:.2f} G lookups/s match={ok} fallback={st['fallback_keys']} "
+                f"layout={st['layout']} leaves={st['leaves']} avg_lines={st['avg_lines']:.3f}")
         bt = rows["btree"]["lookups_per_s"]
         for label in ("learned_plr", "learned_mlp"):
             rows[label]["vs_btree"] = rows[label]["lookups_per_s"] / bt

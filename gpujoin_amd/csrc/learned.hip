@@ -538,9 +538,10 @@ struct LinesBuild {
 
 // Places the keys under target model d (d.leaves / d.mult already set; d.lines unused)
 // over `targets` target lines plus whatever tail lines the last full lines push keys into.
+// src_line (optional) receives, per line, the smallest target line among its keys.
 template <class K>
 void place_lines(const gj_index& ix, LinesDev<K> d, const uint8_t* rp, uint64_t targets, LinesBuild& out,
-                 hipStream_t s) {
+                 hipStream_t s, std::vector<uint32_t>* src_line = nullptr) {
   using D = LinesDev<K>;
   const K* col = static_cast<const K*>(ix.d_col);
   const uint64_t n = ix.n;
@@ -586,10 +587,15 @@ void place_lines(const gj_index& ix, LinesDev<K> d, const uint8_t* rp, uint64_t 
     base.push_back(static_cast<uint32_t>(n));
     // largest spill: the first key of a line has the smallest target in it
     uint64_t t = 0, max_spill = 0;
+    if (src_line) src_line->assign(total, 0);
     for (uint64_t l = 0; l < total; ++l) {
-      if (base[l + 1] == base[l]) continue;
+      if (base[l + 1] == base[l]) {
+        if (src_line) (*src_line)[l] = static_cast<uint32_t>(l);
+        continue;
+      }
       while (t + 1 < targets && init[t + 1] <= base[l]) ++t;
       max_spill = std::max<uint64_t>(max_spill, l - t);
+      if (src_line) (*src_line)[l] = static_cast<uint32_t>(t);
     }
     out.num_lines = total;
     out.avg_lines = 1.0 + (sum_line - sum_target) / static_cast<double>(n);
@@ -649,32 +655,58 @@ void lines_two_stage(const gj_index& ix, const RootModel& root, uint32_t bits, u
     GJ_HIP_CHECK(hipStreamSynchronize(s));
     GJ_REQUIRE(hinv == 0, GJ_ERR_INTERNAL, "learned lines: root model is not monotone");
   }
-  std::vector<uint32_t> line0(L + 1, 0);
-  uint64_t lines = 0;
-  for (uint32_t l = 0; l < L; ++l) {
-    line0[l] = static_cast<uint32_t>(lines);
-    const uint64_t c = static_cast<uint64_t>(hs[l + 1]) - hs[l];
-    lines += ceil_div(c, line_keys);
-  }
-  GJ_REQUIRE(lines < 0xFFFFFFFFull, GJ_ERR_OVERFLOW, "learned lines: too many lines");
-  line0[L] = static_cast<uint32_t>(lines);
-  dbuf d_line0(line0.size() * sizeof(uint32_t));
-  GJ_HIP_CHECK(hipMemcpyAsync(d_line0.p, line0.data(), line0.size() * 4, hipMemcpyHostToDevice, s));
+  // A leaf whose keys are not evenly spread over its key span overfills some of its lines,
+  // and the surplus runs on through the following lines; a lookup wave walks as far as its
+  // farthest probe.  So the leaves that source a run of two or more lines get twice their
+  // lines (half the load where their keys crowd), and the placement is redone, up to
+  // kInflateRounds times.
+  constexpr int kInflateRounds = 4;
+  constexpr uint32_t kMaxInflate = 16;
+  std::vector<uint32_t> inflate(L, 1), line0(L + 1, 0), src;
+  dbuf d_line0((static_cast<uint64_t>(L) + 1) * sizeof(uint32_t));
   out.leaves.alloc(sizeof(LineLeaf) * L);
-  hipLaunchKernelGGL(line_leaf_kernel<K>, dim3(grid_for(L, 256)), dim3(256), 0, s, col, start.as<const uint32_t>(),
-                     d_line0.as<const uint32_t>(), L, out.leaves.as<LineLeaf>());
-  GJ_HIP_CHECK(hipGetLastError());
   out.num_leaves = L;
   out.root = root;
-  LinesDev<K> d{};
-  d.min_key = kmin;
-  d.max_key = static_cast<K>(ix.max_key);
-  d.root = root;
-  d.leaves = out.leaves.as<const LineLeaf>();
-  d.num_leaves = L;
-  // FP32 rounding can carry a leaf's last keys to floor(p) = its line count, the first line
-  // of the next leaf (still monotone); for the last leaf that is target line `lines`
-  place_lines<K>(ix, d, reinterpret_cast<const uint8_t*>(root.params()), lines + 1, out, s);
+  for (int round = 0;; ++round) {
+    uint64_t lines = 0;
+    for (uint32_t l = 0; l < L; ++l) {
+      line0[l] = static_cast<uint32_t>(lines);
+      const uint64_t c = static_cast<uint64_t>(hs[l + 1]) - hs[l];
+      lines += ceil_div(c * inflate[l], line_keys);
+    }
+    GJ_REQUIRE(lines < 0xFFFFFFFFull, GJ_ERR_OVERFLOW, "learned lines: too many lines");
+    line0[L] = static_cast<uint32_t>(lines);
+    GJ_HIP_CHECK(hipMemcpyAsync(d_line0.p, line0.data(), line0.size() * 4, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(line_leaf_kernel<K>, dim3(grid_for(L, 256)), dim3(256), 0, s, col,
+                       start.as<const uint32_t>(), d_line0.as<const uint32_t>(), L, out.leaves.as<LineLeaf>());
+    GJ_HIP_CHECK(hipGetLastError());
+    LinesDev<K> d{};
+    d.min_key = kmin;
+    d.max_key = static_cast<K>(ix.max_key);
+    d.root = root;
+    d.leaves = out.leaves.as<const LineLeaf>();
+    d.num_leaves = L;
+    // FP32 rounding can carry a leaf's last keys to floor(p) = its line count, the first
+    // line of the next leaf (still monotone); for the last leaf that is target line `lines`
+    place_lines<K>(ix, d, reinterpret_cast<const uint8_t*>(root.params()), lines + 1, out, s, &src);
+    if (out.max_lines <= 2 || round + 1 >= kInflateRounds) break;
+    std::vector<uint8_t> mark(L, 0);
+    for (uint64_t l = 0; l < src.size(); ++l) {
+      if (l - src[l] < 2) continue;
+      // the leaf whose lines hold the source line
+      const uint32_t j = static_cast<uint32_t>(
+          std::upper_bound(line0.begin(), line0.end(), src[l]) - line0.begin() - 1);
+      if (j < L) mark[j] = 1;
+    }
+    bool grew = false;
+    for (uint32_t j = 0; j < L; ++j) {
+      if (mark[j] && inflate[j] < kMaxInflate) {
+        inflate[j] *= 2;
+        grew = true;
+      }
+    }
+    if (!grew) break;
+  }
 }
 
 // Adopts a lines build into the index: the column layout's leaves and the B-tree
@@ -865,10 +897,14 @@ void build_learned(gj_index& ix, hipStream_t s) {
   // leaves that each spread their keys over their own lines, then (learned_plr) a
   // spline root of the column's CDF routing to leaves.  The one with the fewest lines
   // per present key wins; the column layout takes over in auto mode when even that
-  // reads more than 1.25 lines per key, or when the line copy does not fit in memory.
+  // reads more than 1.10 lines per key or up to more than 4 lines, or when the line
+  // copy does not fit in memory.
+  // A column below 16 MiB is served from L2 and the MALL, where the column's denser lines
+  // (16 or 32 keys against 10 or 20) win; auto keeps it.
   const uint32_t layout = ix.params.learned_layout;
   GJ_REQUIRE(layout <= GJ_LAYOUT_LINES, GJ_ERR_INVALID, "learned_layout must be 0, 1 or 2");
-  if (layout != GJ_LAYOUT_COLUMN) {
+  constexpr uint64_t kLinesMinColumnBytes = 16ull << 20;
+  if (layout == GJ_LAYOUT_LINES || (layout == GJ_LAYOUT_AUTO && n * sizeof(K) >= kLinesMinColumnBytes)) {
     using D = LinesDev<K>;
     const uint32_t lk = ix.params.learned_line_keys ? ix.params.learned_line_keys : (sizeof(K) == 8 ? 10u : 20u);
     GJ_REQUIRE(lk >= 1 && lk <= D::CAP, GJ_ERR_INVALID, "learned_line_keys must be in [1, keys per line]");
@@ -896,7 +932,9 @@ void build_learned(gj_index& ix, hipStream_t s) {
     if (ix.kind == GJ_LEARNED_PLR && linear_root && ix.params.learned_root_segments == 0 &&
         (!best.ok || best.avg_lines > kGoodLines) && n >= 4)
       consider([&](LinesBuild& c) { lines_two_stage<K>(ix, spline_root(2048), lbits, lk, c, s); });
-    if (best.ok && (layout == GJ_LAYOUT_LINES || best.avg_lines <= 1.25)) {
+    // a lookup wave walks as many lines as its farthest probe needs, so auto also bounds
+    // the longest walk, not only the mean
+    if (best.ok && (layout == GJ_LAYOUT_LINES || (best.avg_lines <= 1.10 && best.max_lines <= 4))) {
       if (best.root.segments == 0) {
         ix.lm_knots.release();
         ix.stats.spline_points = 0;

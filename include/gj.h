@@ -110,8 +110,10 @@ typedef struct gj_index_params {
  *                     only for the keys a full line pushed on, the lines after it, at most the
  *                     build's measured max_lines.  About 12.8 bytes per 8-byte key at the default
  *                     load, against the column's 8.
- *   GJ_LAYOUT_AUTO    lines when the build's mean lines per present key stays below 1.25 and the
- *                     line copy fits, else column */
+ *   GJ_LAYOUT_AUTO    lines for a column of 16 MiB or more (smaller ones are served from the
+ *                     caches, where the column's denser lines win) when the build's mean lines per
+ *                     present key stays at most 1.10, no key needs more than 4 lines and the line
+ *                     copy fits; else column */
 #define GJ_LAYOUT_AUTO 0u
 #define GJ_LAYOUT_COLUMN 1u
 #define GJ_LAYOUT_LINES 2u

@@ -173,10 +173,14 @@ def test_lines_every_driver_and_plan(gj, orc, gpu, kind):
 
 
 def test_auto_layout_prefers_lines_on_uniform_keys(gj, gpu):
-    keys = gj.gen_build_keys(1 << 20, 8, _lib.KEYS_UNIFORM, 63, 0x750)
+    # 2^22 u64 keys = a 32 MiB column, past the size auto keeps in the column layout
+    keys = gj.gen_build_keys(1 << 22, 8, _lib.KEYS_UNIFORM, 63, 0x750)
     col = gj.DeviceArray.from_numpy(keys)
     st = gj.Index("learned_plr", col, host_column=keys).stats()
     assert st["layout"] == LINES and st["avg_lines"] < 1.05
     st_col = gj.Index("learned_plr", col, host_column=keys,
                       params=gj.index_params(learned_layout=_lib.LAYOUT_COLUMN)).stats()
     assert st_col["layout"] == _lib.LAYOUT_COLUMN
+    small = gj.gen_build_keys(1 << 20, 8, _lib.KEYS_UNIFORM, 63, 0x751)
+    st_small = gj.Index("learned_plr", gj.DeviceArray.from_numpy(small), host_column=small).stats()
+    assert st_small["layout"] == _lib.LAYOUT_COLUMN  # an 8 MiB column stays in the column layout
