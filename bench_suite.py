@@ -456,25 +456,27 @@ def join_aggregate_leg(gj, stream, cpu=True, fact_rows=60_000_000, reps=6):
     res = gj.DeviceArray(4, np.int64)
     for kind in ("learned_plr", "btree", "hash"):
         idx = gj.Index(kind, dkey, host_column=skey, stream=stream)
-        ok = True
-        for c in copies:  # warm-up and check on both copies
-            res.fill(0, stream)
-            gj.join_aggregate(idx, *c["dev"], dflag, lo, hi, res, stream=stream)
+        for strategy in (("lane_refill", "plain", "partitioned") if kind == "learned_plr" else ("lane_refill",)):
+            ok = True
+            for c in copies:  # warm-up and check on both copies
+                res.fill(0, stream)
+                gj.join_aggregate(idx, *c["dev"], dflag, lo, hi, res, stream=stream, strategy=strategy)
+                stream.synchronize()
+                ok &= tuple(int(v) for v in res.numpy()[:3]) == c["want"]
+            e = [gj.Event() for _ in range(reps + 1)]
             stream.synchronize()
-            ok &= tuple(int(v) for v in res.numpy()[:3]) == c["want"]
-        e = [gj.Event() for _ in range(reps + 1)]
-        stream.synchronize()
-        e[0].record(stream)
-        for r in range(reps):
-            res.fill(0, stream)
-            gj.join_aggregate(idx, *copies[r % 2]["dev"], dflag, lo, hi, res, stream=stream)
-            e[r + 1].record(stream)
-        stream.synchronize()
-        ms = e[0].elapsed_ms(e[reps]) / reps
-        alg = fact_rows * 4 + copies[0]["n_qual"] * 13 + copies[0]["want"][2]
-        out["one_batch"][kind] = {"rows_per_s": fact_rows / ms * 1e3, "ms": ms, "sums_match": ok,
-                                  "algorithmic_GBps": alg / (ms * 1e-3) / 1e9}
-        log(f"C5 one batch {kind}: {fact_rows / ms * 1e-6:.0f} G rows/s match={ok}")
+            e[0].record(stream)
+            for r in range(reps):
+                res.fill(0, stream)
+                gj.join_aggregate(idx, *copies[r % 2]["dev"], dflag, lo, hi, res, stream=stream, strategy=strategy)
+                e[r + 1].record(stream)
+            stream.synchronize()
+            ms = e[0].elapsed_ms(e[reps]) / reps
+            alg = fact_rows * 4 + copies[0]["n_qual"] * 13 + copies[0]["want"][2]
+            label = kind if strategy == "lane_refill" else f"{kind}_{strategy}"
+            out["one_batch"][label] = {"rows_per_s": fact_rows / ms * 1e3, "ms": ms, "sums_match": ok,
+                                       "strategy": strategy, "algorithmic_GBps": alg / (ms * 1e-3) / 1e9}
+            log(f"C5 one batch {label}: {fact_rows / ms * 1e-6:.0f} G rows/s match={ok}")
         if kind != "learned_plr":
             idx.close()
             continue

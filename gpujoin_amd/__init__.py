@@ -354,10 +354,13 @@ def partition(keys: DeviceArray, m: int, radix_bits: int, ignore_bits: int, pass
 
 def join_aggregate(index: Index, fkey: DeviceArray, date: DeviceArray, price: DeviceArray, disc: DeviceArray,
                    flag: DeviceArray, date_lo: int, date_hi: int, result: DeviceArray, rows: Optional[int] = None,
-                   offset: int = 0, stream: Optional[Stream] = None):
+                   offset: int = 0, stream: Optional[Stream] = None, strategy="lane_refill", radix_bits: int = 0):
+    """Adds the Q14-shaped sums of the fact rows to `result`; strategy is "lane_refill" (default),
+    "plain" or "partitioned" (gj_agg_strategy)."""
     r = fkey.count if rows is None else int(rows)
     fb = FactBatch(fkey.offset(offset), date.offset(offset), price.offset(offset), disc.offset(offset), r)
-    ap = AggParams(date_lo, date_hi, flag.ptr)
+    st = _lib.AGG_STRATEGIES[strategy] if isinstance(strategy, str) else int(strategy)
+    ap = AggParams(date_lo, date_hi, flag.ptr, st, radix_bits)
     check(_lib.call("gj_join_aggregate", index.handle, C.byref(fb), C.byref(ap), result.ptr, _s(stream)))
 
 
@@ -396,7 +399,7 @@ class AggregatePlan(_Plan):
     def __init__(self, index: Index, batches, flag: DeviceArray, date_lo: int, date_hi: int, results: DeviceArray,
                  mode: int, stream: Optional[Stream] = None):
         arr = (FactBatch * len(batches))(*[FactBatch(*b) for b in batches])
-        ap = AggParams(date_lo, date_hi, flag.ptr)
+        ap = AggParams(date_lo, date_hi, flag.ptr, _lib.AGG_LANE_REFILL, 0)
         h = C.c_void_p()
         check(_lib.call("gj_aggregate_plan_create", index.handle, arr, len(batches), C.byref(ap), results.ptr, mode,
                         _s(stream), C.byref(h)))

@@ -39,6 +39,9 @@ PROBES_UNIFORM, PROBES_ZIPF = range(2)
 # gj_index_params.learned_flags / learned_layout
 LEARNED_UNTRAINED = 0x1
 LAYOUT_AUTO, LAYOUT_COLUMN, LAYOUT_LINES = range(3)
+# gj_agg_strategy
+AGG_LANE_REFILL, AGG_PLAIN, AGG_PARTITIONED = range(3)
+AGG_STRATEGIES = {"lane_refill": AGG_LANE_REFILL, "plain": AGG_PLAIN, "partitioned": AGG_PARTITIONED}
 
 
 class IndexParams(C.Structure):
@@ -88,7 +91,8 @@ class FactBatch(C.Structure):
 
 
 class AggParams(C.Structure):
-    _fields_ = [("date_lo", C.c_int32), ("date_hi", C.c_int32), ("d_flag", C.c_void_p)]
+    _fields_ = [("date_lo", C.c_int32), ("date_hi", C.c_int32), ("d_flag", C.c_void_p), ("strategy", C.c_uint32),
+                ("radix_bits", C.c_uint32)]
 
 
 class AggResult(C.Structure):

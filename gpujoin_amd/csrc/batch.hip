@@ -146,6 +146,42 @@ __device__ __forceinline__ void agg_rows(const Dev& d, const uint8_t* lds, AggQu
   }
 }
 
+// The plain strategy (ij_plain_kernel, src/tpch/tpch_14_ij.cu:34-85): each lane
+// takes its own rows and looks up the qualifying ones in place; the wave's lookup
+// runs with only the qualifying lanes active (about 1 % of them at the Q14 window).
+template <int U, class Dev>
+__device__ __forceinline__ void agg_rows_plain(const Dev& d, const uint8_t* lds, const FactBatchDev& b, uint64_t r0,
+                                               uint64_t r1, uint64_t step, int32_t lo, int32_t hi,
+                                               const uint8_t* __restrict__ flag, AggAcc& acc) {
+  const int ln = lane();
+  for (uint64_t base = r0; base < r1; base += step) {
+    int32_t dt[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const uint64_t i = base + static_cast<uint64_t>(k * kWave + ln);
+      dt[k] = i < r1 ? load_stream(b.date + i) : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const uint64_t i = base + static_cast<uint64_t>(k * kWave + ln);
+      const bool qual = i < r1 && dt[k] >= lo && dt[k] < hi;
+      if (ballot(qual) == 0) continue;  // wave-uniform
+      uint32_t fk = 0;
+      int64_t sv = 0;
+      if (qual) {
+        fk = static_cast<uint32_t>(b.fkey[i]);
+        sv = b.price[i] * (100 - static_cast<int64_t>(b.disc[i]));
+      }
+      const uint32_t r = d.lookup(fk, qual, lds);
+      if (qual && r != kNotFound) {
+        acc.den += sv;
+        if (flag[r]) acc.num += sv;
+        acc.matches += 1;
+      }
+    }
+  }
+}
+
 // Per-workgroup sums of the one-batch kernel and per-wave sums of the persistent
 // one (no result atomics; see agg_sum_kernel).
 struct AggPartial {
@@ -187,7 +223,7 @@ inline size_t agg_queue_bytes(int unroll) { return (kAggBlock / kWave) * agg_que
 // waves that took about half of the kernel's time.  Letting the last workgroup
 // add the partials instead needs a device-scope fence in every workgroup, whose
 // L2 write-back cost more than the second launch.)
-template <class Dev>
+template <class Dev, bool Plain>
 __global__ __launch_bounds__(kAggBlock) void join_agg_kernel(Dev d, FactBatchDev b, int32_t lo, int32_t hi,
                                                              const uint8_t* __restrict__ flag,
                                                              AggPartial* __restrict__ partial, bool direct) {
@@ -202,8 +238,12 @@ __global__ __launch_bounds__(kAggBlock) void join_agg_kernel(Dev d, FactBatchDev
   const uint64_t wave = static_cast<uint64_t>(blockIdx.x) * (blockDim.x / kWave) + w;
   AggAcc acc;
   const uint64_t span = static_cast<uint64_t>(kWave) * kAggUnrollBatch;
-  agg_rows<kAggUnrollBatch>(d, lds, q, b, wave * span, b.rows, waves * span, lo, hi, flag, acc);
-  while (q.count > 0) agg_drain(d, lds, q, q.count < kWave ? q.count : kWave, flag, acc);
+  if constexpr (Plain) {
+    agg_rows_plain<kAggUnrollBatch>(d, lds, b, wave * span, b.rows, waves * span, lo, hi, flag, acc);
+  } else {
+    agg_rows<kAggUnrollBatch>(d, lds, q, b, wave * span, b.rows, waves * span, lo, hi, flag, acc);
+    while (q.count > 0) agg_drain(d, lds, q, q.count < kWave ? q.count : kWave, flag, acc);
+  }
   const int64_t num = wave_sum_i64(acc.num);
   const int64_t den = wave_sum_i64(acc.den);
   const int64_t mt = wave_sum_i64(static_cast<int64_t>(acc.matches));
@@ -376,18 +416,175 @@ __global__ __launch_bounds__(256) void lookup_persistent_kernel(Dev d, const Pro
   }
 }
 
+// ---- partitioned strategy (ij_partitioning, src/tpch/tpch_14_common.cu:651-704,
+// src/tpch/tpch_14_ij_partitioning.cu:187-237): the qualifying rows are compacted to
+// (fkey, summand) pairs, radix-partitioned on the fkey, and the partitions are looked
+// up in key order, so neighbouring lanes and waves probe neighbouring index paths.
+
+constexpr uint64_t kFilterChunk = 16384;  // fact rows per filter workgroup
+
+// counts[c] = qualifying rows of chunk c.
+__global__ __launch_bounds__(kAggBlock) void agg_filter_count_kernel(const int32_t* __restrict__ date, uint64_t rows,
+                                                                     int32_t lo, int32_t hi,
+                                                                     uint32_t* __restrict__ counts) {
+  __shared__ uint32_t wc[kAggBlock / kWave];
+  const uint64_t c0 = static_cast<uint64_t>(blockIdx.x) * kFilterChunk;
+  const uint64_t c1 = c0 + kFilterChunk < rows ? c0 + kFilterChunk : rows;
+  uint32_t cnt = 0;
+  for (uint64_t i = c0 + threadIdx.x; i < c1; i += blockDim.x) {
+    const int32_t dt = load_stream(date + i);
+    cnt += (dt >= lo && dt < hi) ? 1u : 0u;
+  }
+  for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, kWave);
+  if (lane() == 0) wc[threadIdx.x / kWave] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int w = 0; w < kAggBlock / kWave; ++w) t += wc[w];
+    counts[blockIdx.x] = t;
+  }
+}
+
+// In place: counts[c] <- counts[0] + ... + counts[c - 1]; *total <- the sum.  One block.
+__global__ __launch_bounds__(1024) void agg_scan_counts_kernel(uint32_t* __restrict__ counts, uint64_t n,
+                                                               uint64_t* __restrict__ total) {
+  __shared__ uint32_t wsum[1024 / kWave];
+  const int w = threadIdx.x / kWave;
+  const int ln = lane();
+  uint64_t carry = 0;
+  for (uint64_t t0 = 0; t0 < n; t0 += blockDim.x) {
+    const uint64_t i = t0 + threadIdx.x;
+    const uint32_t v = i < n ? counts[i] : 0u;
+    uint32_t x = v;
+    for (int off = 1; off < kWave; off <<= 1) {
+      const uint32_t y = __shfl_up(x, off, kWave);
+      if (ln >= off) x += y;
+    }
+    if (ln == kWave - 1) wsum[w] = x;
+    __syncthreads();
+    uint64_t before = 0, all = 0;
+    for (int u = 0; u < static_cast<int>(blockDim.x / kWave); ++u) {
+      before += u < w ? wsum[u] : 0u;
+      all += wsum[u];
+    }
+    if (i < n) counts[i] = static_cast<uint32_t>(carry + before + x - v);
+    carry += all;
+    __syncthreads();  // wsum is rewritten by the next tile
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+
+// Writes chunk c's qualifying rows, in row order, from position offsets[c]:
+// key = fkey, val = price * (100 - discount).
+__global__ __launch_bounds__(kAggBlock) void agg_filter_write_kernel(FactBatchDev b, int32_t lo, int32_t hi,
+                                                                     const uint32_t* __restrict__ offsets,
+                                                                     uint32_t* __restrict__ out_key,
+                                                                     int64_t* __restrict__ out_val) {
+  __shared__ uint32_t wc[kAggBlock / kWave];
+  const uint64_t c0 = static_cast<uint64_t>(blockIdx.x) * kFilterChunk;
+  const uint64_t c1 = c0 + kFilterChunk < b.rows ? c0 + kFilterChunk : b.rows;
+  const int w = threadIdx.x / kWave;
+  const int ln = lane();
+  const uint64_t lt = (ln == 0) ? 0ull : (~0ull >> (64 - ln));
+  uint32_t run = offsets[blockIdx.x];
+  for (uint64_t t0 = c0; t0 < c1; t0 += blockDim.x) {
+    const uint64_t i = t0 + threadIdx.x;
+    bool qual = false;
+    if (i < c1) {
+      const int32_t dt = b.date[i];
+      qual = dt >= lo && dt < hi;
+    }
+    const uint64_t mask = ballot(qual);
+    if (ln == 0) wc[w] = static_cast<uint32_t>(__popcll(mask));
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+    for (int v = 0; v < kAggBlock / kWave; ++v) {
+      before += v < w ? wc[v] : 0u;
+      total += wc[v];
+    }
+    if (qual) {
+      const uint32_t at = run + before + static_cast<uint32_t>(__popcll(mask & lt));
+      out_key[at] = static_cast<uint32_t>(b.fkey[i]);
+      out_val[at] = b.price[i] * (100 - static_cast<int64_t>(b.disc[i]));
+    }
+    run += total;
+    __syncthreads();  // wc is rewritten by the next tile
+  }
+}
+
+// Looks up partitioned (fkey, position) tuples and sums the matches' summands; the
+// sweep over the tuples is lookup_tuples_kernel's (each XCD one contiguous eighth,
+// its workgroups in step).  Padding slots carry position 0xFFFFFFFF.
+template <class Dev>
+__global__ __launch_bounds__(kAggBlock) void agg_tuples_kernel(Dev d, const uint32_t* __restrict__ keys,
+                                                               const uint32_t* __restrict__ pos,
+                                                               const uint64_t* __restrict__ d_total,
+                                                               uint64_t total_max, const int64_t* __restrict__ vals,
+                                                               const uint8_t* __restrict__ flag,
+                                                               AggPartial* __restrict__ partial) {
+  extern __shared__ __attribute__((aligned(16))) uint8_t lds[];
+  __shared__ int64_t red[kAggBlock / kWave][3];
+  d.stage(lds);
+  __syncthreads();
+  uint64_t total = total_max;
+  if (d_total) total = *d_total < total_max ? *d_total : total_max;
+  const uint32_t groups = (gridDim.x % 8 == 0) ? 8u : 1u;  // XCDs
+  const uint32_t per = gridDim.x / groups;
+  const uint64_t chunk = ((total + groups - 1) / groups + kWave - 1) / kWave * kWave;
+  const uint64_t lo = static_cast<uint64_t>(blockIdx.x % groups) * chunk;
+  const uint64_t hi = (lo + chunk < total) ? lo + chunk : total;
+  const uint64_t w = threadIdx.x / kWave;
+  const uint64_t step = static_cast<uint64_t>(per) * blockDim.x;
+  AggAcc acc;
+  for (uint64_t base = lo + static_cast<uint64_t>(blockIdx.x / groups) * blockDim.x + w * kWave; base < hi;
+       base += step) {
+    const uint64_t i = base + static_cast<uint64_t>(lane());
+    uint32_t p = 0xFFFFFFFFu, k = 0;
+    if (i < hi) {
+      p = load_stream(pos + i);
+      k = load_stream(keys + i);
+    }
+    const bool active = p != 0xFFFFFFFFu;
+    const uint32_t r = d.lookup(k, active, lds);
+    if (active && r != kNotFound) {
+      const int64_t sv = vals[p];
+      acc.den += sv;
+      if (flag[r]) acc.num += sv;
+      acc.matches += 1;
+    }
+  }
+  const int64_t num = wave_sum_i64(acc.num);
+  const int64_t den = wave_sum_i64(acc.den);
+  const int64_t mt = wave_sum_i64(static_cast<int64_t>(acc.matches));
+  if (lane() == 0) {
+    red[w][0] = num;
+    red[w][1] = den;
+    red[w][2] = mt;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    AggPartial p{0, 0, 0, 0};
+    for (int v = 0; v < kAggBlock / kWave; ++v) {
+      p.num += red[v][0];
+      p.den += red[v][1];
+      p.matches += static_cast<uint64_t>(red[v][2]);
+    }
+    partial[blockIdx.x] = p;
+  }
+}
+
 namespace {
 
 // Workgroup partial sums: at most 8 workgroups of 256 threads fit one CU.
 inline uint64_t agg_partial_cap(int cu) { return 8ull * static_cast<uint64_t>(cu); }
 
 // partial holds partial_cap slots (0: agg_partial_cap(cu), what the callers' buffers are sized for).
-template <class Dev>
+template <class Dev, bool Plain = false>
 void launch_join_agg(const Dev& d, const FactBatchDev& b, int32_t lo, int32_t hi, const uint8_t* flag, int cu,
                      AggPartial* partial, hipStream_t s, uint64_t partial_cap = 0) {
   if (b.rows == 0) return;
   const size_t lds = (d.lds_bytes() + 15) / 16 * 16 + agg_queue_bytes(kAggUnrollBatch);
-  const void* fn = reinterpret_cast<const void*>(&join_agg_kernel<Dev>);
+  const void* fn = reinterpret_cast<const void*>(&join_agg_kernel<Dev, Plain>);
   uint64_t blocks = static_cast<uint64_t>(blocks_per_cu(fn, kAggBlock, lds)) * cu;
   blocks = std::min<uint64_t>(blocks, ceil_div(b.rows, static_cast<uint64_t>(kAggBlock) * kAggUnrollBatch));
   blocks = std::max<uint64_t>(blocks, 1);
@@ -400,8 +597,8 @@ void launch_join_agg(const Dev& d, const FactBatchDev& b, int32_t lo, int32_t hi
   // a small grid adds its workgroup sums to the result itself (a few dozen atomics
   // cost less than the second launch)
   const bool direct = blocks <= kAggDirectBlocks;
-  hipLaunchKernelGGL((join_agg_kernel<Dev>), dim3(static_cast<unsigned>(blocks)), dim3(kAggBlock), lds, s, d, b, lo, hi,
-                     flag, partial, direct);
+  hipLaunchKernelGGL((join_agg_kernel<Dev, Plain>), dim3(static_cast<unsigned>(blocks)), dim3(kAggBlock), lds, s, d, b,
+                     lo, hi, flag, partial, direct);
   GJ_HIP_CHECK(hipGetLastError());
   if (direct) return;
   hipLaunchKernelGGL(agg_sum_kernel, dim3(1), dim3(1024), 0, s, partial, static_cast<uint32_t>(blocks), b.result);
@@ -437,6 +634,85 @@ AggPartial* noalloc_partials() {
 void check_agg_index(const gj_index& ix) {
   GJ_REQUIRE(ix.key_bytes == 4, GJ_ERR_INVALID, "join-aggregate needs an index over 4-byte keys");
   GJ_REQUIRE(ix.kind != GJ_NO_OP, GJ_ERR_INVALID, "join-aggregate needs a real index");
+}
+
+uint32_t key_bits(uint64_t max_key) { return max_key ? 64u - static_cast<uint32_t>(__builtin_clzll(max_key)) : 1u; }
+
+// The partitioned strategy.  The count of qualifying rows sizes the buffers, so the
+// stream is synchronised once between the filter and the rest (the reference splits
+// its run the same way, phase1 / phase2, src/tpch/tpch_14_common.cu:720-731).
+template <class Dev>
+void join_aggregate_partitioned(const gj_index& ix, const Dev& d, const FactBatchDev& b, const gj_agg_params& p,
+                                hipStream_t s) {
+  if (b.rows == 0) return;
+  const uint32_t bits = p.radix_bits ? p.radix_bits : 11u;
+  GJ_REQUIRE(bits >= 1 && bits <= 12, GJ_ERR_INVALID, "join-aggregate radix_bits must be in [1, 12]");
+  const uint64_t chunks = ceil_div(b.rows, kFilterChunk);
+  GJ_REQUIRE(chunks < (1ull << 31), GJ_ERR_INVALID, "fact batch too large");
+  auto up = [](uint64_t v) { return (v + 255) / 256 * 256; };
+  // the chunk offsets (and, before them, the qualifying-row total) stay at the front of
+  // the stream's scratch; the rest is sized by the total
+  dbuf& sc = ix.scratch_for(s).a;
+  sc.reserve(256 + up(chunks * 4));
+  auto* total = sc.as<uint64_t>();
+  auto* offs = reinterpret_cast<uint32_t*>(sc.as<uint8_t>() + 256);
+  hipLaunchKernelGGL(agg_filter_count_kernel, dim3(static_cast<unsigned>(chunks)), dim3(kAggBlock), 0, s, b.date,
+                     b.rows, p.date_lo, p.date_hi, offs);
+  GJ_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(agg_scan_counts_kernel, dim3(1), dim3(1024), 0, s, offs, chunks, total);
+  GJ_HIP_CHECK(hipGetLastError());
+  uint64_t m = 0;
+  GJ_HIP_CHECK(hipMemcpyAsync(&m, total, sizeof(m), hipMemcpyDeviceToHost, s));
+  GJ_HIP_CHECK(hipStreamSynchronize(s));
+  if (m == 0) return;
+  GJ_REQUIRE(m < 0xFFFFFFFFull, GJ_ERR_OVERFLOW, "more than 2^32 - 2 qualifying rows");
+  const uint32_t ignore = key_bits(ix.max_key) > bits ? key_bits(ix.max_key) - bits : 0u;
+  const uint64_t fan = 1ull << bits;
+  const uint64_t cap = partition_capacity(m, 4, bits);
+  const uint64_t nparts = agg_partial_cap(ix.cu);
+  // total | offsets | compact keys | compact summands | tuple keys | tuple positions | start
+  // | count | partitioner scratch | workgroup partials (reserve keeps the front's contents
+  // only when it does not reallocate, so a larger buffer is set up before the count)
+  const uint64_t o_off = 256, o_key = o_off + up(chunks * 4), o_val = o_key + up(m * 4), o_tk = o_val + up(m * 8),
+                 o_tp = o_tk + up(cap * 4), o_st = o_tp + up(cap * 4), o_ct = o_st + up((fan + 1) * 8),
+                 o_ps = o_ct + up(fan * 8), o_pa = o_ps + up(partition_scratch(m, 4, bits, 1)),
+                 bytes = o_pa + up(nparts * sizeof(AggPartial));
+  if (bytes > sc.bytes) {
+    // grow and redo the count into the new buffer (dbuf::reserve does not keep contents)
+    sc.reserve(bytes);
+    total = sc.as<uint64_t>();
+    offs = reinterpret_cast<uint32_t*>(sc.as<uint8_t>() + 256);
+    hipLaunchKernelGGL(agg_filter_count_kernel, dim3(static_cast<unsigned>(chunks)), dim3(kAggBlock), 0, s, b.date,
+                       b.rows, p.date_lo, p.date_hi, offs);
+    GJ_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(agg_scan_counts_kernel, dim3(1), dim3(1024), 0, s, offs, chunks, total);
+    GJ_HIP_CHECK(hipGetLastError());
+  }
+  uint8_t* base = sc.as<uint8_t>();
+  auto* ckey = reinterpret_cast<uint32_t*>(base + o_key);
+  auto* cval = reinterpret_cast<int64_t*>(base + o_val);
+  auto* tkey = reinterpret_cast<uint32_t*>(base + o_tk);
+  auto* tpos = reinterpret_cast<uint32_t*>(base + o_tp);
+  auto* start = reinterpret_cast<uint64_t*>(base + o_st);
+  auto* count = reinterpret_cast<uint64_t*>(base + o_ct);
+  auto* partial = reinterpret_cast<AggPartial*>(base + o_pa);
+  hipLaunchKernelGGL(agg_filter_write_kernel, dim3(static_cast<unsigned>(chunks)), dim3(kAggBlock), 0, s, b,
+                     p.date_lo, p.date_hi, offs, ckey, cval);
+  GJ_HIP_CHECK(hipGetLastError());
+  partition_keys<uint32_t>(ckey, m, 0, bits, ignore, 1, false, tkey, tpos, start, count, base + o_ps,
+                           ix.dropped.as<unsigned long long>(), ix.cu, s);
+  const size_t lds = (d.lds_bytes() + 15) / 16 * 16;
+  const void* fn = reinterpret_cast<const void*>(&agg_tuples_kernel<Dev>);
+  uint64_t blocks = static_cast<uint64_t>(blocks_per_cu(fn, kAggBlock, lds)) * ix.cu;
+  blocks = std::min<uint64_t>(blocks, ceil_div(cap, kAggBlock));
+  blocks = std::min<uint64_t>(blocks, nparts);
+  if (blocks >= 8) blocks = blocks / 8 * 8;  // whole XCD rounds for the XCD-major sweep
+  blocks = std::max<uint64_t>(blocks, 1);
+  hipLaunchKernelGGL((agg_tuples_kernel<Dev>), dim3(static_cast<unsigned>(blocks)), dim3(kAggBlock), lds, s, d, tkey,
+                     tpos, start + fan, cap, cval, p.d_flag, partial);
+  GJ_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(agg_sum_kernel, dim3(1), dim3(1024), 0, s, partial, static_cast<uint32_t>(blocks), b.result);
+  GJ_HIP_CHECK(hipGetLastError());
 }
 
 FactBatchDev to_dev(const gj_fact_batch& b, gj_agg_result* res) {
@@ -680,11 +956,21 @@ int gj_join_aggregate(const gj_index* index, const gj_fact_batch* batch, const g
     check_agg_index(*index);
     GJ_REQUIRE(params->d_flag, GJ_ERR_INVALID, "null flag column");
     const FactBatchDev b = to_dev(*batch, d_result);
-    dbuf& partial = index->scratch_for(S(stream)).a;
-    partial.reserve(agg_partial_cap(index->cu) * sizeof(AggPartial));
+    GJ_REQUIRE(params->strategy <= GJ_AGG_PARTITIONED, GJ_ERR_INVALID, "unknown join-aggregate strategy");
+    const hipStream_t s = S(stream);
     with_index_dev<uint32_t>(*index, [&](auto d) {
-      launch_join_agg(d, b, params->date_lo, params->date_hi, params->d_flag, index->cu, partial.as<AggPartial>(),
-                      S(stream));
+      if (params->strategy == GJ_AGG_PARTITIONED) {
+        join_aggregate_partitioned(*index, d, b, *params, s);
+        return;
+      }
+      dbuf& partial = index->scratch_for(s).a;
+      partial.reserve(agg_partial_cap(index->cu) * sizeof(AggPartial));
+      if (params->strategy == GJ_AGG_PLAIN)
+        launch_join_agg<decltype(d), true>(d, b, params->date_lo, params->date_hi, params->d_flag, index->cu,
+                                           partial.as<AggPartial>(), s);
+      else
+        launch_join_agg(d, b, params->date_lo, params->date_hi, params->d_flag, index->cu, partial.as<AggPartial>(),
+                        s);
     });
   });
 }
@@ -695,11 +981,18 @@ int gj_join_aggregate_noalloc(const gj_index* index, const gj_fact_batch* batch,
     GJ_REQUIRE(index && batch && params && d_result, GJ_ERR_INVALID, "null argument");
     check_agg_index(*index);
     GJ_REQUIRE(params->d_flag, GJ_ERR_INVALID, "null flag column");
+    GJ_REQUIRE(params->strategy <= GJ_AGG_PARTITIONED, GJ_ERR_INVALID, "unknown join-aggregate strategy");
+    GJ_REQUIRE(params->strategy != GJ_AGG_PARTITIONED, GJ_ERR_UNSUPPORTED,
+               "the partitioned strategy needs scratch memory; use gj_join_aggregate");
     const FactBatchDev b = to_dev(*batch, d_result);
     AggPartial* partial = noalloc_partials();
     with_index_dev<uint32_t>(*index, [&](auto d) {
-      launch_join_agg(d, b, params->date_lo, params->date_hi, params->d_flag, index->cu, partial, S(stream),
-                      kNoAllocPartials);
+      if (params->strategy == GJ_AGG_PLAIN)
+        launch_join_agg<decltype(d), true>(d, b, params->date_lo, params->date_hi, params->d_flag, index->cu,
+                                           partial, S(stream), kNoAllocPartials);
+      else
+        launch_join_agg(d, b, params->date_lo, params->date_hi, params->d_flag, index->cu, partial, S(stream),
+                        kNoAllocPartials);
     });
   });
 }
@@ -746,6 +1039,8 @@ int gj_aggregate_plan_create(const gj_index* index, const gj_fact_batch* h_batch
     GJ_REQUIRE(mode >= GJ_BATCH_LAUNCH && mode <= GJ_BATCH_PERSISTENT, GJ_ERR_INVALID, "unknown batch mode");
     check_agg_index(*index);
     GJ_REQUIRE(params->d_flag, GJ_ERR_INVALID, "null flag column");
+    GJ_REQUIRE(params->strategy == GJ_AGG_LANE_REFILL, GJ_ERR_UNSUPPORTED,
+               "aggregate plans run the lane-refill strategy only");
     *out = nullptr;
     auto pl = std::make_unique<gj_batch_plan>();
     pl->ix = index;

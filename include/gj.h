@@ -264,10 +264,26 @@ typedef struct gj_fact_batch {
   uint64_t rows;
 } gj_fact_batch;
 
+/* How the qualifying rows meet the index (the reference's Q14 approaches,
+ * src/tpch/tpch_14_common.cu:748-755).  Every strategy gives the same sums.
+ *   GJ_AGG_LANE_REFILL  qualifying rows queue per wave in LDS and are looked up 64 at a time,
+ *                       fused with the sums (ij_np_lf_lr_plain's lane refill,
+ *                       src/tpch/tpch_14_ij.cu:464-566, without materialising join entries)
+ *   GJ_AGG_PLAIN        each lane looks up its own qualifying rows in place (ij_plain_kernel,
+ *                       src/tpch/tpch_14_ij.cu:34-85, with the correct date predicate)
+ *   GJ_AGG_PARTITIONED  compact the qualifying (fkey, summand) pairs, radix-partition them on the
+ *                       fkey, then look the partitions up in key order and sum (ij_partitioning,
+ *                       src/tpch/tpch_14_common.cu:651-704, src/tpch/tpch_14_ij_partitioning.cu:187-237);
+ *                       synchronises the stream once, after the filter counts the qualifying rows */
+typedef enum gj_agg_strategy { GJ_AGG_LANE_REFILL = 0, GJ_AGG_PLAIN = 1, GJ_AGG_PARTITIONED = 2 } gj_agg_strategy;
+
 typedef struct gj_agg_params {
   int32_t date_lo;        /* inclusive */
   int32_t date_hi;        /* exclusive */
   const uint8_t* d_flag;  /* per row of the sorted dimension: 1 = promo */
+  uint32_t strategy;      /* gj_agg_strategy, default GJ_AGG_LANE_REFILL (the only one batch plans run) */
+  uint32_t radix_bits;    /* GJ_AGG_PARTITIONED: log2 fan-out over the fkey's top bits, 1..12; 0 = 11, the
+                             reference's (src/tpch/tpch_14_common.cu:657) */
 } gj_agg_params;
 
 typedef struct gj_agg_result {
@@ -283,7 +299,8 @@ int gj_join_aggregate(const gj_index* index, const gj_fact_batch* batch, const g
 /* gj_join_aggregate for callers that cannot allocate device memory (code under
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  No
  * counterpart in the reference.  Same arguments, semantics and results as
- * gj_join_aggregate, but the call allocates nothing: the workgroup partial sums
+ * gj_join_aggregate (GJ_AGG_PARTITIONED, which needs scratch, is rejected with
+ * GJ_ERR_UNSUPPORTED), but the call allocates nothing: the workgroup partial sums
  * live in a static device array of 4096 slots (128 KiB per GPU, held until the
  * process ends) instead of index-owned scratch.  A launch uses at most 8
  * workgroups per CU, so devices with more than 512 CUs are rejected with

@@ -30,8 +30,10 @@ def _agg_index(gj, kind, skey):
     return col, gj.Index(kind, col, host_column=skey, params=gj.index_params(learned_leaf_bits=10, mlp_steps=100))
 
 
+@pytest.mark.parametrize("strategy", ["lane_refill", "plain", "partitioned"])
 @pytest.mark.parametrize("kind", AGG_KINDS)
-def test_join_aggregate_matches_oracle(gj, orc, gpu, kind):
+def test_join_aggregate_matches_oracle(gj, orc, gpu, kind, strategy):
+    print(f"[case] join-aggregate {kind} {strategy}", flush=True)
     dim_rows = 32 * 1000
     skey, sflag = _dimension(gj, dim_rows)
     # a quarter of the fact keys miss the dimension (keys above dim_rows)
@@ -42,12 +44,46 @@ def test_join_aggregate_matches_oracle(gj, orc, gpu, kind):
     dflag = gj.DeviceArray.from_numpy(sflag)
     res = gj.DeviceArray(4, np.int64)
     res.fill(0)
-    gj.join_aggregate(idx, *d, dflag, lo, hi, res)
+    gj.join_aggregate(idx, *d, dflag, lo, hi, res, strategy=strategy)
     gj.synchronize()
     num, den, mt = (int(v) for v in res.numpy()[:3])
     onum, oden, omt = orc.join_aggregate(skey, sflag, fkey, date, price, disc, lo, hi)
     assert (num, den, mt) == (onum, oden, omt)
     assert gj.promo_revenue(num, den) == orc.promo_revenue(onum, oden)
+    if strategy == "partitioned":
+        assert idx.stats()["dropped_tuples"] == 0
+
+
+@pytest.mark.parametrize("bits", [1, 4, 11, 12])
+@pytest.mark.parametrize("rows,lo,hi", [(1, 0, 2557), (5000, 0, 400), (700_001, 1000, 1030), (300_000, 3000, 3100)])
+def test_join_aggregate_partitioned_fanouts(gj, orc, gpu, bits, rows, lo, hi):
+    # fan-outs from 2 to 4096 over the fkey; (3000, 3100) is past the last date, so no
+    # row qualifies and the result stays as it was
+    print(f"[case] partitioned join-aggregate bits={bits} rows={rows} window=[{lo}, {hi})", flush=True)
+    dim_rows = 32 * 700
+    skey, sflag = _dimension(gj, dim_rows, seed=0x47)
+    fkey, date, price, disc = _fact(gj, rows, dim_rows + dim_rows // 5, seed=0x48 + rows)
+    col, idx = _agg_index(gj, "learned_plr", skey)
+    d = [gj.DeviceArray.from_numpy(a) for a in (fkey, date, price, disc)]
+    dflag = gj.DeviceArray.from_numpy(sflag)
+    res = gj.DeviceArray.from_numpy(np.array([5, 7, 11, 0], dtype=np.int64))
+    gj.join_aggregate(idx, *d, dflag, lo, hi, res, strategy="partitioned", radix_bits=bits)
+    gj.synchronize()
+    onum, oden, omt = orc.join_aggregate(skey, sflag, fkey, date, price, disc, lo, hi)
+    assert tuple(int(v) for v in res.numpy()[:3]) == (5 + onum, 7 + oden, 11 + omt)  # added to the result
+    assert idx.stats()["dropped_tuples"] == 0
+
+
+def test_join_aggregate_rejects_bad_strategy(gj, gpu):
+    skey, sflag = _dimension(gj, 1000, seed=0x49)
+    col, idx = _agg_index(gj, "btree", skey)
+    d = [gj.DeviceArray.from_numpy(a) for a in _fact(gj, 100, 1000, seed=0x4A)]
+    dflag = gj.DeviceArray.from_numpy(sflag)
+    res = gj.DeviceArray(4, np.int64)
+    with pytest.raises(gj.GjError):
+        gj.join_aggregate(idx, *d, dflag, 0, 10, res, strategy=7)
+    with pytest.raises(gj.GjError):
+        gj.join_aggregate(idx, *d, dflag, 0, 10, res, strategy="partitioned", radix_bits=13)
 
 
 @pytest.mark.parametrize("rows", [1, 1024, 5000, 128 * 1024, 128 * 1024 + 1, 140 * 1024])
@@ -63,13 +99,13 @@ def test_join_aggregate_small_grids(gj, orc, gpu, rows):
     d = [gj.DeviceArray.from_numpy(a) for a in (fkey, date, price, disc)]
     dflag = gj.DeviceArray.from_numpy(sflag)
     res = gj.DeviceArray(4, np.int64)
-    for rep in range(2):  # the second call adds to the first call's result
+    for rep in range(3):  # every call adds to the earlier calls' result
         if rep == 0:
             res.fill(0)
-        gj.join_aggregate(idx, *d, dflag, lo, hi, res)
+        gj.join_aggregate(idx, *d, dflag, lo, hi, res, strategy=("lane_refill", "plain", "partitioned")[rep])
     gj.synchronize()
     onum, oden, omt = orc.join_aggregate(skey, sflag, fkey, date, price, disc, lo, hi)
-    assert tuple(int(v) for v in res.numpy()[:3]) == (2 * onum, 2 * oden, 2 * omt)
+    assert tuple(int(v) for v in res.numpy()[:3]) == (3 * onum, 3 * oden, 3 * omt)
 
 
 def test_join_aggregate_all_rows_closed_form(gj, gpu):
