@@ -68,7 +68,8 @@ class IndexStats(C.Structure):
         ("fallback_leaves", C.c_uint64), ("fallback_keys", C.c_uint64), ("max_window", C.c_uint32),
         ("avg_window", C.c_double), ("train_loss", C.c_float), ("root_segments", C.c_uint32),
         ("split_leaves", C.c_uint32), ("dropped_tuples", C.c_uint64), ("layout", C.c_uint32),
-        ("max_lines", C.c_uint32), ("lines", C.c_uint64), ("avg_lines", C.c_double),
+        ("max_lines", C.c_uint32), ("lines", C.c_uint64), ("avg_lines", C.c_double), ("root_trained", C.c_uint32),
+        ("pad0", C.c_uint32),
     ]
 
 

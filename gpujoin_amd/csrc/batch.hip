@@ -646,7 +646,7 @@ void join_aggregate_partitioned(const gj_index& ix, const Dev& d, const FactBatc
                                 hipStream_t s) {
   if (b.rows == 0) return;
   const uint32_t bits = p.radix_bits ? p.radix_bits : 11u;
-  GJ_REQUIRE(bits >= 1 && bits <= 12, GJ_ERR_INVALID, "join-aggregate radix_bits must be in [1, 12]");
+  GJ_REQUIRE(bits >= 1 && bits <= 16, GJ_ERR_INVALID, "join-aggregate radix_bits must be in [1, 16]");
   const uint64_t chunks = ceil_div(b.rows, kFilterChunk);
   GJ_REQUIRE(chunks < (1ull << 31), GJ_ERR_INVALID, "fact batch too large");
   auto up = [](uint64_t v) { return (v + 255) / 256 * 256; };

@@ -32,7 +32,7 @@ template <class K>
 void lookup_partitioned(const gj_index& ix, const K* probes, uint64_t m, uint32_t* out, const gj_lookup_params& p,
                         hipStream_t s) {
   const uint32_t bits = p.radix_bits ? p.radix_bits : 10u;
-  GJ_REQUIRE(bits >= 1 && bits <= 12, GJ_ERR_INVALID, "radix_bits must be in [1, 12]");
+  GJ_REQUIRE(bits >= 1 && bits <= 16, GJ_ERR_INVALID, "radix_bits must be in [1, 16]");
   const uint32_t passes = p.passes == 2 ? 2u : 1u;
   GJ_REQUIRE(m < 0xFFFFFFFFull, GJ_ERR_OVERFLOW, "partitioning needs fewer than 2^32 - 1 probes");
   uint32_t ignore = p.ignore_bits;

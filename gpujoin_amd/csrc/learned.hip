@@ -759,6 +759,8 @@ void build_learned(gj_index& ix, hipStream_t s) {
   const uint64_t range = ix.max_key - ix.min_key;
   RootModel root{};
   root.inv_range = range ? 1.0f / static_cast<float>(range) : 0.0f;
+  std::vector<float> mlp_init;  // the MLP root's parameters before training
+  ix.stats.root_trained = (ix.kind == GJ_LEARNED_MLP && !(ix.params.learned_flags & GJ_LEARNED_UNTRAINED)) ? 1u : 0u;
 
   if (ix.kind == GJ_LEARNED_PLR) {
     const unsigned blocks = grid_for(n, 256, 2048);
@@ -805,6 +807,7 @@ void build_learned(gj_index& ix, hipStream_t s) {
       p0[2 * h + k] = 1.0f / static_cast<float>(h);
     }
     p0[3 * h] = 0.0f;
+    mlp_init = p0;
     ix.lm_mlp.alloc(((np * 4 + 15) / 16) * 16);
     fill_bytes(ix.lm_mlp.p, 0, ix.lm_mlp.bytes, s);
     GJ_HIP_CHECK(hipMemcpyAsync(ix.lm_mlp.p, p0.data(), np * 4, hipMemcpyHostToDevice, s));
@@ -921,7 +924,10 @@ void build_learned(gj_index& ix, hipStream_t s) {
         (void)hipGetLastError();
         return;  // auto: the line copy does not fit; try the next candidate or the column layout
       }
-      if (!best.ok || c.avg_lines < best.avg_lines) best = std::move(c);
+      // a candidate within the auto bounds beats one outside them; then fewer lines per key
+      auto good = [](const LinesBuild& b) { return b.avg_lines <= 1.10 && b.max_lines <= 4; };
+      if (!best.ok || (good(c) && !good(best)) || (good(c) == good(best) && c.avg_lines < best.avg_lines))
+        best = std::move(c);
     };
     constexpr double kGoodLines = 1.05;  // below this, a costlier candidate cannot gain much
     const bool linear_root = root.hidden == 0 && root.segments == 0;
@@ -932,6 +938,29 @@ void build_learned(gj_index& ix, hipStream_t s) {
     if (ix.kind == GJ_LEARNED_PLR && linear_root && ix.params.learned_root_segments == 0 &&
         (!best.ok || best.avg_lines > kGoodLines) && n >= 4)
       consider([&](LinesBuild& c) { lines_two_stage<K>(ix, spline_root(2048), lbits, lk, c, s); });
+    // Training lowers the MLP root's mean squared error, but that can leave a flat stretch
+    // that routes a crowd of keys to a few leaves (lognormal keys at 2^26: 700 lines per
+    // key); the quantile initialisation routes equal key counts per ramp.  So the build
+    // also places the keys under the initial root, and keeps the trained one unless the
+    // initial one does better.
+    bool init_root = false;
+    if (root.hidden && !mlp_init.empty() && !(ix.params.learned_flags & GJ_LEARNED_UNTRAINED) &&
+        (!best.ok || best.avg_lines > kGoodLines || best.max_lines > 4)) {
+      dbuf init_params(ix.lm_mlp.bytes);
+      fill_bytes(init_params.p, 0, init_params.bytes, s);
+      GJ_HIP_CHECK(hipMemcpyAsync(init_params.p, mlp_init.data(), mlp_init.size() * 4, hipMemcpyHostToDevice, s));
+      RootModel r0 = root;
+      r0.mlp = init_params.as<const float>();
+      consider([&](LinesBuild& c) { lines_two_stage<K>(ix, r0, lbits, lk, c, s); });
+      init_root = best.ok && best.root.mlp == r0.mlp;
+      if (init_root) {
+        // the index serves lookups with the initial parameters
+        GJ_HIP_CHECK(hipMemcpyAsync(ix.lm_mlp.p, init_params.p, init_params.bytes, hipMemcpyDeviceToDevice, s));
+        GJ_HIP_CHECK(hipStreamSynchronize(s));
+        best.root.mlp = ix.lm_mlp.as<const float>();
+      }
+    }
+    if (init_root) ix.stats.root_trained = 0;
     // a lookup wave walks as many lines as its farthest probe needs, so auto also bounds
     // the longest walk, not only the mean
     if (best.ok && (layout == GJ_LAYOUT_LINES || (best.avg_lines <= 1.10 && best.max_lines <= 4))) {

@@ -33,7 +33,7 @@ constexpr int kPartBlock = 256;
 constexpr int kPartWaves = kPartBlock / kWave;
 constexpr int kPartItems = 16;                        // items per lane per tile
 constexpr int kPartTile = kPartBlock * kPartItems;    // 4096 items
-constexpr uint32_t kMaxRadixBits = 12;
+constexpr uint32_t kMaxRadixBits = 16;
 
 __host__ __device__ __forceinline__ uint32_t part_align(int key_bytes) { return 128u / static_cast<uint32_t>(key_bytes); }
 
@@ -48,17 +48,54 @@ __device__ __forceinline__ void drop(unsigned long long* dropped) {
   if (dropped) atomicAdd(dropped, 1ull);
 }
 
+// Segment mode of the histogram and stable scatter kernels (the second pass of a
+// fan-out above 2^12): the input is cut into segs segments (segment h holds
+// [start[h], start[h] + count[h]), the first pass's partitions), each segment into
+// chunks of `chunk` items; block j handles chunk j - chunk0[h] of the segment h with
+// chunk0[h] <= j < chunk0[h + 1], and partition p of segment h is output partition
+// h * 2^bits + p.  Blocks from chunk0[segs] on have no work.  segs = 0: plain mode,
+// block c handles items [c * chunk, (c + 1) * chunk).
+struct Seg {
+  const uint64_t* chunk0;  // segs + 1 entries
+  const uint64_t* start;   // segs entries
+  const uint64_t* count;   // segs entries
+  uint32_t segs;
+};
+
+// This block's item range and output partition base; false: the block has no work.
+__device__ __forceinline__ bool seg_range(const Seg& sg, uint64_t c, uint64_t chunk, uint64_t m, uint32_t fan,
+                                          uint64_t& lo, uint64_t& hi, uint64_t& pbase) {
+  if (!sg.segs) {
+    lo = c * chunk;
+    hi = std::min<uint64_t>(m, lo + chunk);
+    pbase = 0;
+    return true;
+  }
+  if (c >= sg.chunk0[sg.segs]) return false;
+  uint32_t a = 0, b = sg.segs;  // the segment h with chunk0[h] <= c < chunk0[h + 1]
+  while (b - a > 1) {
+    const uint32_t mid = (a + b) / 2;
+    if (sg.chunk0[mid] <= c) a = mid; else b = mid;
+  }
+  lo = sg.start[a] + (c - sg.chunk0[a]) * chunk;
+  hi = std::min<uint64_t>(sg.start[a] + sg.count[a], lo + chunk);
+  pbase = static_cast<uint64_t>(a) * fan;
+  return true;
+}
+
 // One block per chunk (256 or 1024 threads), four loads in flight per thread.
 template <class K>
 __global__ __launch_bounds__(1024) void part_hist_kernel(const K* __restrict__ keys, uint64_t m, uint64_t chunk,
-                                                         uint32_t shift, uint32_t bits, uint32_t* __restrict__ hist) {
+                                                         uint32_t shift, uint32_t bits, uint32_t* __restrict__ hist,
+                                                         Seg sg) {
   extern __shared__ uint32_t h[];
   const uint32_t fan = 1u << bits;
+  // chunk index; the scatter maps blocks the same way
+  const uint64_t c = sg.segs ? static_cast<uint64_t>(blockIdx.x) : xcd_logical_block();
+  uint64_t lo, hi, pbase;
+  if (!seg_range(sg, c, chunk, m, fan, lo, hi, pbase)) return;  // block-uniform, before any barrier
   for (uint32_t p = threadIdx.x; p < fan; p += blockDim.x) h[p] = 0;
   __syncthreads();
-  const uint64_t c = xcd_logical_block();  // chunk index; the scatter maps blocks the same way
-  const uint64_t lo = c * chunk;
-  const uint64_t hi = std::min<uint64_t>(m, lo + chunk);
   constexpr int kIn = 4;
   for (uint64_t i0 = lo + threadIdx.x; i0 < hi; i0 += kIn * blockDim.x) {
     K v[kIn];
@@ -157,22 +194,22 @@ __global__ __launch_bounds__(kPartBlock) void part_scatter_kernel(
     const K* __restrict__ keys, const uint32_t* __restrict__ in_payload, uint32_t payload_base, uint64_t m,
     uint64_t chunk, uint32_t shift, uint32_t bits, const uint32_t* __restrict__ block_off,
     const uint64_t* __restrict__ start, K* __restrict__ out_keys, uint32_t* __restrict__ out_payload, uint64_t cap,
-    unsigned long long* __restrict__ dropped) {
+    unsigned long long* __restrict__ dropped, Seg sg) {
   extern __shared__ uint32_t sm[];
   const uint32_t fan = 1u << bits;
   uint32_t* cnt = sm;                        // [kPartWaves][fan]: counts, then cursors
   uint32_t* brun = sm + kPartWaves * fan;    // [fan]: block cursor (absolute position)
   // neighbouring chunks run on one XCD, so the partial lines that neighbouring
   // chunks write into a partition's run meet in the same L2
-  const uint64_t c = xcd_logical_block();
+  const uint64_t c = sg.segs ? static_cast<uint64_t>(blockIdx.x) : xcd_logical_block();
+  uint64_t lo, hi, pbase;
+  if (!seg_range(sg, c, chunk, m, fan, lo, hi, pbase)) return;  // block-uniform, before any barrier
   for (uint32_t p = threadIdx.x; p < fan; p += blockDim.x) {
-    brun[p] = static_cast<uint32_t>(start[p]) + block_off[c * fan + p];
+    brun[p] = static_cast<uint32_t>(start[pbase + p]) + block_off[c * fan + p];
 #pragma unroll
     for (int w = 0; w < kPartWaves; ++w) cnt[w * fan + p] = 0;
   }
   __syncthreads();
-  const uint64_t lo = c * chunk;
-  const uint64_t hi = std::min<uint64_t>(m, lo + chunk);
   const int w = threadIdx.x / kWave;
   const int ln = lane();
   const uint64_t lt = (ln == 0) ? 0ull : (~0ull >> (64 - ln));  // lanes below this one
@@ -549,6 +586,36 @@ __global__ __launch_bounds__(wc_block(Stable)) void part_scatter_wc_kernel(
   }
 }
 
+// Chunks of the segmented pass: chunk0[h] = chunks of the segments before h (each
+// segment cut into ceil(count[h] / chunk) chunks), chunk0[segs] = all chunks.  One thread.
+__global__ void seg_chunks_kernel(const uint64_t* __restrict__ count, uint32_t segs, uint64_t chunk,
+                                  uint64_t* __restrict__ chunk0) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  uint64_t run = 0;
+  for (uint32_t h = 0; h < segs; ++h) {
+    chunk0[h] = run;
+    run += (count[h] + chunk - 1) / chunk;
+  }
+  chunk0[segs] = run;
+}
+
+// Column scan of the segmented pass: for output partition g = h * fan + p, the
+// histogram rows of segment h's chunks become exclusive prefixes and count[g] the
+// total.  One thread per output partition.
+__global__ __launch_bounds__(256) void seg_colscan_kernel(uint32_t* __restrict__ hist, const uint64_t* __restrict__ chunk0,
+                                                          uint32_t segs, uint32_t fan, uint64_t* __restrict__ count) {
+  const uint64_t g = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (g >= static_cast<uint64_t>(segs) * fan) return;
+  const uint64_t h = g / fan, p = g % fan;
+  uint64_t run = 0;
+  for (uint64_t j = chunk0[h]; j < chunk0[h + 1]; ++j) {
+    const uint32_t v = hist[j * fan + p];
+    hist[j * fan + p] = static_cast<uint32_t>(run);
+    run += v;
+  }
+  count[g] = run;
+}
+
 namespace {
 
 struct PartPlan {
@@ -593,10 +660,29 @@ uint32_t wc_seg_bits(uint32_t bits, int key_bytes, bool stable) {
   return 0;
 }
 
-// histograms | (2 passes: first-pass keys and payloads) | partition starts for gj_partition
+// Above 2^12 partitions: a first pass on the high seg_high_bits() bits, then the
+// segmented pass on the rest.
+constexpr uint32_t kMaxSinglePassBits = 12;
+uint32_t seg_low_bits(uint32_t bits) { return bits / 2; }
+uint32_t seg_high_bits(uint32_t bits) { return bits - seg_low_bits(bits); }
+uint64_t seg_max_chunks(uint64_t m, uint32_t bits) {
+  return ceil_div(m, kPartTile) + (1ull << seg_high_bits(bits));
+}
+
+// Histogram region of a partitioning (either pass's rows).
+uint64_t hist_region(uint64_t m, uint32_t bits, int cu) {
+  if (bits <= kMaxSinglePassBits) return hist_bytes(m, bits, cu);
+  const uint64_t seg_rows = seg_max_chunks(m, bits) * (1ull << seg_low_bits(bits)) * 4;
+  return std::max<uint64_t>(hist_bytes(m, seg_high_bits(bits), cu), ceil_div(seg_rows + 256, 256) * 256);
+}
+
+// histograms | (2 passes: first-pass keys and payloads [| segment starts, counts, chunk
+// starts]) | partition starts for gj_partition
 uint64_t scratch_for(uint64_t m, int key_bytes, uint32_t bits, uint32_t passes, int cu) {
-  uint64_t b = hist_bytes(m, bits, cu);
-  if (passes == 2) b += (m + 32) * (key_bytes + 4) + 512;  // first-pass keys + payloads
+  uint64_t b = hist_region(m, bits, cu);
+  if (passes == 2 || bits > kMaxSinglePassBits)  // first-pass keys + payloads, each on a 256-byte boundary
+    b += ceil_div((m + 32) * key_bytes, 256) * 256 + ceil_div((m + 32) * 4, 256) * 256;
+  if (bits > kMaxSinglePassBits) b += 3 * ((1ull << seg_high_bits(bits)) + 2) * 8 + 256;
   b += ((1ull << bits) + 2) * 8 + 16;
   return b;
 }
@@ -621,7 +707,7 @@ void partition_pass(const K* keys, const uint32_t* in_payload, uint32_t payload_
   const size_t lds = p.fan * sizeof(uint32_t);
   // with one or two chunks per CU a chunk gets 1024 threads
   hipLaunchKernelGGL(part_hist_kernel<K>, dim3(p.blocks), dim3(per_cu <= 2 ? 1024 : kPartBlock), lds, s, keys, m,
-                     p.chunk, shift, bits, hist);
+                     p.chunk, shift, bits, hist, Seg{});
   GJ_HIP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(part_colscan_kernel, dim3((p.fan + 3) / 4), dim3(256), 0, s, hist, p.blocks, p.fan, count);
   GJ_HIP_CHECK(hipGetLastError());
@@ -659,7 +745,7 @@ void partition_pass(const K* keys, const uint32_t* in_payload, uint32_t payload_
     GJ_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&part_scatter_kernel<K>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(rlds)));
   hipLaunchKernelGGL(part_scatter_kernel<K>, dim3(p.blocks), dim3(kPartBlock), rlds, s, keys, in_payload, payload_base,
-                     m, p.chunk, shift, bits, hist, start, out_keys, out_payload, cap, dropped);
+                     m, p.chunk, shift, bits, hist, start, out_keys, out_payload, cap, dropped, Seg{});
   GJ_HIP_CHECK(hipGetLastError());
 }
 
@@ -671,14 +757,69 @@ uint64_t partition_scratch(uint64_t m, int key_bytes, uint32_t bits, uint32_t pa
   return scratch_for(m, key_bytes, bits, passes, device_cu_count());
 }
 
-// Full partitioning into `out` (1 or 2 passes); scratch must hold partition_scratch() bytes;
-// start must hold 2^bits + 1 entries (start[2^bits] = padded total).
+// The second pass above 2^12 partitions: the first pass left segment h (its partition
+// on the high bits) at [start1[h], start1[h] + count1[h]) of keys / payload; each segment
+// is partitioned, stably, on the low `bits` bits (digit (k >> shift) & (2^bits - 1)) into
+// output partitions h * 2^bits + p, with 128-byte aligned starts.  Chunks of one tile
+// never straddle segments, so every histogram row and cursor belongs to one segment.
+template <class K>
+void partition_segmented(const K* keys, const uint32_t* payload, uint64_t m, uint32_t shift, uint32_t bits,
+                         uint32_t segs, const uint64_t* start1, const uint64_t* count1, uint64_t* chunk0,
+                         uint32_t align, K* out_keys, uint32_t* out_payload, uint64_t* start, uint64_t* count,
+                         uint32_t* hist, unsigned long long* dropped, hipStream_t s) {
+  const uint32_t fan = 1u << bits;
+  const uint64_t total_fan = static_cast<uint64_t>(segs) * fan;
+  const uint64_t blocks = ceil_div(m, kPartTile) + segs;  // an upper bound on the chunks
+  GJ_REQUIRE(blocks < (1ull << 31), GJ_ERR_INVALID, "too many partition chunks");
+  hipLaunchKernelGGL(seg_chunks_kernel, dim3(1), dim3(64), 0, s, count1, segs, static_cast<uint64_t>(kPartTile),
+                     chunk0);
+  GJ_HIP_CHECK(hipGetLastError());
+  const Seg sg{chunk0, start1, count1, segs};
+  hipLaunchKernelGGL(part_hist_kernel<K>, dim3(static_cast<unsigned>(blocks)), dim3(kPartBlock), fan * sizeof(uint32_t),
+                     s, keys, m, static_cast<uint64_t>(kPartTile), shift, bits, hist, sg);
+  GJ_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(seg_colscan_kernel, dim3(static_cast<unsigned>(ceil_div(total_fan, 256))), dim3(256), 0, s, hist,
+                     chunk0, segs, fan, count);
+  GJ_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(part_scan_starts_kernel, dim3(1), dim3(1024), 0, s, count, static_cast<uint32_t>(total_fan), align,
+                     start);
+  GJ_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(part_pad_kernel<K>, dim3(static_cast<unsigned>(total_fan)), dim3(64), 0, s, start, count,
+                     static_cast<uint32_t>(total_fan), align, out_keys, out_payload);
+  GJ_HIP_CHECK(hipGetLastError());
+  const uint64_t cap = m + total_fan * align;
+  const size_t rlds = static_cast<size_t>(kPartWaves + 1) * fan * sizeof(uint32_t);
+  hipLaunchKernelGGL(part_scatter_kernel<K>, dim3(static_cast<unsigned>(blocks)), dim3(kPartBlock), rlds, s, keys,
+                     payload, 0u, m, static_cast<uint64_t>(kPartTile), shift, bits, hist, start, out_keys,
+                     out_payload, cap, dropped, sg);
+  GJ_HIP_CHECK(hipGetLastError());
+}
+
+// Full partitioning into `out` (1 or 2 passes; fan-outs above 2^12 always take two, the
+// second one segmented); scratch must hold partition_scratch() bytes; start must hold
+// 2^bits + 1 entries (start[2^bits] = padded total).
 template <class K>
 void partition_keys(const K* keys, uint64_t m, uint32_t payload_base, uint32_t bits, uint32_t ignore, uint32_t passes,
                     bool stable, K* out_keys, uint32_t* out_payload, uint64_t* start, uint64_t* count, uint8_t* scratch,
                     unsigned long long* dropped, int cu, hipStream_t s) {
   uint32_t* hist = reinterpret_cast<uint32_t*>(scratch);
   const uint32_t align = part_align(sizeof(K));
+  if (bits > kMaxSinglePassBits) {
+    const uint32_t hb = seg_high_bits(bits), lb = seg_low_bits(bits);
+    const uint32_t segs = 1u << hb;
+    uint8_t* tmp = scratch + hist_region(m, bits, cu);
+    K* tk = reinterpret_cast<K*>(tmp);
+    uint32_t* tp = reinterpret_cast<uint32_t*>(tmp + ceil_div((m + 32) * sizeof(K), 256) * 256);
+    uint8_t* small = tmp + ceil_div((m + 32) * sizeof(K), 256) * 256 + ceil_div((m + 32) * 4, 256) * 256;
+    auto* start1 = reinterpret_cast<uint64_t*>(small);
+    auto* count1 = start1 + segs + 2;
+    auto* chunk0 = count1 + segs + 2;
+    partition_pass<K>(keys, nullptr, payload_base, m, ignore + lb, hb, 1, stable, tk, tp, start1, count1, hist,
+                      dropped, cu, s);
+    partition_segmented<K>(tk, tp, m, ignore, lb, segs, start1, count1, chunk0, align, out_keys, out_payload, start,
+                           count, hist, dropped, s);
+    return;
+  }
   if (passes == 1 || bits < 2) {
     partition_pass<K>(keys, nullptr, payload_base, m, ignore, bits, align, stable, out_keys, out_payload, start, count,
                       hist, dropped, cu, s);
@@ -725,7 +866,7 @@ int gj_partition(const void* d_keys, uint64_t m, int key_bytes, const gj_partiti
     GJ_REQUIRE(params && out, GJ_ERR_INVALID, "null argument");
     GJ_REQUIRE(key_bytes == 4 || key_bytes == 8, GJ_ERR_INVALID, "key_bytes must be 4 or 8");
     const uint32_t bits = params->radix_bits;
-    GJ_REQUIRE(bits >= 1 && bits <= kMaxRadixBits, GJ_ERR_INVALID, "radix_bits must be in [1, 12]");
+    GJ_REQUIRE(bits >= 1 && bits <= kMaxRadixBits, GJ_ERR_INVALID, "radix_bits must be in [1, 16]");
     GJ_REQUIRE(params->ignore_bits < static_cast<uint32_t>(8 * key_bytes), GJ_ERR_INVALID,
                "ignore_bits must be below the key width");
     GJ_REQUIRE(params->passes == 1 || params->passes == 2, GJ_ERR_INVALID, "passes must be 1 or 2");

@@ -132,8 +132,8 @@ typedef struct gj_lookup_params {
   uint32_t grid_blocks;      /* 0 = enough to cover the input (plain) or 4 x CUs (persistent kinds);
                                 the partitioned driver's lookup then uses one workgroup per CU
                                 when the column is page-locked host memory */
-  uint32_t radix_bits;       /* partitioning: log2 fan-out, 1..12 (default 10) */
-  uint32_t passes;           /* partitioning: 1 or 2 (default 1) */
+  uint32_t radix_bits;       /* partitioning: log2 fan-out, 1..16 (default 10) */
+  uint32_t passes;           /* partitioning: 1 or 2 (default 1); above 12 bits always 2 */
   uint32_t ignore_bits;      /* partitioning: low key bits skipped; 0xFFFFFFFF = derive from the max key
                                 (partitioning_approach_dynamic_bit_range, src/index_lookup_new.cu:63-67) */
   uint64_t window;           /* partitioning: probes per window (default 2^24) */
@@ -196,6 +196,10 @@ typedef struct gj_index_stats {
   uint32_t max_lines;         /* lines layout: most lines a present key's lookup reads (1 + the largest spill) */
   uint64_t lines;             /* lines layout: 128-byte lines held */
   double avg_lines;           /* lines layout: mean lines a present key's lookup reads */
+  uint32_t root_trained;      /* learned_mlp: 1 when the trained parameters serve lookups; 0 when the build
+                                 ran untrained (GJ_LEARNED_UNTRAINED) or, in the lines layout, kept the
+                                 quantile initialisation because it placed the keys in fewer lines */
+  uint32_t pad0;
 } gj_index_stats;
 
 /* Fills *out.  dropped_tuples is read from the device: the call waits for the
@@ -221,9 +225,12 @@ const char* gj_index_type_name(gj_index_kind kind);
  * src/partitioned_relation.hpp:7-10).  Tuple order inside a partition
  * follows the input order (stable). */
 typedef struct gj_partition_params {
-  uint32_t radix_bits;  /* 1..12 */
+  uint32_t radix_bits;  /* 1..16 */
   uint32_t ignore_bits; /* < key bits */
-  uint32_t passes;      /* 1 or 2 (2: two scatter passes of ceil/floor(radix_bits/2) bits each) */
+  uint32_t passes;      /* 1 or 2 (2: a pass on the high ceil(radix_bits/2) bits, then the full digit).
+                           Above 12 bits always two: the high ceil(radix_bits/2) bits, then each of those
+                           partitions split on the low floor(radix_bits/2) bits (the reference's limit at
+                           48 KiB of shared memory is 8 bits per pass, src/index_lookup_partitioning.cu:50-52) */
 } gj_partition_params;
 
 typedef struct gj_partitioned {
@@ -282,7 +289,7 @@ typedef struct gj_agg_params {
   int32_t date_hi;        /* exclusive */
   const uint8_t* d_flag;  /* per row of the sorted dimension: 1 = promo */
   uint32_t strategy;      /* gj_agg_strategy, default GJ_AGG_LANE_REFILL (the only one batch plans run) */
-  uint32_t radix_bits;    /* GJ_AGG_PARTITIONED: log2 fan-out over the fkey's top bits, 1..12; 0 = 11, the
+  uint32_t radix_bits;    /* GJ_AGG_PARTITIONED: log2 fan-out over the fkey's top bits, 1..16; 0 = 11, the
                              reference's (src/tpch/tpch_14_common.cu:657) */
 } gj_agg_params;
 

@@ -62,7 +62,8 @@ def test_partition_capacity_and_promo_revenue(gj):
     lib = gj._lib.load()
     assert lib.gj_partition_capacity(1000, 8, 4) == 1000 + 16 * 16
     assert lib.gj_partition_capacity(1000, 4, 4) == 1000 + 16 * 32
-    assert lib.gj_partition_capacity(1000, 8, 13) == 0
+    assert lib.gj_partition_capacity(1000, 8, 16) == 1000 + 65536 * 16
+    assert lib.gj_partition_capacity(1000, 8, 17) == 0
     assert gj.promo_revenue(16380778, 100000000) == 100 * 16380778 * 1000 // (100000000 // 1000)
 
 

@@ -83,7 +83,7 @@ def test_join_aggregate_rejects_bad_strategy(gj, gpu):
     with pytest.raises(gj.GjError):
         gj.join_aggregate(idx, *d, dflag, 0, 10, res, strategy=7)
     with pytest.raises(gj.GjError):
-        gj.join_aggregate(idx, *d, dflag, 0, 10, res, strategy="partitioned", radix_bits=13)
+        gj.join_aggregate(idx, *d, dflag, 0, 10, res, strategy="partitioned", radix_bits=17)
 
 
 @pytest.mark.parametrize("rows", [1, 1024, 5000, 128 * 1024, 128 * 1024 + 1, 140 * 1024])

@@ -17,7 +17,8 @@ KINDS = ["binary_search", "btree", "harmonia", "radix_spline", "learned_plr", "l
 
 
 @pytest.mark.parametrize("kb", [4, 8])
-@pytest.mark.parametrize("bits,passes", [(1, 1), (4, 1), (8, 1), (10, 1), (12, 1), (8, 2), (12, 2)])
+@pytest.mark.parametrize("bits,passes", [(1, 1), (4, 1), (8, 1), (10, 1), (12, 1), (8, 2), (12, 2), (13, 2), (14, 2),
+                                         (16, 2)])
 @pytest.mark.parametrize("m", [0, 1, 1000, 100003])
 def test_partition_matches_oracle(gj, orc, gpu, kb, bits, passes, m):
     print(f"[case] partition kb={kb} bits={bits} passes={passes} m={m}", flush=True)
@@ -49,7 +50,7 @@ def test_partition_matches_oracle(gj, orc, gpu, kb, bits, passes, m):
         assert np.all(gp[o + c:pad_end] == 0xFFFFFFFF)
 
 
-@pytest.mark.parametrize("bits,passes", [(6, 1), (7, 1), (10, 1), (12, 1), (12, 2)])
+@pytest.mark.parametrize("bits,passes", [(6, 1), (7, 1), (10, 1), (12, 1), (12, 2), (14, 2), (16, 2)])
 def test_partition_many_tiles_per_block(gj, orc, gpu, bits, passes):
     # 5e6 keys: every block of the scatter walks several tiles (write-combining
     # scatter up to 7 bits, register tiles above), and the last tile of the last
@@ -77,7 +78,7 @@ def test_partition_many_tiles_per_block(gj, orc, gpu, bits, passes):
 def test_partition_rejects_bad_args(gj, gpu):
     keys = gj.DeviceArray(16, np.uint64)
     with pytest.raises(gj.GjError):
-        gj.partition(keys, 16, 13, 0)
+        gj.partition(keys, 16, 17, 0)
     with pytest.raises(gj.GjError):
         gj.partition(keys, 16, 4, 64)
 
@@ -97,6 +98,8 @@ def _setup(gj, kind, n, kb, dist=_lib.KEYS_UNIFORM):
     ("partitioning", {"radix_bits": 8, "window": 1 << 14}),
     ("partitioning", {"radix_bits": 12, "passes": 2, "window": 100000}),
     ("partitioning", {"radix_bits": 10}),
+    ("partitioning", {"radix_bits": 14, "window": 200000}),
+    ("partitioning", {"radix_bits": 16}),
     ("hj", {}),
 ])
 def test_approaches_match_oracle(gj, orc, gpu, kind, kb, approach, extra):
@@ -115,7 +118,7 @@ def test_approaches_match_oracle(gj, orc, gpu, kind, kb, approach, extra):
     assert np.array_equal(out.numpy(), want)
 
 
-@pytest.mark.parametrize("kb,bits", [(8, 8), (8, 10), (4, 10), (4, 11)])
+@pytest.mark.parametrize("kb,bits", [(8, 8), (8, 10), (4, 10), (4, 11), (8, 14), (8, 16), (4, 16)])
 def test_partitioned_lookup_many_tiles(gj, orc, gpu, kb, bits):
     # 3e6 probes in one window: each block of the unordered write-combining scatter
     # walks several tiles and carries open segments between them (u32 keys at

@@ -316,3 +316,57 @@ def test_compact_leaves_every_driver(gj, orc, gpu, kind):
             assert np.array_equal(out.numpy(), want), (mode, rep)
         plan.close()
     idx.close()
+
+
+def _rate(gj, idx, dp, out, m, steps=3):
+    s = gj.Stream()
+    idx.lookup(dp, out, m=m, stream=s)
+    s.synchronize()
+    e0, e1 = gj.Event(), gj.Event()
+    e0.record(s)
+    for _ in range(steps):
+        idx.lookup(dp, out, m=m, stream=s)
+    e1.record(s)
+    s.synchronize()
+    return m * steps / (e0.elapsed_ms(e1) * 1e-3)
+
+
+@pytest.mark.parametrize("dist", [_lib.KEYS_LOGNORMAL, _lib.KEYS_CLUSTERED])
+def test_what_training_gains(gj, orc, gpu, dist):
+    # The MLP root with its GPU training against the same root left at its initialisation
+    # (the piecewise-linear CDF through 33 quantiles), in both layouts, on 2^22 skewed keys
+    # with 10 % misses.  Training must lower the root's loss.  What that buys, measured:
+    #  - column layout, lognormal keys: the trained root spreads the keys so that no leaf
+    #    needs the B-tree (76 078 fallback keys untrained), about 2x the lookup rate;
+    #  - clustered keys: no gain: 64 narrow clusters defeat a 32-ramp root either way (every
+    #    key falls back in the column layout; the lines layout's leaves absorb it).
+    # The spline root of learned_plr is printed alongside as the method without a network.
+    n, m = 1 << 22, 1 << 23
+    keys = gj.gen_build_keys(n, 8, dist, 63, 0x31 + dist)
+    probes, exp = gj.gen_probes(keys, m, _lib.PROBES_UNIFORM, 1.0, 0.1, 63, 0x32 + dist)
+    col, dp = gj.DeviceArray.from_numpy(keys), gj.DeviceArray.from_numpy(probes)
+    out = gj.DeviceArray(m, np.uint32)
+    rows = {}
+    for layout in (_lib.LAYOUT_COLUMN, _lib.LAYOUT_LINES):
+        for label, flags in (("trained", 0), ("untrained", _lib.LEARNED_UNTRAINED)):
+            idx = gj.Index("learned_mlp", col, host_column=keys,
+                           params=gj.index_params(learned_layout=layout, learned_flags=flags))
+            st = idx.stats()
+            rate = _rate(gj, idx, dp, out, m)
+            assert np.array_equal(out.numpy(), exp), (layout, label)
+            rows[(layout, label)] = (st, rate)
+            print(f"[gain] dist={dist} layout={layout} {label}: loss={st['train_loss']:.3e} "
+                  f"avg_window={st['avg_window']:.1f} fallback_keys={st['fallback_keys']} "
+                  f"avg_lines={st['avg_lines']:.4f} max_lines={st['max_lines']} trained_root={st['root_trained']} "
+                  f"{rate / 1e9:.2f} G lookups/s", flush=True)
+            idx.close()
+        assert rows[(layout, "trained")][0]["train_loss"] < rows[(layout, "untrained")][0]["train_loss"]
+        assert rows[(layout, "untrained")][0]["root_trained"] == 0
+    spline = gj.Index("learned_plr", col, host_column=keys)
+    print(f"[gain] dist={dist} learned_plr (spline root) layout={spline.stats()['layout']}: "
+          f"{_rate(gj, spline, dp, out, m) / 1e9:.2f} G lookups/s", flush=True)
+    spline.close()
+    if dist == _lib.KEYS_LOGNORMAL:
+        tr, un = rows[(_lib.LAYOUT_COLUMN, "trained")], rows[(_lib.LAYOUT_COLUMN, "untrained")]
+        assert tr[0]["fallback_keys"] < un[0]["fallback_keys"]
+        assert tr[1] > 1.3 * un[1]
