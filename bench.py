@@ -91,7 +91,7 @@ def load_l2(kernel_tag: str):
 
 
 def _index_params_str(p) -> str:
-    return (f"lanes={p.harmonia_group_lanes} btree_lds={p.btree_lds_bytes} rs_bits={p.rs_radix_bits} "
+    return (f"lanes={p.harmonia_group_lanes} btree_lds={p.btree_lds_bytes} rs_bits={p.rs_radix_bits} rs_lds_bits={p.rs_lds_bits} "
             f"rs_err={p.rs_max_error} leaf_bits={p.learned_leaf_bits} max_err={p.learned_max_error}")
 
 

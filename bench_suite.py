@@ -419,7 +419,36 @@ def extras(gj, G, stream, keys64, d_keys64, d_out, cpu=True, steps=5, warmup=2):
     res["C4_learned_misses"] = c4
 
     res["C5_join_aggregate"] = join_aggregate_leg(gj, stream, cpu=cpu)
+    res["host_placement"] = host_placement_leg(gj, stream, keys64, d_keys64, d_out)
     return res
+
+
+def host_placement_leg(gj, stream, keys, d_keys, d_out, mh=1 << 24):
+    """The reference's placement: index structures (and the column) in page-locked host
+    memory read over the host link, plain against partitioned lookups (fan-out 4096, one
+    window).  2^26 u64 keys, 2^24 uniform probes in HBM."""
+    log = lambda *a: print("[extras]", *a, file=sys.stderr, flush=True)
+    probes, exp = gj.gen_probes(keys, mh, _lib.PROBES_UNIFORM, 1.0, 0.0, 63, 0x25)
+    dp = gj.DeviceArray.from_numpy(probes, stream)
+    hcol = gj.HostArray.from_numpy(keys)
+    out = {"workload": "2^26 u64 keys, 2^24 uniform probes (in HBM); index structures in page-locked host memory",
+           "rows": {}}
+    for col_name, col in (("column_in_hbm", d_keys), ("column_in_host", hcol)):
+        for kind in ("btree", "learned_plr"):
+            idx = gj.Index(kind, col, host_column=keys, stream=stream,
+                           params=gj.index_params(placement=_lib.PLACE_HOST))
+            st = idx.stats()
+            for appr, kw in (("plain", {}), ("partitioning", {"radix_bits": 12, "window": mh})):
+                lp = gj.lookup_params(appr, **kw)
+                ms = _timed(gj, lambda: idx.lookup(dp, d_out, m=mh, params=lp, stream=stream), stream, 2, 1)
+                ok = bool(np.array_equal(d_out.numpy(mh), exp))
+                label = f"{kind}_{appr}_{col_name}"
+                out["rows"][label] = {"lookups_per_s": mh / ms * 1e3, "rowids_match": ok, "layout": st["layout"],
+                                      "index_bytes_in_host": st["memory_bytes"]}
+                log(f"host placement {label}: {mh / ms * 1e-6:.3f} G lookups/s match={ok}")
+            idx.close()
+    hcol.free()
+    return out
 
 
 def join_aggregate_leg(gj, stream, cpu=True, fact_rows=60_000_000, reps=6):

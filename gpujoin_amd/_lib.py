@@ -39,6 +39,8 @@ PROBES_UNIFORM, PROBES_ZIPF = range(2)
 # gj_index_params.learned_flags / learned_layout
 LEARNED_UNTRAINED = 0x1
 LAYOUT_AUTO, LAYOUT_COLUMN, LAYOUT_LINES = range(3)
+# gj_index_params.placement
+PLACE_DEVICE, PLACE_HOST = range(2)
 # gj_agg_strategy
 AGG_LANE_REFILL, AGG_PLAIN, AGG_PARTITIONED = range(3)
 AGG_STRATEGIES = {"lane_refill": AGG_LANE_REFILL, "plain": AGG_PLAIN, "partitioned": AGG_PARTITIONED}
@@ -50,7 +52,8 @@ class IndexParams(C.Structure):
         ("btree_lds_bytes", C.c_uint32), ("learned_leaf_bits", C.c_uint32), ("learned_max_error", C.c_uint32),
         ("mlp_hidden", C.c_uint32), ("mlp_steps", C.c_uint32), ("mlp_batch", C.c_uint32), ("mlp_lr", C.c_float),
         ("seed", C.c_uint64), ("learned_root_segments", C.c_uint32), ("learned_flags", C.c_uint32),
-        ("learned_layout", C.c_uint32), ("learned_line_keys", C.c_uint32),
+        ("learned_layout", C.c_uint32), ("learned_line_keys", C.c_uint32), ("rs_lds_bits", C.c_uint32),
+        ("placement", C.c_uint32),
     ]
 
 
@@ -69,7 +72,7 @@ class IndexStats(C.Structure):
         ("avg_window", C.c_double), ("train_loss", C.c_float), ("root_segments", C.c_uint32),
         ("split_leaves", C.c_uint32), ("dropped_tuples", C.c_uint64), ("layout", C.c_uint32),
         ("max_lines", C.c_uint32), ("lines", C.c_uint64), ("avg_lines", C.c_double), ("root_trained", C.c_uint32),
-        ("pad0", C.c_uint32),
+        ("placement", C.c_uint32),
     ]
 
 

@@ -14,6 +14,7 @@ namespace {
 void fill_defaults(gj_index_params& p) {
   if (!p.rs_radix_bits) p.rs_radix_bits = 18;
   if (!p.rs_max_error) p.rs_max_error = 32;
+  if (!p.rs_lds_bits) p.rs_lds_bits = 13;
   if (!p.harmonia_group_lanes) p.harmonia_group_lanes = 8;  // one 128-byte line per load
   if (!p.btree_lds_bytes) p.btree_lds_bytes = 16384;
   // learned_leaf_bits 0: chosen from the column size at build
@@ -120,7 +121,16 @@ int gj_index_build(gj_index_kind kind, const void* h_sorted_keys, const void* d_
                                   key_bytes, hipMemcpyDeviceToHost, s));
       GJ_HIP_CHECK(hipStreamSynchronize(s));
     }
+    GJ_REQUIRE(ix->params.placement <= GJ_PLACE_HOST, GJ_ERR_INVALID, "placement must be 0 or 1");
     build_index(*ix, s);
+    if (ix->params.placement == GJ_PLACE_HOST) {
+      GJ_HIP_CHECK(hipStreamSynchronize(s));
+      for (dbuf* b : {&ix->samples, &ix->btree_nodes, &ix->hm_keys, &ix->hm_children, &ix->rs_radix, &ix->rs_knots,
+                      &ix->lm_leaves, &ix->lm_mlp, &ix->lm_knots, &ix->lm_seps, &ix->lm_subs, &ix->lm_groups,
+                      &ix->lm_lines, &ix->lm_line_leaves, &ix->ht})
+        b->to_host();
+      ix->stats.placement = GJ_PLACE_HOST;
+    }
     ix->dropped.alloc(16);
     fill_bytes(ix->dropped.p, 0, ix->dropped.bytes, s);
     GJ_HIP_CHECK(hipStreamSynchronize(s));

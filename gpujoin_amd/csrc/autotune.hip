@@ -109,7 +109,9 @@ std::vector<gj_index_params> index_candidates(gj_index_kind kind, uint64_t n, in
       for (uint32_t b : {4096u, 16384u, 32768u}) add([&](gj_index_params& p) { p.btree_lds_bytes = b; });
       break;
     case GJ_RADIX_SPLINE:
-      for (uint32_t r : {16u, 18u, 20u})
+      // radix bits whose table LDS holds (12, 13) and the larger global tables (16, 18) x the
+      // error bound
+      for (uint32_t r : {12u, 13u, 16u, 18u})
         for (uint32_t e : {16u, 32u, 64u})
           add([&](gj_index_params& p) {
             p.rs_radix_bits = r;

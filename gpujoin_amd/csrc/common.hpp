@@ -58,18 +58,43 @@ int guarded(F&& f) {
   }
 }
 
-// Owning HBM buffer (the role of device_array_wrapper, src/device_array.hpp:132-236).
+// Owning buffer (the role of device_array_wrapper, src/device_array.hpp:132-236): HBM,
+// or page-locked host memory mapped into the GPU's address space after to_host().
 struct dbuf {
   void* p = nullptr;
   size_t bytes = 0;
+  bool host = false;
   dbuf() = default;
   explicit dbuf(size_t b) { alloc(b); }
   dbuf(const dbuf&) = delete;
   dbuf& operator=(const dbuf&) = delete;
-  dbuf(dbuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  dbuf(dbuf&& o) noexcept : p(o.p), bytes(o.bytes), host(o.host) { o.p = nullptr; o.bytes = 0; o.host = false; }
   dbuf& operator=(dbuf&& o) noexcept {
-    if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+    if (this != &o) {
+      release();
+      p = o.p;
+      bytes = o.bytes;
+      host = o.host;
+      o.p = nullptr;
+      o.bytes = 0;
+      o.host = false;
+    }
     return *this;
+  }
+  // Moves the contents to page-locked host memory (the index placement of the reference's
+  // host allocators, src/index_lookup_config.hpp:52-66); kernels then read it over the host link.
+  void to_host() {
+    if (!p || host) return;
+    void* h = nullptr;
+    GJ_HIP_CHECK(hipHostMalloc(&h, bytes, hipHostMallocDefault));
+    const hipError_t e = hipMemcpy(h, p, bytes, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+      (void)hipHostFree(h);
+      GJ_HIP_CHECK(e);
+    }
+    (void)hipFree(p);
+    p = h;
+    host = true;
   }
   ~dbuf() { release(); }
   void alloc(size_t b) {
@@ -83,9 +108,10 @@ struct dbuf {
     if (b > bytes) alloc(b);
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) (void)(host ? hipHostFree(p) : hipFree(p));
     p = nullptr;
     bytes = 0;
+    host = false;
   }
   template <class T> T* as() const { return static_cast<T*>(p); }
 };

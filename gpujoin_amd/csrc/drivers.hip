@@ -64,7 +64,8 @@ void lookup_partitioned(const gj_index& ix, const K* probes, uint64_t m, uint32_
   // sweeps a narrow key range and the host pages in flight stay within reach of
   // the GPU's address translation (2x on a 2^26-key column)
   gj_lookup_params tp = p;
-  if (ix.col_in_host && !tp.grid_blocks) tp.grid_blocks = static_cast<uint32_t>(ix.cu);
+  if ((ix.col_in_host || ix.params.placement == GJ_PLACE_HOST) && !tp.grid_blocks)
+    tp.grid_blocks = static_cast<uint32_t>(ix.cu);
   for (uint64_t w0 = 0; w0 < m; w0 += window) {
     const uint64_t w = std::min<uint64_t>(window, m - w0);
     // unordered: the payload carries each probe's position back to `out`

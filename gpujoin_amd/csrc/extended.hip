@@ -147,12 +147,12 @@ void build_harmonia(gj_index& ix, hipStream_t s) {
 // until it does (the FP64 interpolation can round across the exact bound).
 // ---------------------------------------------------------------------------
 template <class K>
-__global__ void rs_check_kernel(RadixSplineDev<K> d, unsigned long long* bad) {
+__global__ void rs_check_kernel(RadixSplineDev<K, false> d, unsigned long long* bad) {
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
   unsigned long long cnt = 0;
   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < d.n; i += stride) {
     uint64_t lo, hi;
-    d.window(d.estimate(d.col[i]), lo, hi);
+    d.window(d.estimate(d.col[i], d.radix), lo, hi);
     cnt += (i < lo || i >= hi) ? 1ull : 0ull;
   }
   if (cnt) atomicAdd(bad, cnt);
@@ -184,6 +184,9 @@ void build_rs(gj_index& ix, hipStream_t s) {
   ix.rs_points = rs.knot_x.size();
   ix.rs_radix.alloc(rs.radix.size() * sizeof(uint32_t));
   GJ_HIP_CHECK(hipMemcpyAsync(ix.rs_radix.p, rs.radix.data(), ix.rs_radix.bytes, hipMemcpyHostToDevice, s));
+  // staged in LDS when the table has at most 2^rs_lds_bits + 2 entries
+  ix.rs_lds = rs.radix.size() <= (1ull << std::min<uint32_t>(ix.params.rs_lds_bits, 16)) + 2;
+  ix.stats.lds_levels = ix.rs_lds ? 1 : 0;
   std::vector<RsKnot> kn(rs.knot_x.size());
   for (size_t i = 0; i < kn.size(); ++i) kn[i] = RsKnot{static_cast<uint64_t>(rs.knot_x[i]), rs.knot_y[i]};
   ix.rs_knots.alloc(kn.size() * sizeof(RsKnot));
@@ -192,7 +195,7 @@ void build_rs(gj_index& ix, hipStream_t s) {
   dbuf bad(sizeof(unsigned long long));
   for (int attempt = 0; attempt < 40; ++attempt) {
     fill_bytes(bad.p, 0, bad.bytes, s);
-    hipLaunchKernelGGL(rs_check_kernel<K>, dim3(grid_for(ix.n, 256, 8192)), dim3(256), 0, s, ix.rs_dev<K>(),
+    hipLaunchKernelGGL(rs_check_kernel<K>, dim3(grid_for(ix.n, 256, 8192)), dim3(256), 0, s, ix.rs_dev<K, false>(),
                        bad.as<unsigned long long>());
     GJ_HIP_CHECK(hipGetLastError());
     unsigned long long h = 0;

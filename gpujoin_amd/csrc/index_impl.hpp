@@ -54,6 +54,7 @@ struct gj_index {
   // RadixSpline
   gj::dbuf rs_radix, rs_knots;
   uint32_t rs_shift = 0, rs_err = 0;
+  bool rs_lds = false;  // the radix table is staged in LDS
   uint64_t rs_points = 0;
 
   // learned two-stage models (B-tree above is their fallback)
@@ -120,12 +121,13 @@ struct gj_index {
     return d;
   }
 
-  template <class K>
-  gj::RadixSplineDev<K> rs_dev() const {
-    gj::RadixSplineDev<K> d{};
+  template <class K, bool Lds>
+  gj::RadixSplineDev<K, Lds> rs_dev() const {
+    gj::RadixSplineDev<K, Lds> d{};
     d.col = static_cast<const K*>(d_col);
     d.n = n;
     d.radix = rs_radix.as<const uint32_t>();
+    d.radix_count = static_cast<uint32_t>(rs_radix.bytes / sizeof(uint32_t));
     d.knots = rs_knots.as<const gj::RsKnot>();
     d.min_key = static_cast<K>(min_key);
     d.max_key = static_cast<K>(max_key);

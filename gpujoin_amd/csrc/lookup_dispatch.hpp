@@ -109,7 +109,12 @@ void with_index_dev(const gj_index& ix, F&& f) {
           break;
       }
       throw error(GJ_ERR_INVALID, "unsupported harmonia_group_lanes");
-    case GJ_RADIX_SPLINE: f(ix.rs_dev<K>()); return;
+    case GJ_RADIX_SPLINE:
+      if (ix.rs_lds)
+        f(ix.rs_dev<K, true>());
+      else
+        f(ix.rs_dev<K, false>());
+      return;
     case GJ_LEARNED_PLR:
     case GJ_LEARNED_MLP:
       if (ix.lm_layout == GJ_LAYOUT_LINES)

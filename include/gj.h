@@ -98,7 +98,19 @@ typedef struct gj_index_params {
   uint32_t learned_layout;        /* GJ_LAYOUT_AUTO (default), GJ_LAYOUT_COLUMN or GJ_LAYOUT_LINES */
   uint32_t learned_line_keys;     /* lines layout: build keys per 128-byte line on average; default 10 of the
                                      15 a line holds for 8-byte keys, 20 of 30 for 4-byte keys */
+  uint32_t rs_lds_bits;           /* RadixSpline: the radix table is staged in LDS when rs_radix_bits <= this
+                                     (at most 16), else read from global memory; default 13 (32 KiB) */
+  uint32_t placement;             /* GJ_PLACE_DEVICE (default) or GJ_PLACE_HOST: where the index's own
+                                     structures live (the column is the caller's and stays where it is) */
 } gj_index_params;
+
+/* placement: the reference keeps its indexes in host memory by default and reads them over
+ * the interconnect (src/index_lookup_config.hpp:52-66, src/btree.cuh:440-473).  GJ_PLACE_HOST
+ * builds in HBM, then moves every index structure (tree nodes, Harmonia keys and children,
+ * spline knots and radix table, learned leaves and lines, the hash table) to page-locked host
+ * memory mapped into the GPU's address space; lookups read it in place over the host link. */
+#define GJ_PLACE_DEVICE 0u
+#define GJ_PLACE_HOST 1u
 
 /* Layouts of the learned kinds.
  *   GJ_LAYOUT_COLUMN  the model predicts a position in the borrowed column and a bounded window
@@ -199,7 +211,7 @@ typedef struct gj_index_stats {
   uint32_t root_trained;      /* learned_mlp: 1 when the trained parameters serve lookups; 0 when the build
                                  ran untrained (GJ_LEARNED_UNTRAINED) or, in the lines layout, kept the
                                  quantile initialisation because it placed the keys in fewer lines */
-  uint32_t pad0;
+  uint32_t placement;         /* GJ_PLACE_DEVICE or GJ_PLACE_HOST: where the index structures live */
 } gj_index_stats;
 
 /* Fills *out.  dropped_tuples is read from the device: the call waits for the
@@ -365,7 +377,7 @@ int gj_autotune_lookup(const gj_index* index, const void* d_probe_sample, uint64
  * the defaults when base is NULL):
  *   harmonia      harmonia_group_lanes 8 / 16 (/ 32 for 4-byte keys): threads per lookup
  *   btree         btree_lds_bytes 4096 / 16384 / 32768: the LDS-resident upper levels
- *   radix_spline  rs_radix_bits 16 / 18 / 20 x rs_max_error 16 / 32 / 64
+ *   radix_spline  rs_radix_bits 12 / 13 (table in LDS) / 16 / 18 (global) x rs_max_error 16 / 32 / 64
  *   learned_plr   learned_leaf_bits 0 (the default: compact 8-byte leaves where the build
  *                 admits them), then 16-byte leaves at default - 1 / default / default + 1
  *                 bits x learned_max_error 64 / 128 / 256

@@ -222,6 +222,7 @@ def test_autotune_returns_exact_config(gj, orc, gpu):
     ("harmonia", 8, "harmonia_group_lanes", {8, 16}),
     ("btree", 8, "btree_lds_bytes", {4096, 16384, 32768}),
     ("radix_spline", 8, "rs_max_error", {16, 32, 64}),
+    ("radix_spline", 8, "rs_radix_bits", {12, 13, 16, 18}),
     ("learned_plr", 8, "learned_leaf_bits", {0, 15, 16, 17}),
 ])
 def test_index_autotuner_picks_a_candidate(gj, orc, gpu, kind, kb, field, allowed):

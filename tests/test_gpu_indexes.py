@@ -86,12 +86,16 @@ def test_radix_spline_matches_oracle(gj, orc, gpu, kb, err, dist):
     assert np.array_equal(ref.lookup(probes), want)
 
 
-@pytest.mark.parametrize("bits", [1, 4, 12])
-def test_radix_spline_radix_bits(gj, orc, gpu, bits):
+@pytest.mark.parametrize("bits", [1, 4, 12, 13, 14, 18])
+@pytest.mark.parametrize("lds_bits", [6, 13, 16])
+def test_radix_spline_radix_bits(gj, orc, gpu, bits, lds_bits):
+    # the radix table is staged in LDS when it has at most 2^lds_bits + 2 entries
     keys = gj.gen_build_keys(50000, 8, _lib.KEYS_UNIFORM, 0, 17)
     probes = _probes(gj, keys, 20000, 18)
-    got, _ = _lookup(gj, "radix_spline", keys, probes, params=gj.index_params(rs_radix_bits=bits, rs_max_error=16))
+    got, st = _lookup(gj, "radix_spline", keys, probes,
+                      params=gj.index_params(rs_radix_bits=bits, rs_lds_bits=lds_bits, rs_max_error=16))
     assert np.array_equal(got, orc.lookup_lower_bound(keys, probes))
+    assert st["lds_levels"] == (1 if bits <= lds_bits else 0)
 
 
 # ---- learned two-stage models -------------------------------------------------
