@@ -383,8 +383,15 @@ def extras(gj, G, stream, keys64, d_keys64, d_out, cpu=True, steps=5, warmup=2):
     # C4: learned indexes over three key distributions, 10 % misses, B-tree fallback counted
     c4 = {"workload": "2^26 unique u64 keys (uniform, lognormal(0, 2), 64 Gaussian clusters), 2^28 probes with "
                       "10% misses", "distributions": {}}
-    variants = [("btree", {}), ("radix_spline", {"rs_max_error": 8}), ("radix_spline", {"rs_max_error": 32}),
-                ("radix_spline", {"rs_max_error": 64}), ("learned_plr", {}), ("learned_mlp", {})]
+    # the learned index over the column itself (no key copy, a few MiB of index) beside the
+    # default lines layout; RadixSpline with its radix table staged in LDS (2^13 entries)
+    variants = [("btree", "btree", {}), ("radix_spline_err8", "radix_spline", {"rs_max_error": 8}),
+                ("radix_spline_err32", "radix_spline", {"rs_max_error": 32}),
+                ("radix_spline_err64", "radix_spline", {"rs_max_error": 64}),
+                ("radix_spline_lds13_err32", "radix_spline", {"rs_radix_bits": 13, "rs_max_error": 32}),
+                ("learned_plr", "learned_plr", {}), ("learned_mlp", "learned_mlp", {}),
+                ("learned_plr_column", "learned_plr", {"learned_layout": _lib.LAYOUT_COLUMN}),
+                ("learned_mlp_column", "learned_mlp", {"learned_layout": _lib.LAYOUT_COLUMN})]
     for dist, dname in ((_lib.KEYS_UNIFORM, "uniform"), (_lib.KEYS_LOGNORMAL, "lognormal"),
                         (_lib.KEYS_CLUSTERED, "clustered")):
         k4 = gj.gen_build_keys(1 << 26, 8, dist, 63, 0x31 + dist)
@@ -393,8 +400,7 @@ def extras(gj, G, stream, keys64, d_keys64, d_out, cpu=True, steps=5, warmup=2):
         dp4 = gj.DeviceArray.from_numpy(p4, stream)
         del p4
         rows = {}
-        for kind, prm in variants:
-            label = kind + (f"_err{prm['rs_max_error']}" if prm else "")
+        for label, kind, prm in variants:
             t = time.time()
             idx = gj.Index(kind, dk4, host_column=k4, params=gj.index_params(**prm), stream=stream)
             build_s = time.time() - t
@@ -407,12 +413,13 @@ def extras(gj, G, stream, keys64, d_keys64, d_out, cpu=True, steps=5, warmup=2):
                            "fallback_leaves": st["fallback_leaves"], "avg_window": st["avg_window"],
                            "max_window": st["max_window"], "train_loss": st["train_loss"],
                            "index_bytes": st["memory_bytes"], "layout": st["layout"], "leaves": st["leaves"],
-                           "avg_lines": st["avg_lines"], "max_lines": st["max_lines"]}
+                           "avg_lines": st["avg_lines"], "max_lines": st["max_lines"],
+                           "lds_levels": st["lds_levels"]}
             idx.close()
             log(f"C4 {dname} {label}: {m / ms * 1e-6:.2f} G lookups/s match={ok} fallback={st['fallback_keys']} "
                 f"layout={st['layout']} leaves={st['leaves']} avg_lines={st['avg_lines']:.3f}")
         bt = rows["btree"]["lookups_per_s"]
-        for label in ("learned_plr", "learned_mlp"):
+        for label in ("learned_plr", "learned_mlp", "learned_plr_column", "learned_mlp_column"):
             rows[label]["vs_btree"] = rows[label]["lookups_per_s"] / bt
         c4["distributions"][dname] = rows
         del dk4, dp4, k4, e4
@@ -516,7 +523,8 @@ def join_aggregate_leg(gj, stream, cpu=True, fact_rows=60_000_000, reps=6):
     res = gj.DeviceArray(4, np.int64)
     for kind in ("learned_plr", "btree", "hash"):
         idx = gj.Index(kind, dkey, host_column=skey, stream=stream)
-        for strategy in (("lane_refill", "plain", "partitioned") if kind == "learned_plr" else ("lane_refill",)):
+        for strategy in (("lane_refill", "plain", "partitioned", "sorted") if kind == "learned_plr"
+                         else ("lane_refill",)):
             ok = True
             for c in copies:  # warm-up and check on both copies
                 res.fill(0, stream)

@@ -42,8 +42,9 @@ LAYOUT_AUTO, LAYOUT_COLUMN, LAYOUT_LINES = range(3)
 # gj_index_params.placement
 PLACE_DEVICE, PLACE_HOST = range(2)
 # gj_agg_strategy
-AGG_LANE_REFILL, AGG_PLAIN, AGG_PARTITIONED = range(3)
-AGG_STRATEGIES = {"lane_refill": AGG_LANE_REFILL, "plain": AGG_PLAIN, "partitioned": AGG_PARTITIONED}
+AGG_LANE_REFILL, AGG_PLAIN, AGG_PARTITIONED, AGG_SORTED = range(4)
+AGG_STRATEGIES = {"lane_refill": AGG_LANE_REFILL, "plain": AGG_PLAIN, "partitioned": AGG_PARTITIONED,
+                  "sorted": AGG_SORTED}
 
 
 class IndexParams(C.Structure):

@@ -270,6 +270,153 @@ __global__ __launch_bounds__(kAggBlock) void join_agg_kernel(Dev d, FactBatchDev
   }
 }
 
+// The blockwise-sorted strategy (ij_pbws, src/tpch/tpch_14_ij.cu:93-362): a
+// workgroup collects its qualifying rows (dimension key, summand) in an LDS buffer
+// of kAggSortTile slots, sorts the buffered keys (bitonic network over
+// key << 32 | slot), and looks them up in key order, 64 neighbouring keys per wave
+// step, so a wave's lanes walk neighbouring index paths and share lines; then it
+// refills.  Rows are taken as by join_agg_kernel (64 x kAggSortUnroll per wave
+// step, grid stride).  Slots are claimed with one LDS atomic per wave and 64-row
+// slice.  A wave starts a step only while at most kAggSortTile - (waves per
+// workgroup) x (rows per step) slots are taken: at most one step per wave is then
+// in flight past that point, so the buffer never overflows.
+constexpr int kAggSortTile = 2048;
+constexpr int kAggSortUnroll = 4;
+
+template <class Dev>
+__global__ __launch_bounds__(kAggBlock) void join_agg_sorted_kernel(Dev d, FactBatchDev b, int32_t lo, int32_t hi,
+                                                                    const uint8_t* __restrict__ flag,
+                                                                    AggPartial* __restrict__ partial, bool direct) {
+  constexpr int U = kAggSortUnroll;
+  constexpr int kWaves = kAggBlock / kWave;
+  constexpr int kSpan = kWave * U;
+  static_assert(kWaves * kSpan <= kAggSortTile / 2, "a fill round must fit the buffer's second half");
+  extern __shared__ __attribute__((aligned(16))) uint8_t lds[];
+  __shared__ int64_t red[kWaves][3];
+  __shared__ int taken;
+  d.stage(lds);
+  const size_t ilds = (d.lds_bytes() + 15) / 16 * 16;
+  uint64_t* ent = reinterpret_cast<uint64_t*>(lds + ilds);                      // key << 32 | slot
+  int64_t* val = reinterpret_cast<int64_t*>(lds + ilds + kAggSortTile * 8);     // summand by slot
+  if (threadIdx.x == 0) taken = 0;
+  __syncthreads();
+  const int ln = lane();
+  const uint64_t lt = (ln == 0) ? 0ull : (~0ull >> (64 - ln));
+  const int w = threadIdx.x / kWave;
+  const uint64_t step = static_cast<uint64_t>(gridDim.x) * kWaves * kSpan;
+  uint64_t base = (static_cast<uint64_t>(blockIdx.x) * kWaves + w) * kSpan;
+  AggAcc acc;
+  for (;;) {
+    while (base < b.rows &&
+           __hip_atomic_load(&taken, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) <= kAggSortTile - kWaves * kSpan) {
+      int32_t dt[U];
+#pragma unroll
+      for (int k = 0; k < U; ++k) {
+        const uint64_t i = base + static_cast<uint64_t>(k * kWave + ln);
+        dt[k] = i < b.rows ? load_stream(b.date + i) : 0;
+      }
+      uint32_t qm = 0;
+#pragma unroll
+      for (int k = 0; k < U; ++k) {
+        const uint64_t i = base + static_cast<uint64_t>(k * kWave + ln);
+        qm |= ((i < b.rows && dt[k] >= lo && dt[k] < hi) ? 1u : 0u) << k;
+      }
+      if (ballot(qm != 0) != 0) {  // wave-uniform
+        uint32_t fk[U];
+        int64_t sv[U];
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+          fk[k] = 0;
+          sv[k] = 0;
+          if ((qm >> k) & 1u) {
+            const uint64_t i = base + static_cast<uint64_t>(k * kWave + ln);
+            fk[k] = static_cast<uint32_t>(b.fkey[i]);
+            sv[k] = b.price[i] * (100 - static_cast<int64_t>(b.disc[i]));
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+          const bool qual = (qm >> k) & 1u;
+          const uint64_t mask = ballot(qual);
+          if (mask == 0) continue;  // wave-uniform
+          int first = 0;
+          if (ln == 0) first = atomicAdd(&taken, __popcll(mask));
+          first = __shfl(first, 0, kWave);
+          if (qual) {
+            const int slot = first + __popcll(mask & lt);
+            ent[slot] = (static_cast<uint64_t>(fk[k]) << 32) | static_cast<uint32_t>(slot);
+            val[slot] = sv[k];
+          }
+        }
+      }
+      base += step;
+    }
+    __syncthreads();  // every wave has stopped filling: `taken` is final for this round
+    const int total = taken;
+    const bool more = __syncthreads_or(base < b.rows) != 0;
+    if (total > 0) {  // workgroup-uniform
+      int n2 = kWave;
+      while (n2 < total) n2 <<= 1;
+      for (int t = total + threadIdx.x; t < n2; t += kAggBlock) ent[t] = ~0ull;
+      __syncthreads();
+      for (int k = 2; k <= n2; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+          for (int t = threadIdx.x; t < n2 / 2; t += kAggBlock) {
+            const int a = 2 * t - (t & (j - 1));  // lower element of the pair (a, a + j)
+            const uint64_t ea = ent[a], eb = ent[a + j];
+            if ((ea > eb) == ((a & k) == 0)) {
+              ent[a] = eb;
+              ent[a + j] = ea;
+            }
+          }
+          __syncthreads();
+        }
+      }
+      for (int t0 = w * kWave; t0 < total; t0 += kAggBlock) {  // wave-uniform bound
+        const int t = t0 + ln;
+        const bool active = t < total;
+        const uint64_t e = active ? ent[t] : 0ull;
+        const uint32_t key = static_cast<uint32_t>(e >> 32);
+        const int64_t sv = active ? val[static_cast<uint32_t>(e)] : 0;
+        const uint32_t r = d.lookup(key, active, lds);
+        if (active && r != kNotFound) {
+          acc.den += sv;
+          if (flag[r]) acc.num += sv;
+          acc.matches += 1;
+        }
+      }
+      __syncthreads();  // the buffer is read out before it refills
+      if (threadIdx.x == 0) taken = 0;
+      __syncthreads();
+    }
+    if (!more) break;
+  }
+  const int64_t num = wave_sum_i64(acc.num);
+  const int64_t den = wave_sum_i64(acc.den);
+  const int64_t mt = wave_sum_i64(static_cast<int64_t>(acc.matches));
+  if (ln == 0) {
+    red[w][0] = num;
+    red[w][1] = den;
+    red[w][2] = mt;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    AggPartial p{0, 0, 0, 0};
+    for (int v = 0; v < kWaves; ++v) {
+      p.num += red[v][0];
+      p.den += red[v][1];
+      p.matches += static_cast<uint64_t>(red[v][2]);
+    }
+    if (!direct) {
+      partial[blockIdx.x] = p;
+    } else if ((p.num | p.den | static_cast<int64_t>(p.matches)) != 0) {
+      atomicAdd(reinterpret_cast<unsigned long long*>(&b.result->numerator), static_cast<unsigned long long>(p.num));
+      atomicAdd(reinterpret_cast<unsigned long long*>(&b.result->denominator), static_cast<unsigned long long>(p.den));
+      atomicAdd(reinterpret_cast<unsigned long long*>(&b.result->matches), static_cast<unsigned long long>(p.matches));
+    }
+  }
+}
+
 // Adds the workgroups' partial sums to the result: one workgroup, exact int64.
 __global__ __launch_bounds__(1024) void agg_sum_kernel(const AggPartial* __restrict__ partial, uint32_t blocks,
                                                        gj_agg_result* __restrict__ out) {
@@ -579,14 +726,23 @@ namespace {
 inline uint64_t agg_partial_cap(int cu) { return 8ull * static_cast<uint64_t>(cu); }
 
 // partial holds partial_cap slots (0: agg_partial_cap(cu), what the callers' buffers are sized for).
-template <class Dev, bool Plain = false>
+// Strategy: GJ_AGG_LANE_REFILL, GJ_AGG_PLAIN or GJ_AGG_SORTED (one kernel each, plus the sum kernel).
+template <class Dev, int Strategy = GJ_AGG_LANE_REFILL>
 void launch_join_agg(const Dev& d, const FactBatchDev& b, int32_t lo, int32_t hi, const uint8_t* flag, int cu,
                      AggPartial* partial, hipStream_t s, uint64_t partial_cap = 0) {
+  static_assert(Strategy == GJ_AGG_LANE_REFILL || Strategy == GJ_AGG_PLAIN || Strategy == GJ_AGG_SORTED,
+                "one-kernel join-aggregate strategies only");
   if (b.rows == 0) return;
-  const size_t lds = (d.lds_bytes() + 15) / 16 * 16 + agg_queue_bytes(kAggUnrollBatch);
-  const void* fn = reinterpret_cast<const void*>(&join_agg_kernel<Dev, Plain>);
+  constexpr bool kSorted = Strategy == GJ_AGG_SORTED;
+  constexpr bool kPlain = Strategy == GJ_AGG_PLAIN;
+  const size_t ilds = (d.lds_bytes() + 15) / 16 * 16;
+  const size_t lds = ilds + (kSorted ? static_cast<size_t>(kAggSortTile) * 16
+                                     : (kPlain ? 0 : agg_queue_bytes(kAggUnrollBatch)));
+  const void* fn = kSorted ? reinterpret_cast<const void*>(&join_agg_sorted_kernel<Dev>)
+                           : reinterpret_cast<const void*>(&join_agg_kernel<Dev, kPlain>);
+  const uint64_t rows_per_step = static_cast<uint64_t>(kAggBlock) * (kSorted ? kAggSortUnroll : kAggUnrollBatch);
   uint64_t blocks = static_cast<uint64_t>(blocks_per_cu(fn, kAggBlock, lds)) * cu;
-  blocks = std::min<uint64_t>(blocks, ceil_div(b.rows, static_cast<uint64_t>(kAggBlock) * kAggUnrollBatch));
+  blocks = std::min<uint64_t>(blocks, ceil_div(b.rows, rows_per_step));
   blocks = std::max<uint64_t>(blocks, 1);
   if (partial_cap == 0) {
     GJ_REQUIRE(blocks <= agg_partial_cap(cu), GJ_ERR_INTERNAL, "join-aggregate grid exceeds the partial buffer");
@@ -597,12 +753,29 @@ void launch_join_agg(const Dev& d, const FactBatchDev& b, int32_t lo, int32_t hi
   // a small grid adds its workgroup sums to the result itself (a few dozen atomics
   // cost less than the second launch)
   const bool direct = blocks <= kAggDirectBlocks;
-  hipLaunchKernelGGL((join_agg_kernel<Dev, Plain>), dim3(static_cast<unsigned>(blocks)), dim3(kAggBlock), lds, s, d, b,
-                     lo, hi, flag, partial, direct);
+  if constexpr (kSorted) {
+    hipLaunchKernelGGL((join_agg_sorted_kernel<Dev>), dim3(static_cast<unsigned>(blocks)), dim3(kAggBlock), lds, s, d,
+                       b, lo, hi, flag, partial, direct);
+  } else {
+    hipLaunchKernelGGL((join_agg_kernel<Dev, kPlain>), dim3(static_cast<unsigned>(blocks)), dim3(kAggBlock), lds, s,
+                       d, b, lo, hi, flag, partial, direct);
+  }
   GJ_HIP_CHECK(hipGetLastError());
   if (direct) return;
   hipLaunchKernelGGL(agg_sum_kernel, dim3(1), dim3(1024), 0, s, partial, static_cast<uint32_t>(blocks), b.result);
   GJ_HIP_CHECK(hipGetLastError());
+}
+
+// The one-kernel strategies by their run-time value.
+template <class Dev>
+void launch_join_agg_strategy(uint32_t strategy, const Dev& d, const FactBatchDev& b, int32_t lo, int32_t hi,
+                              const uint8_t* flag, int cu, AggPartial* partial, hipStream_t s,
+                              uint64_t partial_cap = 0) {
+  switch (strategy) {
+    case GJ_AGG_PLAIN: launch_join_agg<Dev, GJ_AGG_PLAIN>(d, b, lo, hi, flag, cu, partial, s, partial_cap); return;
+    case GJ_AGG_SORTED: launch_join_agg<Dev, GJ_AGG_SORTED>(d, b, lo, hi, flag, cu, partial, s, partial_cap); return;
+    default: launch_join_agg<Dev, GJ_AGG_LANE_REFILL>(d, b, lo, hi, flag, cu, partial, s, partial_cap); return;
+  }
 }
 
 // Workgroup partial sums of gj_join_aggregate_noalloc, which must not allocate: a
@@ -956,7 +1129,7 @@ int gj_join_aggregate(const gj_index* index, const gj_fact_batch* batch, const g
     check_agg_index(*index);
     GJ_REQUIRE(params->d_flag, GJ_ERR_INVALID, "null flag column");
     const FactBatchDev b = to_dev(*batch, d_result);
-    GJ_REQUIRE(params->strategy <= GJ_AGG_PARTITIONED, GJ_ERR_INVALID, "unknown join-aggregate strategy");
+    GJ_REQUIRE(params->strategy <= GJ_AGG_SORTED, GJ_ERR_INVALID, "unknown join-aggregate strategy");
     const hipStream_t s = S(stream);
     with_index_dev<uint32_t>(*index, [&](auto d) {
       if (params->strategy == GJ_AGG_PARTITIONED) {
@@ -965,12 +1138,8 @@ int gj_join_aggregate(const gj_index* index, const gj_fact_batch* batch, const g
       }
       dbuf& partial = index->scratch_for(s).a;
       partial.reserve(agg_partial_cap(index->cu) * sizeof(AggPartial));
-      if (params->strategy == GJ_AGG_PLAIN)
-        launch_join_agg<decltype(d), true>(d, b, params->date_lo, params->date_hi, params->d_flag, index->cu,
-                                           partial.as<AggPartial>(), s);
-      else
-        launch_join_agg(d, b, params->date_lo, params->date_hi, params->d_flag, index->cu, partial.as<AggPartial>(),
-                        s);
+      launch_join_agg_strategy(params->strategy, d, b, params->date_lo, params->date_hi, params->d_flag, index->cu,
+                               partial.as<AggPartial>(), s);
     });
   });
 }
@@ -981,18 +1150,14 @@ int gj_join_aggregate_noalloc(const gj_index* index, const gj_fact_batch* batch,
     GJ_REQUIRE(index && batch && params && d_result, GJ_ERR_INVALID, "null argument");
     check_agg_index(*index);
     GJ_REQUIRE(params->d_flag, GJ_ERR_INVALID, "null flag column");
-    GJ_REQUIRE(params->strategy <= GJ_AGG_PARTITIONED, GJ_ERR_INVALID, "unknown join-aggregate strategy");
+    GJ_REQUIRE(params->strategy <= GJ_AGG_SORTED, GJ_ERR_INVALID, "unknown join-aggregate strategy");
     GJ_REQUIRE(params->strategy != GJ_AGG_PARTITIONED, GJ_ERR_UNSUPPORTED,
                "the partitioned strategy needs scratch memory; use gj_join_aggregate");
     const FactBatchDev b = to_dev(*batch, d_result);
     AggPartial* partial = noalloc_partials();
     with_index_dev<uint32_t>(*index, [&](auto d) {
-      if (params->strategy == GJ_AGG_PLAIN)
-        launch_join_agg<decltype(d), true>(d, b, params->date_lo, params->date_hi, params->d_flag, index->cu,
-                                           partial, S(stream), kNoAllocPartials);
-      else
-        launch_join_agg(d, b, params->date_lo, params->date_hi, params->d_flag, index->cu, partial, S(stream),
-                        kNoAllocPartials);
+      launch_join_agg_strategy(params->strategy, d, b, params->date_lo, params->date_hi, params->d_flag, index->cu,
+                               partial, S(stream), kNoAllocPartials);
     });
   });
 }

@@ -184,8 +184,9 @@ void build_rs(gj_index& ix, hipStream_t s) {
   ix.rs_points = rs.knot_x.size();
   ix.rs_radix.alloc(rs.radix.size() * sizeof(uint32_t));
   GJ_HIP_CHECK(hipMemcpyAsync(ix.rs_radix.p, rs.radix.data(), ix.rs_radix.bytes, hipMemcpyHostToDevice, s));
-  // staged in LDS when the table has at most 2^rs_lds_bits + 2 entries
-  ix.rs_lds = rs.radix.size() <= (1ull << std::min<uint32_t>(ix.params.rs_lds_bits, 16)) + 2;
+  // staged in LDS when the table has at most 2^rs_lds_bits + 2 entries (2^14: 64 KiB, a
+  // workgroup's LDS)
+  ix.rs_lds = rs.radix.size() <= (1ull << std::min<uint32_t>(ix.params.rs_lds_bits, 14)) + 2;
   ix.stats.lds_levels = ix.rs_lds ? 1 : 0;
   std::vector<RsKnot> kn(rs.knot_x.size());
   for (size_t i = 0; i < kn.size(); ++i) kn[i] = RsKnot{static_cast<uint64_t>(rs.knot_x[i]), rs.knot_y[i]};

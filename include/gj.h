@@ -99,7 +99,8 @@ typedef struct gj_index_params {
   uint32_t learned_line_keys;     /* lines layout: build keys per 128-byte line on average; default 10 of the
                                      15 a line holds for 8-byte keys, 20 of 30 for 4-byte keys */
   uint32_t rs_lds_bits;           /* RadixSpline: the radix table is staged in LDS when rs_radix_bits <= this
-                                     (at most 16), else read from global memory; default 13 (32 KiB) */
+                                     (at most 14, a 64 KiB table), else read from global memory; default 13
+                                     (32 KiB) */
   uint32_t placement;             /* GJ_PLACE_DEVICE (default) or GJ_PLACE_HOST: where the index's own
                                      structures live (the column is the caller's and stays where it is) */
 } gj_index_params;
@@ -293,8 +294,16 @@ typedef struct gj_fact_batch {
  *   GJ_AGG_PARTITIONED  compact the qualifying (fkey, summand) pairs, radix-partition them on the
  *                       fkey, then look the partitions up in key order and sum (ij_partitioning,
  *                       src/tpch/tpch_14_common.cu:651-704, src/tpch/tpch_14_ij_partitioning.cu:187-237);
- *                       synchronises the stream once, after the filter counts the qualifying rows */
-typedef enum gj_agg_strategy { GJ_AGG_LANE_REFILL = 0, GJ_AGG_PLAIN = 1, GJ_AGG_PARTITIONED = 2 } gj_agg_strategy;
+ *                       synchronises the stream once, after the filter counts the qualifying rows
+ *   GJ_AGG_SORTED       each workgroup buffers its qualifying rows in LDS (2048 slots), sorts them by
+ *                       fkey and looks them up in key order, 64 neighbouring keys per wave step
+ *                       (ij_pbws, the pipelined blockwise sort, src/tpch/tpch_14_ij.cu:93-362) */
+typedef enum gj_agg_strategy {
+  GJ_AGG_LANE_REFILL = 0,
+  GJ_AGG_PLAIN = 1,
+  GJ_AGG_PARTITIONED = 2,
+  GJ_AGG_SORTED = 3
+} gj_agg_strategy;
 
 typedef struct gj_agg_params {
   int32_t date_lo;        /* inclusive */

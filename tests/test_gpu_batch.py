@@ -30,7 +30,7 @@ def _agg_index(gj, kind, skey):
     return col, gj.Index(kind, col, host_column=skey, params=gj.index_params(learned_leaf_bits=10, mlp_steps=100))
 
 
-@pytest.mark.parametrize("strategy", ["lane_refill", "plain", "partitioned"])
+@pytest.mark.parametrize("strategy", ["lane_refill", "plain", "partitioned", "sorted"])
 @pytest.mark.parametrize("kind", AGG_KINDS)
 def test_join_aggregate_matches_oracle(gj, orc, gpu, kind, strategy):
     print(f"[case] join-aggregate {kind} {strategy}", flush=True)
@@ -99,13 +99,36 @@ def test_join_aggregate_small_grids(gj, orc, gpu, rows):
     d = [gj.DeviceArray.from_numpy(a) for a in (fkey, date, price, disc)]
     dflag = gj.DeviceArray.from_numpy(sflag)
     res = gj.DeviceArray(4, np.int64)
-    for rep in range(3):  # every call adds to the earlier calls' result
+    strategies = ("lane_refill", "plain", "partitioned", "sorted")
+    for rep, strategy in enumerate(strategies):  # every call adds to the earlier calls' result
         if rep == 0:
             res.fill(0)
-        gj.join_aggregate(idx, *d, dflag, lo, hi, res, strategy=("lane_refill", "plain", "partitioned")[rep])
+        gj.join_aggregate(idx, *d, dflag, lo, hi, res, strategy=strategy)
     gj.synchronize()
     onum, oden, omt = orc.join_aggregate(skey, sflag, fkey, date, price, disc, lo, hi)
-    assert tuple(int(v) for v in res.numpy()[:3]) == (3 * onum, 3 * oden, 3 * omt)
+    k = len(strategies)
+    assert tuple(int(v) for v in res.numpy()[:3]) == (k * onum, k * oden, k * omt)
+
+
+@pytest.mark.parametrize("kind", ["btree", "learned_plr", "radix_spline"])
+@pytest.mark.parametrize("lo,hi", [(0, 2 ** 31 - 1), (0, 1300), (2400, 2500)])
+def test_join_aggregate_sorted_refills(gj, orc, gpu, kind, lo, hi):
+    # every row, about half of them, and a few percent qualify: the sorted strategy's
+    # workgroups refill and sort their 2048-slot buffers many times, once, or never fill
+    print(f"[case] sorted join-aggregate {kind} window=[{lo}, {hi})", flush=True)
+    dim_rows = 32 * 3000
+    skey, sflag = _dimension(gj, dim_rows, seed=0x4B)
+    fkey, date, price, disc = _fact(gj, 3_000_017, dim_rows + dim_rows // 7, seed=0x4C)
+    col, idx = _agg_index(gj, kind, skey)
+    d = [gj.DeviceArray.from_numpy(a) for a in (fkey, date, price, disc)]
+    dflag = gj.DeviceArray.from_numpy(sflag)
+    res = gj.DeviceArray(4, np.int64)
+    res.fill(0)
+    gj.join_aggregate(idx, *d, dflag, lo, hi, res, strategy="sorted")
+    gj.synchronize()
+    want = orc.join_aggregate(skey, sflag, fkey, date, price, disc, lo, hi)
+    assert tuple(int(v) for v in res.numpy()[:3]) == tuple(want)
+    assert want[2] > 0
 
 
 def test_join_aggregate_all_rows_closed_form(gj, gpu):

@@ -35,7 +35,7 @@ def test_noalloc_join_aggregate_matches_allocating_one(gj, orc, gpu, kind):
         cols_h = gj.gen_fact(rows, dim_rows, 2557, 90000, 10500000, 10, seed)
         cols = [gj.DeviceArray.from_numpy(a) for a in cols_h]
         want = orc.join_aggregate(skey, sflag, *cols_h, lo, hi)
-        for strategy in (_lib.AGG_LANE_REFILL, _lib.AGG_PLAIN):
+        for strategy in (_lib.AGG_LANE_REFILL, _lib.AGG_PLAIN, _lib.AGG_SORTED):
             res_a = gj.DeviceArray(4, np.int64)
             res_b = gj.DeviceArray(4, np.int64)
             res_a.fill(0)
