@@ -523,7 +523,7 @@ def join_aggregate_leg(gj, stream, cpu=True, fact_rows=60_000_000, reps=6):
     res = gj.DeviceArray(4, np.int64)
     for kind in ("learned_plr", "btree", "hash"):
         idx = gj.Index(kind, dkey, host_column=skey, stream=stream)
-        for strategy in (("lane_refill", "plain", "partitioned", "sorted") if kind == "learned_plr"
+        for strategy in (("lane_refill", "plain", "partitioned", "sorted", "lookup_first") if kind == "learned_plr"
                          else ("lane_refill",)):
             ok = True
             for c in copies:  # warm-up and check on both copies

@@ -356,7 +356,7 @@ def join_aggregate(index: Index, fkey: DeviceArray, date: DeviceArray, price: De
                    flag: DeviceArray, date_lo: int, date_hi: int, result: DeviceArray, rows: Optional[int] = None,
                    offset: int = 0, stream: Optional[Stream] = None, strategy="lane_refill", radix_bits: int = 0):
     """Adds the Q14-shaped sums of the fact rows to `result`; strategy is "lane_refill" (default),
-    "plain", "partitioned" or "sorted" (gj_agg_strategy)."""
+    "plain", "partitioned", "sorted" or "lookup_first" (gj_agg_strategy)."""
     r = fkey.count if rows is None else int(rows)
     fb = FactBatch(fkey.offset(offset), date.offset(offset), price.offset(offset), disc.offset(offset), r)
     st = _lib.AGG_STRATEGIES[strategy] if isinstance(strategy, str) else int(strategy)

@@ -42,9 +42,9 @@ LAYOUT_AUTO, LAYOUT_COLUMN, LAYOUT_LINES = range(3)
 # gj_index_params.placement
 PLACE_DEVICE, PLACE_HOST = range(2)
 # gj_agg_strategy
-AGG_LANE_REFILL, AGG_PLAIN, AGG_PARTITIONED, AGG_SORTED = range(4)
+AGG_LANE_REFILL, AGG_PLAIN, AGG_PARTITIONED, AGG_SORTED, AGG_LOOKUP_FIRST = range(5)
 AGG_STRATEGIES = {"lane_refill": AGG_LANE_REFILL, "plain": AGG_PLAIN, "partitioned": AGG_PARTITIONED,
-                  "sorted": AGG_SORTED}
+                  "sorted": AGG_SORTED, "lookup_first": AGG_LOOKUP_FIRST}
 
 
 class IndexParams(C.Structure):

@@ -659,6 +659,104 @@ __global__ __launch_bounds__(kAggBlock) void agg_filter_write_kernel(FactBatchDe
   }
 }
 
+// The lookup-first strategy, first pass (ij_lookup_kernel of ij_np_lf_plain,
+// src/tpch/tpch_14_ij.cu:371-407, with the correct date predicate): chunk c's
+// qualifying rows are looked up in place, one per lane, and every match is
+// written as a join entry (fact row, dimension row) from position offsets[c] on;
+// matched[c] receives the chunk's match count.  (The reference reserves entries with
+// one global atomic counter; here the filter count has sized every chunk's region.)
+template <class Dev>
+__global__ __launch_bounds__(kAggBlock) void agg_lookup_entries_kernel(Dev d, FactBatchDev b, int32_t lo, int32_t hi,
+                                                                       const uint32_t* __restrict__ offsets,
+                                                                       uint32_t* __restrict__ matched,
+                                                                       uint32_t* __restrict__ ent_row,
+                                                                       uint32_t* __restrict__ ent_dim) {
+  extern __shared__ __attribute__((aligned(16))) uint8_t lds[];
+  __shared__ uint32_t wc[kAggBlock / kWave];
+  d.stage(lds);
+  __syncthreads();
+  const uint64_t c0 = static_cast<uint64_t>(blockIdx.x) * kFilterChunk;
+  const uint64_t c1 = c0 + kFilterChunk < b.rows ? c0 + kFilterChunk : b.rows;
+  const int w = threadIdx.x / kWave;
+  const int ln = lane();
+  const uint64_t lt = (ln == 0) ? 0ull : (~0ull >> (64 - ln));
+  const uint32_t first = offsets[blockIdx.x];
+  uint32_t run = first;
+  for (uint64_t t0 = c0; t0 < c1; t0 += blockDim.x) {
+    const uint64_t i = t0 + threadIdx.x;
+    bool qual = false;
+    if (i < c1) {
+      const int32_t dt = load_stream(b.date + i);
+      qual = dt >= lo && dt < hi;
+    }
+    bool match = false;
+    uint32_t r = kNotFound;
+    if (ballot(qual) != 0) {  // wave-uniform
+      const uint32_t k = qual ? static_cast<uint32_t>(b.fkey[i]) : 0u;
+      r = d.lookup(k, qual, lds);
+      match = qual && r != kNotFound;
+    }
+    const uint64_t mask = ballot(match);
+    if (ln == 0) wc[w] = static_cast<uint32_t>(__popcll(mask));
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+    for (int v = 0; v < kAggBlock / kWave; ++v) {
+      before += v < w ? wc[v] : 0u;
+      total += wc[v];
+    }
+    if (match) {
+      const uint32_t at = run + before + static_cast<uint32_t>(__popcll(mask & lt));
+      ent_row[at] = static_cast<uint32_t>(i);
+      ent_dim[at] = r;
+    }
+    run += total;
+    __syncthreads();  // wc is rewritten by the next tile
+  }
+  if (threadIdx.x == 0) matched[blockIdx.x] = run - first;
+}
+
+// The lookup-first strategy, second pass (ij_join_kernel, src/tpch/tpch_14_ij.cu:416-461):
+// chunk c's join entries gather their price and discount by fact row and their
+// flag by dimension row; the workgroup's sums go to partial[c].
+__global__ __launch_bounds__(kAggBlock) void agg_join_entries_kernel(FactBatchDev b,
+                                                                     const uint32_t* __restrict__ offsets,
+                                                                     const uint32_t* __restrict__ matched,
+                                                                     const uint32_t* __restrict__ ent_row,
+                                                                     const uint32_t* __restrict__ ent_dim,
+                                                                     const uint8_t* __restrict__ flag,
+                                                                     AggPartial* __restrict__ partial) {
+  __shared__ int64_t red[kAggBlock / kWave][3];
+  const uint64_t e0 = offsets[blockIdx.x], e1 = e0 + matched[blockIdx.x];
+  AggAcc acc;
+  for (uint64_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+    const uint32_t row = load_stream(ent_row + e);
+    const uint32_t dim = load_stream(ent_dim + e);
+    const int64_t sv = b.price[row] * (100 - static_cast<int64_t>(b.disc[row]));
+    acc.den += sv;
+    if (flag[dim]) acc.num += sv;
+    acc.matches += 1;
+  }
+  const int w = threadIdx.x / kWave;
+  const int64_t num = wave_sum_i64(acc.num);
+  const int64_t den = wave_sum_i64(acc.den);
+  const int64_t mt = wave_sum_i64(static_cast<int64_t>(acc.matches));
+  if (lane() == 0) {
+    red[w][0] = num;
+    red[w][1] = den;
+    red[w][2] = mt;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    AggPartial p{0, 0, 0, 0};
+    for (int v = 0; v < kAggBlock / kWave; ++v) {
+      p.num += red[v][0];
+      p.den += red[v][1];
+      p.matches += static_cast<uint64_t>(red[v][2]);
+    }
+    partial[blockIdx.x] = p;
+  }
+}
+
 // Looks up partitioned (fkey, position) tuples and sums the matches' summands; the
 // sweep over the tuples is lookup_tuples_kernel's (each XCD one contiguous eighth,
 // its workgroups in step).  Padding slots carry position 0xFFFFFFFF.
@@ -885,6 +983,59 @@ void join_aggregate_partitioned(const gj_index& ix, const Dev& d, const FactBatc
                      tpos, start + fan, cap, cval, p.d_flag, partial);
   GJ_HIP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(agg_sum_kernel, dim3(1), dim3(1024), 0, s, partial, static_cast<uint32_t>(blocks), b.result);
+  GJ_HIP_CHECK(hipGetLastError());
+}
+
+// The lookup-first strategy: filter count, scan, then the two passes above and the
+// sum of the chunks' partials.  Like the partitioned strategy it synchronises the
+// stream once, to size the join entries by the qualifying rows.
+template <class Dev>
+void join_aggregate_lookup_first(const gj_index& ix, const Dev& d, const FactBatchDev& b, const gj_agg_params& p,
+                                 hipStream_t s) {
+  if (b.rows == 0) return;
+  GJ_REQUIRE(b.rows < (1ull << 32), GJ_ERR_INVALID, "the lookup-first strategy takes at most 2^32 - 1 fact rows");
+  const uint64_t chunks = ceil_div(b.rows, kFilterChunk);
+  auto up = [](uint64_t v) { return (v + 255) / 256 * 256; };
+  dbuf& sc = ix.scratch_for(s).a;
+  sc.reserve(256 + up(chunks * 4));
+  auto* total = sc.as<uint64_t>();
+  auto* offs = reinterpret_cast<uint32_t*>(sc.as<uint8_t>() + 256);
+  hipLaunchKernelGGL(agg_filter_count_kernel, dim3(static_cast<unsigned>(chunks)), dim3(kAggBlock), 0, s, b.date,
+                     b.rows, p.date_lo, p.date_hi, offs);
+  GJ_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(agg_scan_counts_kernel, dim3(1), dim3(1024), 0, s, offs, chunks, total);
+  GJ_HIP_CHECK(hipGetLastError());
+  uint64_t m = 0;
+  GJ_HIP_CHECK(hipMemcpyAsync(&m, total, sizeof(m), hipMemcpyDeviceToHost, s));
+  GJ_HIP_CHECK(hipStreamSynchronize(s));
+  if (m == 0) return;
+  // total | offsets | match counts | entry rows | entry dimension rows | chunk partials
+  const uint64_t o_off = 256, o_mt = o_off + up(chunks * 4), o_er = o_mt + up(chunks * 4), o_ed = o_er + up(m * 4),
+                 o_pa = o_ed + up(m * 4), bytes = o_pa + up(chunks * sizeof(AggPartial));
+  if (bytes > sc.bytes) {
+    // grow and redo the count into the new buffer (dbuf::reserve does not keep contents)
+    sc.reserve(bytes);
+    total = sc.as<uint64_t>();
+    offs = reinterpret_cast<uint32_t*>(sc.as<uint8_t>() + 256);
+    hipLaunchKernelGGL(agg_filter_count_kernel, dim3(static_cast<unsigned>(chunks)), dim3(kAggBlock), 0, s, b.date,
+                       b.rows, p.date_lo, p.date_hi, offs);
+    GJ_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(agg_scan_counts_kernel, dim3(1), dim3(1024), 0, s, offs, chunks, total);
+    GJ_HIP_CHECK(hipGetLastError());
+  }
+  uint8_t* base = sc.as<uint8_t>();
+  auto* matched = reinterpret_cast<uint32_t*>(base + o_mt);
+  auto* erow = reinterpret_cast<uint32_t*>(base + o_er);
+  auto* edim = reinterpret_cast<uint32_t*>(base + o_ed);
+  auto* partial = reinterpret_cast<AggPartial*>(base + o_pa);
+  const size_t lds = (d.lds_bytes() + 15) / 16 * 16;
+  hipLaunchKernelGGL((agg_lookup_entries_kernel<Dev>), dim3(static_cast<unsigned>(chunks)), dim3(kAggBlock), lds, s,
+                     d, b, p.date_lo, p.date_hi, offs, matched, erow, edim);
+  GJ_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(agg_join_entries_kernel, dim3(static_cast<unsigned>(chunks)), dim3(kAggBlock), 0, s, b, offs,
+                     matched, erow, edim, p.d_flag, partial);
+  GJ_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(agg_sum_kernel, dim3(1), dim3(1024), 0, s, partial, static_cast<uint32_t>(chunks), b.result);
   GJ_HIP_CHECK(hipGetLastError());
 }
 
@@ -1129,11 +1280,15 @@ int gj_join_aggregate(const gj_index* index, const gj_fact_batch* batch, const g
     check_agg_index(*index);
     GJ_REQUIRE(params->d_flag, GJ_ERR_INVALID, "null flag column");
     const FactBatchDev b = to_dev(*batch, d_result);
-    GJ_REQUIRE(params->strategy <= GJ_AGG_SORTED, GJ_ERR_INVALID, "unknown join-aggregate strategy");
+    GJ_REQUIRE(params->strategy <= GJ_AGG_LOOKUP_FIRST, GJ_ERR_INVALID, "unknown join-aggregate strategy");
     const hipStream_t s = S(stream);
     with_index_dev<uint32_t>(*index, [&](auto d) {
       if (params->strategy == GJ_AGG_PARTITIONED) {
         join_aggregate_partitioned(*index, d, b, *params, s);
+        return;
+      }
+      if (params->strategy == GJ_AGG_LOOKUP_FIRST) {
+        join_aggregate_lookup_first(*index, d, b, *params, s);
         return;
       }
       dbuf& partial = index->scratch_for(s).a;
@@ -1150,9 +1305,9 @@ int gj_join_aggregate_noalloc(const gj_index* index, const gj_fact_batch* batch,
     GJ_REQUIRE(index && batch && params && d_result, GJ_ERR_INVALID, "null argument");
     check_agg_index(*index);
     GJ_REQUIRE(params->d_flag, GJ_ERR_INVALID, "null flag column");
-    GJ_REQUIRE(params->strategy <= GJ_AGG_SORTED, GJ_ERR_INVALID, "unknown join-aggregate strategy");
-    GJ_REQUIRE(params->strategy != GJ_AGG_PARTITIONED, GJ_ERR_UNSUPPORTED,
-               "the partitioned strategy needs scratch memory; use gj_join_aggregate");
+    GJ_REQUIRE(params->strategy <= GJ_AGG_LOOKUP_FIRST, GJ_ERR_INVALID, "unknown join-aggregate strategy");
+    GJ_REQUIRE(params->strategy != GJ_AGG_PARTITIONED && params->strategy != GJ_AGG_LOOKUP_FIRST, GJ_ERR_UNSUPPORTED,
+               "the partitioned and lookup-first strategies need scratch memory; use gj_join_aggregate");
     const FactBatchDev b = to_dev(*batch, d_result);
     AggPartial* partial = noalloc_partials();
     with_index_dev<uint32_t>(*index, [&](auto d) {

@@ -297,12 +297,18 @@ typedef struct gj_fact_batch {
  *                       synchronises the stream once, after the filter counts the qualifying rows
  *   GJ_AGG_SORTED       each workgroup buffers its qualifying rows in LDS (2048 slots), sorts them by
  *                       fkey and looks them up in key order, 64 neighbouring keys per wave step
- *                       (ij_pbws, the pipelined blockwise sort, src/tpch/tpch_14_ij.cu:93-362) */
+ *                       (ij_pbws, the pipelined blockwise sort, src/tpch/tpch_14_ij.cu:93-362)
+ *   GJ_AGG_LOOKUP_FIRST look up every qualifying row in place and write the matches as join entries
+ *                       (fact row, dimension row), then a second kernel gathers price, discount and
+ *                       flag per entry and sums (ij_np_lf_plain: ij_lookup_kernel + ij_join_kernel,
+ *                       src/tpch/tpch_14_ij.cu:371-461, src/tpch/tpch_14_common.cu:302-335); at most
+ *                       2^32 - 1 fact rows; synchronises the stream once, like GJ_AGG_PARTITIONED */
 typedef enum gj_agg_strategy {
   GJ_AGG_LANE_REFILL = 0,
   GJ_AGG_PLAIN = 1,
   GJ_AGG_PARTITIONED = 2,
-  GJ_AGG_SORTED = 3
+  GJ_AGG_SORTED = 3,
+  GJ_AGG_LOOKUP_FIRST = 4
 } gj_agg_strategy;
 
 typedef struct gj_agg_params {
@@ -327,7 +333,7 @@ int gj_join_aggregate(const gj_index* index, const gj_fact_batch* batch, const g
 /* gj_join_aggregate for callers that cannot allocate device memory (code under
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  No
  * counterpart in the reference.  Same arguments, semantics and results as
- * gj_join_aggregate (GJ_AGG_PARTITIONED, which needs scratch, is rejected with
+ * gj_join_aggregate (GJ_AGG_PARTITIONED and GJ_AGG_LOOKUP_FIRST, which need scratch, are rejected with
  * GJ_ERR_UNSUPPORTED), but the call allocates nothing: the workgroup partial sums
  * live in a static device array of 4096 slots (128 KiB per GPU, held until the
  * process ends) instead of index-owned scratch.  A launch uses at most 8

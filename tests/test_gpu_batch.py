@@ -30,7 +30,7 @@ def _agg_index(gj, kind, skey):
     return col, gj.Index(kind, col, host_column=skey, params=gj.index_params(learned_leaf_bits=10, mlp_steps=100))
 
 
-@pytest.mark.parametrize("strategy", ["lane_refill", "plain", "partitioned", "sorted"])
+@pytest.mark.parametrize("strategy", ["lane_refill", "plain", "partitioned", "sorted", "lookup_first"])
 @pytest.mark.parametrize("kind", AGG_KINDS)
 def test_join_aggregate_matches_oracle(gj, orc, gpu, kind, strategy):
     print(f"[case] join-aggregate {kind} {strategy}", flush=True)
@@ -52,6 +52,25 @@ def test_join_aggregate_matches_oracle(gj, orc, gpu, kind, strategy):
     assert gj.promo_revenue(num, den) == orc.promo_revenue(onum, oden)
     if strategy == "partitioned":
         assert idx.stats()["dropped_tuples"] == 0
+
+
+@pytest.mark.parametrize("rows,lo,hi", [(1, 0, 2557), (16384, 0, 2557), (16385, 0, 400), (1_000_003, 0, 2557),
+                                        (300_000, 3000, 3100)])
+def test_join_aggregate_lookup_first_chunks(gj, orc, gpu, rows, lo, hi):
+    # one or several 16384-row chunks, every row qualifying, some, or none (the result
+    # then stays as it was)
+    print(f"[case] lookup-first join-aggregate rows={rows} window=[{lo}, {hi})", flush=True)
+    dim_rows = 32 * 900
+    skey, sflag = _dimension(gj, dim_rows, seed=0x4D)
+    fkey, date, price, disc = _fact(gj, rows, dim_rows + dim_rows // 5, seed=0x4E + rows)
+    col, idx = _agg_index(gj, "btree", skey)
+    d = [gj.DeviceArray.from_numpy(a) for a in (fkey, date, price, disc)]
+    dflag = gj.DeviceArray.from_numpy(sflag)
+    res = gj.DeviceArray.from_numpy(np.array([5, 7, 11, 0], dtype=np.int64))
+    gj.join_aggregate(idx, *d, dflag, lo, hi, res, strategy="lookup_first")
+    gj.synchronize()
+    onum, oden, omt = orc.join_aggregate(skey, sflag, fkey, date, price, disc, lo, hi)
+    assert tuple(int(v) for v in res.numpy()[:3]) == (5 + onum, 7 + oden, 11 + omt)
 
 
 @pytest.mark.parametrize("bits", [1, 4, 11, 12])
@@ -99,7 +118,7 @@ def test_join_aggregate_small_grids(gj, orc, gpu, rows):
     d = [gj.DeviceArray.from_numpy(a) for a in (fkey, date, price, disc)]
     dflag = gj.DeviceArray.from_numpy(sflag)
     res = gj.DeviceArray(4, np.int64)
-    strategies = ("lane_refill", "plain", "partitioned", "sorted")
+    strategies = ("lane_refill", "plain", "partitioned", "sorted", "lookup_first")
     for rep, strategy in enumerate(strategies):  # every call adds to the earlier calls' result
         if rep == 0:
             res.fill(0)

@@ -53,13 +53,14 @@ def test_noalloc_join_aggregate_matches_allocating_one(gj, orc, gpu, kind):
 def test_noalloc_rejects_bad_arguments(gj, gpu):
     lib = _lib.load()
     assert lib.gj_join_aggregate_noalloc(None, None, None, None, None) == _lib.GJ_ERR_INVALID
-    # the partitioned strategy needs scratch memory: refused, nothing launched
+    # the partitioned and lookup-first strategies need scratch memory: refused, nothing launched
     key, flag = gj.gen_dimension(1000, 0x55)
     skey, sflag = gj.sort_dimension(key, flag)
     idx = gj.Index("btree", gj.DeviceArray.from_numpy(skey), host_column=skey)
     cols = [gj.DeviceArray.from_numpy(a) for a in gj.gen_fact(100, 1000, 2557, 90000, 10500000, 10, 0x56)]
     res = gj.DeviceArray(4, np.int64)
-    with pytest.raises(gj.GjError) as e:
-        _call(gj, "gj_join_aggregate_noalloc", idx, cols, gj.DeviceArray.from_numpy(sflag), 0, 100, res, 100,
-              _lib.AGG_PARTITIONED)
-    assert e.value.status == _lib.GJ_ERR_UNSUPPORTED
+    for strategy in (_lib.AGG_PARTITIONED, _lib.AGG_LOOKUP_FIRST):
+        with pytest.raises(gj.GjError) as e:
+            _call(gj, "gj_join_aggregate_noalloc", idx, cols, gj.DeviceArray.from_numpy(sflag), 0, 100, res, 100,
+                  strategy)
+        assert e.value.status == _lib.GJ_ERR_UNSUPPORTED
