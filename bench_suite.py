@@ -471,16 +471,19 @@ def host_placement_leg(gj, stream, keys, d_keys, d_out, mh=1 << 24):
     hcol = gj.HostArray.from_numpy(keys)
     out = {"workload": "2^26 u64 keys, 2^24 uniform probes (in HBM); index structures in page-locked host memory",
            "rows": {}}
+    # learned_plr in the default layout (the column layout when placed in host memory) and,
+    # for contrast, in the lines layout, whose key copy then sits in host memory too
     for col_name, col in (("column_in_hbm", d_keys), ("column_in_host", hcol)):
-        for kind in ("btree", "learned_plr"):
+        for kind, name, extra in (("btree", "btree", {}), ("learned_plr", "learned_plr", {}),
+                                  ("learned_plr", "learned_plr_lines", {"learned_layout": _lib.LAYOUT_LINES})):
             idx = gj.Index(kind, col, host_column=keys, stream=stream,
-                           params=gj.index_params(placement=_lib.PLACE_HOST))
+                           params=gj.index_params(placement=_lib.PLACE_HOST, **extra))
             st = idx.stats()
             for appr, kw in (("plain", {}), ("partitioning", {"radix_bits": 12, "window": mh})):
                 lp = gj.lookup_params(appr, **kw)
                 ms = _timed(gj, lambda: idx.lookup(dp, d_out, m=mh, params=lp, stream=stream), stream, 2, 1)
                 ok = bool(np.array_equal(d_out.numpy(mh), exp))
-                label = f"{kind}_{appr}_{col_name}"
+                label = f"{name}_{appr}_{col_name}"
                 out["rows"][label] = {"lookups_per_s": mh / ms * 1e3, "rowids_match": ok, "layout": st["layout"],
                                       "index_bytes_in_host": st["memory_bytes"]}
                 log(f"host placement {label}: {mh / ms * 1e-6:.3f} G lookups/s match={ok}")

@@ -903,11 +903,14 @@ void build_learned(gj_index& ix, hipStream_t s) {
   // reads more than 1.10 lines per key or up to more than 4 lines, or when the line
   // copy does not fit in memory.
   // A column below 16 MiB is served from L2 and the MALL, where the column's denser lines
-  // (16 or 32 keys against 10 or 20) win; auto keeps it.
+  // (16 or 32 keys against 10 or 20) win; auto keeps it.  So it does for structures placed
+  // in host memory: the line copy would move every key lookup onto the host link, where
+  // the column layout's model (a few MiB, cached in L2) leaves the key reads in the column.
   const uint32_t layout = ix.params.learned_layout;
   GJ_REQUIRE(layout <= GJ_LAYOUT_LINES, GJ_ERR_INVALID, "learned_layout must be 0, 1 or 2");
   constexpr uint64_t kLinesMinColumnBytes = 16ull << 20;
-  if (layout == GJ_LAYOUT_LINES || (layout == GJ_LAYOUT_AUTO && n * sizeof(K) >= kLinesMinColumnBytes)) {
+  if (layout == GJ_LAYOUT_LINES || (layout == GJ_LAYOUT_AUTO && n * sizeof(K) >= kLinesMinColumnBytes &&
+                                    ix.params.placement != GJ_PLACE_HOST)) {
     using D = LinesDev<K>;
     const uint32_t lk = ix.params.learned_line_keys ? ix.params.learned_line_keys : (sizeof(K) == 8 ? 10u : 20u);
     GJ_REQUIRE(lk >= 1 && lk <= D::CAP, GJ_ERR_INVALID, "learned_line_keys must be in [1, keys per line]");

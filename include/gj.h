@@ -126,7 +126,8 @@ typedef struct gj_index_params {
  *   GJ_LAYOUT_AUTO    lines for a column of 16 MiB or more (smaller ones are served from the
  *                     caches, where the column's denser lines win) when the build's mean lines per
  *                     present key stays at most 1.10, no key needs more than 4 lines and the line
- *                     copy fits; else column */
+ *                     copy fits; else column.  Always column with GJ_PLACE_HOST (the line copy
+ *                     would put every key read on the host link) */
 #define GJ_LAYOUT_AUTO 0u
 #define GJ_LAYOUT_COLUMN 1u
 #define GJ_LAYOUT_LINES 2u

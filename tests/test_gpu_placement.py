@@ -35,3 +35,21 @@ def test_index_in_host_memory(gj, orc, gpu, kind, column_in_host):
     idx.close()
     if column_in_host:
         col.free()
+
+
+def test_auto_layout_in_host_memory_is_column(gj, orc, gpu):
+    # a 16 MiB column, where auto picks the lines layout in HBM, keeps the column layout
+    # when the index structures go to host memory
+    keys = gj.gen_build_keys(1 << 21, 8, _lib.KEYS_UNIFORM, 63, 0x83)
+    probes, exp = gj.gen_probes(keys, 200000, _lib.PROBES_UNIFORM, 1.0, 0.1, 63, 0x84)
+    col = gj.DeviceArray.from_numpy(keys)
+    dp = gj.DeviceArray.from_numpy(probes)
+    out = gj.DeviceArray(probes.size, np.uint32)
+    for placement, layout in ((_lib.PLACE_DEVICE, _lib.LAYOUT_LINES), (_lib.PLACE_HOST, _lib.LAYOUT_COLUMN)):
+        idx = gj.Index("learned_plr", col, host_column=keys, params=gj.index_params(placement=placement))
+        assert idx.stats()["layout"] == layout, placement
+        out.fill(0x5A)
+        idx.lookup(dp, out)
+        gj.synchronize()
+        assert np.array_equal(out.numpy(), exp), placement
+        idx.close()
