@@ -114,8 +114,9 @@ def main():
                     help="pick each index's own parameters with the index autotuner before building it")
     ap.add_argument("--suite", action="store_true", help="measure the other workload configurations instead")
     ap.add_argument("--quick", action="store_true", help="--suite at reduced sizes")
-    ap.add_argument("--join-aggregate", action="store_true",
-                    help="only the suite's join-aggregate measurements (for rocprofv3 runs)")
+    ap.add_argument("--only", choices=("C3", "C5", "host"), default="",
+                    help="run one leg of the other configurations alone and print its JSON (for rocprofv3 runs): "
+                         "C3 the partitioned driver, C5 the join-aggregate, host the host-memory placement")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -127,10 +128,9 @@ def main():
 
     info = gj.device_info()
     log("device", info)
-    if args.join_aggregate:
-        # the suite's join-aggregate measurements alone (one JSON object per line)
+    if args.only == "C5":
         import bench_suite as suite
-        suite.run_join_aggregate(gj, quick=args.quick, cpu=not args.no_cpu)
+        print(json.dumps({"C5_join_aggregate": suite.join_aggregate_leg(gj, gj.Stream(), cpu=not args.no_cpu)}))
         return
     n = 1 << args.log2_n
     m = 1 << args.log2_m
@@ -144,6 +144,12 @@ def main():
     d_probes = gj.DeviceArray.from_numpy(probes, stream)
     d_out = gj.DeviceArray(m, np.uint32)
     stream.synchronize()
+    if args.only:
+        import bench_suite as suite
+        leg = (suite.partitioned_leg(gj, None, stream, keys, d_keys, d_out) if args.only == "C3"
+               else suite.host_placement_leg(gj, stream, keys, d_keys, d_out))
+        print(json.dumps({args.only: leg}))
+        return
 
     results = {}
 

@@ -452,8 +452,8 @@ def partitioned_leg(gj, G, stream, keys, d_keys, d_out):
                 ms = _timed(gj, lambda: idx.lookup(dp, d_out, m=m, params=lp, stream=stream), stream, 2, 1)
                 ok = bool(np.array_equal(d_out.numpy(m), exp))
                 label = f"{pname}_{kind}_fan{1 << bits}_p{passes}_w2^{wlog}"
-                out["rows"][label] = {"lookups_per_s": m / ms * 1e3, "gather_roofline_frac": m / ms * 1e3 / G,
-                                      "rowids_match": ok}
+                out["rows"][label] = {"lookups_per_s": m / ms * 1e3,
+                                      "gather_roofline_frac": m / ms * 1e3 / G if G else None, "rowids_match": ok}
                 log(f"C3 {label}: {m / ms * 1e-6:.2f} G lookups/s match={ok}")
             assert idx.stats()["dropped_tuples"] == 0
             idx.close()

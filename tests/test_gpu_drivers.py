@@ -75,6 +75,31 @@ def test_partition_many_tiles_per_block(gj, orc, gpu, bits, passes):
     assert np.array_equal(gp[idx], op)
 
 
+@pytest.mark.parametrize("bits", [14, 16])
+def test_partition_skewed_segments(gj, orc, gpu, bits):
+    # 70 % of the keys share the top 8 bits, so the first pass leaves one segment with
+    # most of the chunks (855 of them, not a multiple of the column scan's batch)
+    m = 5_000_003
+    print(f"[case] partition skewed bits={bits} m={m}", flush=True)
+    rng = np.random.default_rng(bits + 0x5E)
+    hot = int(m * 0.7)
+    keys = np.concatenate([rng.integers(0, 2 ** 56, size=hot, dtype=np.uint64),
+                           rng.integers(0, 2 ** 64 - 1, size=m - hot, dtype=np.uint64)])
+    rng.shuffle(keys)
+    ignore = 64 - bits
+    dropped = gj.DeviceArray.from_numpy(np.zeros(1, np.uint64))
+    dk, dp, off, cnt = gj.partition(gj.DeviceArray.from_numpy(keys), m, bits, ignore, 2, dropped=dropped)
+    gj.synchronize()
+    assert int(dropped.numpy()[0]) == 0, "scatter dropped tuples"
+    counts, ok, op = orc.partition(keys, bits, ignore)
+    assert np.array_equal(cnt.numpy(), counts)
+    offs = off.numpy().astype(np.int64)
+    starts = np.concatenate([[0], np.cumsum(counts)[:-1]]).astype(np.int64)
+    idx = np.repeat(offs - starts, counts.astype(np.int64)) + np.arange(m, dtype=np.int64)
+    assert np.array_equal(dk.numpy()[idx], ok)
+    assert np.array_equal(dp.numpy()[idx], op)
+
+
 def test_partition_rejects_bad_args(gj, gpu):
     keys = gj.DeviceArray(16, np.uint64)
     with pytest.raises(gj.GjError):
@@ -135,12 +160,13 @@ def test_partitioned_lookup_many_tiles(gj, orc, gpu, kb, bits):
     assert np.array_equal(out.numpy(), orc.lookup_lower_bound(keys, probes))
 
 
-def test_partitioned_zipf_probes(gj, orc, gpu):
+@pytest.mark.parametrize("kw", [{"radix_bits": 10, "window": 1 << 18}, {"radix_bits": 16, "passes": 2}])
+def test_partitioned_zipf_probes(gj, orc, gpu, kw):
     keys, col, idx = _setup(gj, "btree", 1 << 20, 8)
     probes, exp = gj.gen_probes(keys, 1 << 20, _lib.PROBES_ZIPF, 1.0, 0.0, 0, 0x23)
     dp = gj.DeviceArray.from_numpy(probes)
     out = gj.DeviceArray(probes.size, np.uint32)
-    idx.lookup(dp, out, approach="partitioning", radix_bits=10, window=1 << 18)
+    idx.lookup(dp, out, approach="partitioning", **kw)
     gj.synchronize()
     assert np.array_equal(out.numpy(), exp)
     assert np.array_equal(orc.lookup_lower_bound(keys, probes), exp)
