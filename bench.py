@@ -112,11 +112,10 @@ def main():
     ap.add_argument("--tune-log2", type=int, default=24, help="probe sample size of the autotuner")
     ap.add_argument("--tune-index", action="store_true",
                     help="pick each index's own parameters with the index autotuner before building it")
-    ap.add_argument("--suite", action="store_true", help="measure the other workload configurations instead")
-    ap.add_argument("--quick", action="store_true", help="--suite at reduced sizes")
-    ap.add_argument("--only", choices=("C3", "C5", "host"), default="",
+    ap.add_argument("--only", choices=("C3", "sweep", "C5", "host", "other"), default="",
                     help="run one leg of the other configurations alone and print its JSON (for rocprofv3 runs): "
-                         "C3 the partitioned driver, C5 the join-aggregate, host the host-memory placement")
+                         "C3 the partitioned driver, sweep the index-size sweep, C5 the join-aggregate, host the "
+                         "host-memory placement, other sorted/dense probes, the hash-join driver and lookup plans")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -146,8 +145,11 @@ def main():
     stream.synchronize()
     if args.only:
         import bench_suite as suite
-        leg = (suite.partitioned_leg(gj, None, stream, keys, d_keys, d_out) if args.only == "C3"
-               else suite.host_placement_leg(gj, stream, keys, d_keys, d_out))
+        legs = {"C3": lambda: suite.partitioned_leg(gj, None, stream, keys, d_keys, d_out),
+                "sweep": lambda: suite.index_sweep_leg(gj, None, stream, d_out),
+                "host": lambda: suite.host_placement_leg(gj, stream, keys, d_keys, d_out),
+                "other": lambda: suite.other_leg(gj, None, stream, keys, d_keys, d_out)}
+        leg = legs[args.only]()
         print(json.dumps({args.only: leg}))
         return
 
@@ -201,13 +203,6 @@ def main():
     host_arr.free()
     del host_arr
     log(f"gather from host memory: {G_host / 1e9:.3f} G gathers/s")
-
-    if args.suite:
-        # the other workload configurations (one JSON object per line; not the headline run)
-        import bench_suite as suite
-        del d_probes, d_out, d_keys
-        suite.run(gj, G, quick=args.quick, cpu=not args.no_cpu)
-        return
 
     # ---- index lookups ------------------------------------------------------
     names = [args.profile_index] if args.profile_index else [s for s in args.indexes.split(",") if s]
