@@ -174,8 +174,13 @@ def main():
     src.fill(1, stream)
     per, _ = time_steps(lambda: gj.stream_copy(src, dst, cb, stream), 5, 2, stream, gj)
     copy_gbps = 2 * cb / (statistics.median(per) * 1e-3) / 1e9
-    del src, dst
     log(f"stream copy: {copy_gbps:.0f} GB/s (read + write)")
+    # streaming read of the same 1 GiB (the bound of read-only kernels: the join-aggregate)
+    xo = gj.DeviceArray(1, np.uint64)
+    per, _ = time_steps(lambda: gj.stream_read(src, cb, xo, stream), 5, 2, stream, gj)
+    read_gbps = cb / (statistics.median(per) * 1e-3) / 1e9
+    del src, dst, xo
+    log(f"stream read: {read_gbps:.0f} GB/s")
 
     # dependent pointer chase over 512 MiB (latency)
     nxt_n = 1 << 27
@@ -340,6 +345,7 @@ def main():
                         "source": "rocprofv3 FETCH_SIZE x2 + WRITE_SIZE (copy-calibrated), profiles/traffic.txt"},
         "l2": load_l2(f"{head}_u64_2^{args.log2_n}_2^{args.log2_m}") if head else None,
         "stream_copy_GBps": copy_gbps,
+        "stream_read_GBps": read_gbps,
         "pointer_chase_ns": chase_ns,
         "host_gather_per_s": G_host,
     }

@@ -23,7 +23,7 @@ from ._lib import (APPROACH_NAMES, BATCH_GRAPH, BATCH_LAUNCH, BATCH_PERSISTENT, 
 
 __all__ = [
     "DeviceArray", "Stream", "Event", "Index", "NOT_FOUND", "GjError", "gen_build_keys", "gen_probes",
-    "gen_dimension", "gen_fact", "sort_dimension", "device_info", "gather", "pointer_chase", "stream_copy",
+    "gen_dimension", "gen_fact", "sort_dimension", "device_info", "gather", "pointer_chase", "stream_copy", "stream_read",
     "index_params", "lookup_params", "partition", "join_aggregate", "promo_revenue", "LookupPlan",
     "AggregatePlan", "mlp_loss_grad", "mlp_forward", "autotune", "autotune_index",
 ]
@@ -329,6 +329,11 @@ def pointer_chase(nxt: DeviceArray, chains: int, steps: int, out: DeviceArray, s
 
 def stream_copy(src: DeviceArray, dst: DeviceArray, nbytes: int, stream: Optional[Stream] = None):
     check(_lib.call("gj_stream_copy", src.ptr, dst.ptr, nbytes, _s(stream)))
+
+
+def stream_read(src: DeviceArray, nbytes: int, out: DeviceArray, stream: Optional[Stream] = None):
+    """XORs every 8-byte word of the first nbytes of src into out[0] (a uint64 the caller sets)."""
+    check(_lib.call("gj_stream_read", src.ptr, nbytes, out.ptr, _s(stream)))
 
 
 # ---- partitioner / join-aggregate / plans / learned hooks ------------------

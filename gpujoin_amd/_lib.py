@@ -144,6 +144,7 @@ SIGNATURES = {
     "gj_gather": (_I, [_P, _U64, _P, _U64, _P, _U32, _U32, _P]),
     "gj_pointer_chase": (_I, [_P, _U64, _U32, _U32, _P, _P, _P]),
     "gj_stream_copy": (_I, [_P, _P, _U64, _P]),
+    "gj_stream_read": (_I, [_P, _U64, _P, _P]),
     "gj_last_error": (C.c_char_p, []),
     "gj_version": (C.c_char_p, []),
     # gj_runtime.h

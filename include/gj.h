@@ -434,6 +434,10 @@ int gj_pointer_chase(const uint32_t* d_next, uint64_t n, uint32_t chains, uint32
                      uint32_t* d_out, gj_stream_t stream);
 /* Streaming copy of bytes (multiple of 16) from d_src to d_dst. */
 int gj_stream_copy(const void* d_src, void* d_dst, uint64_t bytes, gj_stream_t stream);
+/* Streaming read of bytes (multiple of 16, 16-byte aligned) from d_src: XORs every
+ * 8-byte word into *d_xor (which the caller sets first), the bandwidth bound of a
+ * read-only kernel such as the join-aggregate. */
+int gj_stream_read(const void* d_src, uint64_t bytes, uint64_t* d_xor, gj_stream_t stream);
 
 /* ---- Diagnostics --------------------------------------------------------- */
 const char* gj_last_error(void);

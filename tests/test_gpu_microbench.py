@@ -1,7 +1,8 @@
 """GPU checks of the roofline probes: the random gather that defines G (the
 successor of random_access<512>, src/random_access.cu:53-66), the dependent
-pointer chase and the streaming copy.  Their outputs are compared with numpy, so
-the denominator of every "% of gather roofline" figure is a verified kernel."""
+pointer chase, the streaming copy and the streaming read.  Their outputs are compared
+with numpy, so the denominator of every "% of gather roofline" figure is a verified
+kernel."""
 import numpy as np
 import pytest
 
@@ -71,7 +72,7 @@ def test_pointer_chase_follows_chain_and_continues(gj, gpu):
     assert int(one.numpy()[0]) == int(perm[0])
 
 
-@pytest.mark.parametrize("nbytes", [16, 4096, (1 << 20) + 48])
+@pytest.mark.parametrize("nbytes", [16, 4096, (1 << 20) + 48, (64 << 20) + 16])
 def test_stream_copy_is_exact(gj, gpu, nbytes):
     src = np.random.default_rng(nbytes).integers(0, 256, size=nbytes, dtype=np.uint8)
     d_src = gj.DeviceArray.from_numpy(src)
@@ -82,3 +83,16 @@ def test_stream_copy_is_exact(gj, gpu, nbytes):
     got = d_dst.numpy()
     assert np.array_equal(got[:nbytes], src)
     assert np.all(got[nbytes:] == 0xA5)  # nothing written past the end
+
+
+@pytest.mark.parametrize("nbytes", [16, 4096, (1 << 20) + 48, (64 << 20) + 16])
+def test_stream_read_xor(gj, gpu, nbytes):
+    # every 8-byte word XORed into the caller's value, whatever the grid and the unrolled
+    # main loop / remainder split
+    words = np.random.default_rng(nbytes + 1).integers(0, 2 ** 64 - 1, size=nbytes // 8, dtype=np.uint64)
+    d_src = gj.DeviceArray.from_numpy(words)
+    seed = np.uint64(0x0123456789ABCDEF)
+    out = gj.DeviceArray.from_numpy(np.array([seed], dtype=np.uint64))
+    gj.stream_read(d_src, nbytes, out)
+    gj.synchronize()
+    assert int(out.numpy()[0]) == int(np.bitwise_xor.reduce(words) ^ seed)
