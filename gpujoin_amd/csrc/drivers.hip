@@ -22,6 +22,15 @@ constexpr uint64_t kMarkFallbackRatio = 400;
 
 uint32_t key_bit_width(uint64_t max_key) { return max_key ? 64u - static_cast<uint32_t>(__builtin_clzll(max_key)) : 0u; }
 
+// Windows below kWindowBatch probes are partitioned a batch at a time (each window on
+// its own, all windows of the batch in one launch per kernel): one window per launch
+// costs about 36 µs of launches and small kernels, which bound 2^16-probe windows at
+// 1.8 G lookups/s.  A batch holds up to kBatchProbes probes and kMaxBatchPartitions
+// partitions.
+constexpr uint64_t kWindowBatch = 1ull << 22;
+constexpr uint64_t kBatchProbes = 1ull << 24;
+constexpr uint64_t kMaxBatchPartitions = 1ull << 18;
+
 // partitioning_approach (src/index_lookup_partitioning.cu:377-513): the probes
 // are cut into windows; per window a stable radix partition on the top key bits
 // (payload = global probe index), then a lookup over the partitioned tuples that
@@ -66,6 +75,34 @@ void lookup_partitioned(const gj_index& ix, const K* probes, uint64_t m, uint32_
   gj_lookup_params tp = p;
   if ((ix.col_in_host || ix.params.placement == GJ_PLACE_HOST) && !tp.grid_blocks)
     tp.grid_blocks = static_cast<uint32_t>(ix.cu);
+  if (window < kWindowBatch && m > window) {
+    const uint64_t per = std::max<uint64_t>(1, std::min<uint64_t>(kBatchProbes / window, kMaxBatchPartitions >> bits));
+    const uint64_t batch = std::min<uint64_t>(per * window, m);
+    const uint64_t parts = ceil_div(batch, window) * fan;
+    const uint64_t bcap = windows_capacity(batch, window, sizeof(K), bits);
+    GJ_REQUIRE(bcap < 0xFFFFFFFFull, GJ_ERR_OVERFLOW, "window batch too large");
+    // scratch: keys | payloads | start (parts + 1) | count (parts) | partitioner scratch
+    const uint64_t b_pay = up(bcap * sizeof(K)), b_start = b_pay + up(bcap * 4), b_count = b_start + up((parts + 1) * 8),
+                   b_part = b_count + up(parts * 8),
+                   b_bytes = b_part + up(windows_scratch(batch, window, sizeof(K), bits, passes));
+    scratch.reserve(b_bytes);
+    uint8_t* bb = scratch.as<uint8_t>();
+    K* bkeys = reinterpret_cast<K*>(bb);
+    uint32_t* bpay = reinterpret_cast<uint32_t*>(bb + b_pay);
+    uint64_t* bstart = reinterpret_cast<uint64_t*>(bb + b_start);
+    uint64_t* bcount = reinterpret_cast<uint64_t*>(bb + b_count);
+    for (uint64_t b0 = 0; b0 < m; b0 += batch) {
+      const uint64_t bm = std::min<uint64_t>(batch, m - b0);
+      // the payload is the position in the batch, so the row ids go to out + b0
+      partition_windows<K>(probes + b0, bm, window, bits, ignore, passes, bkeys, bpay, bstart, bcount, bb + b_part,
+                           ix.dropped.as<unsigned long long>(), s);
+      with_index_dev<K>(ix, [&](auto d) {
+        launch_tuples(d, bkeys, bpay, bstart + ceil_div(bm, window) * fan, windows_capacity(bm, window, sizeof(K), bits),
+                      out + b0, tp, s, ix.cu);
+      });
+    }
+    return;
+  }
   for (uint64_t w0 = 0; w0 < m; w0 += window) {
     const uint64_t w = std::min<uint64_t>(window, m - w0);
     // unordered: the payload carries each probe's position back to `out`

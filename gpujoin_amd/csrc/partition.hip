@@ -182,19 +182,24 @@ __global__ __launch_bounds__(1024) void part_scan_starts_kernel(const uint64_t* 
   if (threadIdx.x == blockDim.x - 1) start[fan] = s[threadIdx.x];  // padded total
 }
 
-// Marks the padding slots after every partition with payload 0xFFFFFFFF and key_max.
+// Marks the padding slots after every partition (fewer than `align` each) with payload
+// 0xFFFFFFFF and key_max; one thread per partition, grid stride (a batch of windows has
+// up to 2^18 partitions).
 template <class K>
 __global__ void part_pad_kernel(const uint64_t* __restrict__ start, const uint64_t* __restrict__ count, uint32_t fan,
                                 uint32_t align, K* __restrict__ keys, uint32_t* __restrict__ payload) {
-  const uint32_t p = blockIdx.x;
-  if (p >= fan) return;
-  const uint64_t end = start[p] + count[p];
-  const uint64_t stop = start[p] + (count[p] + align - 1) / align * align;
-  for (uint64_t i = end + threadIdx.x; i < stop; i += blockDim.x) {
-    keys[i] = key_max<K>();
-    payload[i] = 0xFFFFFFFFu;
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t p = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; p < fan; p += stride) {
+    const uint64_t end = start[p] + count[p];
+    const uint64_t stop = start[p] + (count[p] + align - 1) / align * align;
+    for (uint64_t i = end; i < stop; ++i) {
+      keys[i] = key_max<K>();
+      payload[i] = 0xFFFFFFFFu;
+    }
   }
 }
+
+inline unsigned pad_grid(uint64_t fan) { return static_cast<unsigned>(std::max<uint64_t>(1, ceil_div(fan, 256))); }
 
 // Stable scatter.  in_payload == nullptr: payload = payload_base + input position.
 // A block walks its chunk in tiles of kPartTile items held in registers (wave w
@@ -604,16 +609,58 @@ __global__ __launch_bounds__(wc_block(Stable)) void part_scatter_wc_kernel(
 }
 
 // Chunks of the segmented pass: chunk0[h] = chunks of the segments before h (each
-// segment cut into ceil(count[h] / chunk) chunks), chunk0[segs] = all chunks.  One thread.
-__global__ void seg_chunks_kernel(const uint64_t* __restrict__ count, uint32_t segs, uint64_t chunk,
-                                  uint64_t* __restrict__ chunk0) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
-  uint64_t run = 0;
-  for (uint32_t h = 0; h < segs; ++h) {
-    chunk0[h] = run;
-    run += (count[h] + chunk - 1) / chunk;
+// segment cut into ceil(count[h] / chunk) chunks), chunk0[segs] = all chunks.  One block
+// of 1024 threads, each owning `per` consecutive segments (a batch of windows brings
+// thousands of segments), counts read kScanBatch at a time.
+__global__ __launch_bounds__(1024) void seg_chunks_kernel(const uint64_t* __restrict__ count, uint32_t segs,
+                                                          uint64_t chunk, uint64_t* __restrict__ chunk0) {
+  __shared__ uint64_t s[1024];
+  const uint32_t per = (segs + blockDim.x - 1) / blockDim.x;
+  const uint32_t h0 = threadIdx.x * per;
+  uint64_t local = 0;
+  for (uint32_t q = 0; q < per; q += kScanBatch) {
+    uint64_t c[kScanBatch];
+#pragma unroll
+    for (uint32_t u = 0; u < kScanBatch; ++u) {
+      const uint32_t h = h0 + q + u;
+      c[u] = (q + u < per && h < segs) ? count[h] : 0;
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < kScanBatch; ++u) local += (c[u] + chunk - 1) / chunk;
   }
-  chunk0[segs] = run;
+  s[threadIdx.x] = local;
+  __syncthreads();
+  for (uint32_t off = 1; off < blockDim.x; off <<= 1) {  // Hillis-Steele inclusive scan
+    const uint64_t v = threadIdx.x >= off ? s[threadIdx.x - off] : 0;
+    __syncthreads();
+    s[threadIdx.x] += v;
+    __syncthreads();
+  }
+  uint64_t run = s[threadIdx.x] - local;
+  for (uint32_t q = 0; q < per; q += kScanBatch) {
+    uint64_t c[kScanBatch];
+#pragma unroll
+    for (uint32_t u = 0; u < kScanBatch; ++u) {
+      const uint32_t h = h0 + q + u;
+      c[u] = (q + u < per && h < segs) ? count[h] : 0;
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < kScanBatch; ++u) {
+      const uint32_t h = h0 + q + u;
+      if (q + u < per && h < segs) chunk0[h] = run;
+      run += (c[u] + chunk - 1) / chunk;
+    }
+  }
+  if (threadIdx.x == blockDim.x - 1) chunk0[segs] = s[threadIdx.x];
+}
+
+// Segments of a batch of windows: segment h = probes [h * window, min((h + 1) * window, m)).
+__global__ void seg_windows_kernel(uint64_t* __restrict__ start, uint64_t* __restrict__ count, uint32_t segs,
+                                   uint64_t window, uint64_t m) {
+  const uint64_t h = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (h >= segs) return;
+  start[h] = h * window;
+  count[h] = std::min<uint64_t>(window, m - h * window);
 }
 
 // Column scan of the segmented pass: for output partition g = h * fan + p, the
@@ -747,8 +794,8 @@ void partition_pass(const K* keys, const uint32_t* in_payload, uint32_t payload_
   hipLaunchKernelGGL(part_scan_starts_kernel, dim3(1), dim3(1024), 0, s, count, p.fan, align, start);
   GJ_HIP_CHECK(hipGetLastError());
   if (align > 1) {
-    hipLaunchKernelGGL(part_pad_kernel<K>, dim3(p.fan), dim3(64), 0, s, start, count, p.fan, align, out_keys,
-                       out_payload);
+    hipLaunchKernelGGL(part_pad_kernel<K>, dim3(pad_grid(p.fan)), dim3(256), 0, s, start, count, p.fan, align,
+                       out_keys, out_payload);
     GJ_HIP_CHECK(hipGetLastError());
   }
   const uint64_t cap = m + (align > 1 ? static_cast<uint64_t>(p.fan) * align : 0ull);
@@ -804,7 +851,7 @@ void partition_segmented(const K* keys, const uint32_t* payload, uint64_t m, uin
   const uint64_t total_fan = static_cast<uint64_t>(segs) * fan;
   const uint64_t blocks = ceil_div(m, kPartTile) + segs;  // an upper bound on the chunks
   GJ_REQUIRE(blocks < (1ull << 31), GJ_ERR_INVALID, "too many partition chunks");
-  hipLaunchKernelGGL(seg_chunks_kernel, dim3(1), dim3(64), 0, s, count1, segs, static_cast<uint64_t>(kPartTile),
+  hipLaunchKernelGGL(seg_chunks_kernel, dim3(1), dim3(1024), 0, s, count1, segs, static_cast<uint64_t>(kPartTile),
                      chunk0);
   GJ_HIP_CHECK(hipGetLastError());
   const Seg sg{chunk0, start1, count1, segs};
@@ -817,16 +864,114 @@ void partition_segmented(const K* keys, const uint32_t* payload, uint64_t m, uin
   hipLaunchKernelGGL(part_scan_starts_kernel, dim3(1), dim3(1024), 0, s, count, static_cast<uint32_t>(total_fan), align,
                      start);
   GJ_HIP_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(part_pad_kernel<K>, dim3(static_cast<unsigned>(total_fan)), dim3(64), 0, s, start, count,
-                     static_cast<uint32_t>(total_fan), align, out_keys, out_payload);
-  GJ_HIP_CHECK(hipGetLastError());
-  const uint64_t cap = m + total_fan * align;
-  const size_t rlds = static_cast<size_t>(kPartWaves + 1) * fan * sizeof(uint32_t);
+  if (align > 1) {
+    hipLaunchKernelGGL(part_pad_kernel<K>, dim3(pad_grid(total_fan)), dim3(256), 0, s, start, count,
+                       static_cast<uint32_t>(total_fan), align, out_keys, out_payload);
+    GJ_HIP_CHECK(hipGetLastError());
+  }
+  const uint64_t cap = m + (align > 1 ? total_fan * align : 0ull);
+  const size_t rlds = static_cast<size_t>(kPartWaves + 1) * fan * sizeof(uint32_t);  // <= 80 KiB of the 160
+  if (rlds > 65536)
+    GJ_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&part_scatter_kernel<K>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(rlds)));
   hipLaunchKernelGGL(part_scatter_kernel<K>, dim3(static_cast<unsigned>(blocks)), dim3(kPartBlock), rlds, s, keys,
                      payload, 0u, m, static_cast<uint64_t>(kPartTile), shift, bits, hist, start, out_keys,
                      out_payload, cap, dropped, sg);
   GJ_HIP_CHECK(hipGetLastError());
 }
+
+namespace {
+
+// A batch of windows in one partitioning: the windows are the segments of the
+// segmented pass, so each window is partitioned on its own (output partition
+// w * 2^bits + digit, window-major) while the batch takes one launch per kernel.  Two
+// passes (or a fan-out above 2^12): a segmented pass per window on the high bits, then a
+// segmented pass per (window, high digit) on the low bits.
+uint32_t win_high_bits(uint32_t bits, uint32_t passes) {
+  if (bits > kMaxSinglePassBits) return seg_high_bits(bits);
+  return (passes == 2 && bits >= 2) ? (bits + 1) / 2 : 0u;  // 0: one pass
+}
+
+struct WinLayout {
+  uint64_t windows, segs_b;  // windows; segments of the last pass
+  uint32_t hb, lb;           // digit bits of the first pass (0: one pass) and of the last
+  uint64_t o_tk, o_tp, o_wstart, o_wcount, o_sa, o_ca, o_chunk0, bytes;
+};
+
+WinLayout win_layout(uint64_t m, uint64_t window, int key_bytes, uint32_t bits, uint32_t passes) {
+  WinLayout L{};
+  auto up = [](uint64_t v) { return ceil_div(v, 256) * 256; };
+  L.windows = ceil_div(m, window);
+  L.hb = win_high_bits(bits, passes);
+  L.lb = bits - L.hb;
+  L.segs_b = L.hb ? (L.windows << L.hb) : L.windows;
+  const uint64_t tiles = ceil_div(m, static_cast<uint64_t>(kPartTile));
+  const uint64_t rows_a = L.hb ? (tiles + L.windows) << L.hb : 0;
+  const uint64_t rows_b = (tiles + L.segs_b) << L.lb;
+  uint64_t o = up(std::max(rows_a, rows_b) * 4);  // histogram rows of either pass
+  L.o_tk = o;
+  if (L.hb) o += up((m + 32) * key_bytes);
+  L.o_tp = o;
+  if (L.hb) o += up((m + 32) * 4);
+  L.o_wstart = o;
+  o += up((L.windows + 2) * 8);
+  L.o_wcount = o;
+  o += up((L.windows + 2) * 8);
+  L.o_sa = o;
+  if (L.hb) o += up((L.segs_b + 2) * 8);
+  L.o_ca = o;
+  if (L.hb) o += up((L.segs_b + 2) * 8);
+  L.o_chunk0 = o;
+  o += up((L.segs_b + 2) * 8);
+  L.bytes = o;
+  return L;
+}
+
+}  // namespace
+
+uint64_t windows_capacity(uint64_t m, uint64_t window, int key_bytes, uint32_t bits) {
+  return m + ceil_div(m, window) * (1ull << bits) * part_align(key_bytes);
+}
+
+uint64_t windows_scratch(uint64_t m, uint64_t window, int key_bytes, uint32_t bits, uint32_t passes) {
+  return win_layout(m, window, key_bytes, bits, passes).bytes;
+}
+
+template <class K>
+void partition_windows(const K* keys, uint64_t m, uint64_t window, uint32_t bits, uint32_t ignore, uint32_t passes,
+                       K* out_keys, uint32_t* out_payload, uint64_t* start, uint64_t* count, uint8_t* scratch,
+                       unsigned long long* dropped, hipStream_t s) {
+  if (m == 0) return;
+  const WinLayout L = win_layout(m, window, sizeof(K), bits, passes);
+  GJ_REQUIRE((L.segs_b << L.lb) < (1ull << 31), GJ_ERR_INVALID, "too many partitions in a batch of windows");
+  uint32_t* hist = reinterpret_cast<uint32_t*>(scratch);
+  auto* wstart = reinterpret_cast<uint64_t*>(scratch + L.o_wstart);
+  auto* wcount = reinterpret_cast<uint64_t*>(scratch + L.o_wcount);
+  auto* chunk0 = reinterpret_cast<uint64_t*>(scratch + L.o_chunk0);
+  hipLaunchKernelGGL(seg_windows_kernel, dim3(static_cast<unsigned>(ceil_div(L.windows, 256))), dim3(256), 0, s,
+                     wstart, wcount, static_cast<uint32_t>(L.windows), window, m);
+  GJ_HIP_CHECK(hipGetLastError());
+  const uint32_t align = part_align(sizeof(K));
+  if (!L.hb) {
+    partition_segmented<K>(keys, nullptr, m, ignore, bits, static_cast<uint32_t>(L.windows), wstart, wcount, chunk0,
+                           align, out_keys, out_payload, start, count, hist, dropped, s);
+    return;
+  }
+  K* tk = reinterpret_cast<K*>(scratch + L.o_tk);
+  auto* tp = reinterpret_cast<uint32_t*>(scratch + L.o_tp);
+  auto* sa = reinterpret_cast<uint64_t*>(scratch + L.o_sa);
+  auto* ca = reinterpret_cast<uint64_t*>(scratch + L.o_ca);
+  partition_segmented<K>(keys, nullptr, m, ignore + L.lb, L.hb, static_cast<uint32_t>(L.windows), wstart, wcount,
+                         chunk0, 1, tk, tp, sa, ca, hist, dropped, s);
+  partition_segmented<K>(tk, tp, m, ignore, L.lb, static_cast<uint32_t>(L.segs_b), sa, ca, chunk0, align, out_keys,
+                         out_payload, start, count, hist, dropped, s);
+}
+template void partition_windows<uint32_t>(const uint32_t*, uint64_t, uint64_t, uint32_t, uint32_t, uint32_t,
+                                          uint32_t*, uint32_t*, uint64_t*, uint64_t*, uint8_t*, unsigned long long*,
+                                          hipStream_t);
+template void partition_windows<uint64_t>(const uint64_t*, uint64_t, uint64_t, uint32_t, uint32_t, uint32_t,
+                                          uint64_t*, uint32_t*, uint64_t*, uint64_t*, uint8_t*, unsigned long long*,
+                                          hipStream_t);
 
 // Full partitioning into `out` (1 or 2 passes; fan-outs above 2^12 always take two, the
 // second one segmented); scratch must hold partition_scratch() bytes; start must hold

@@ -108,6 +108,37 @@ def test_partition_rejects_bad_args(gj, gpu):
         gj.partition(keys, 16, 4, 64)
 
 
+@pytest.mark.parametrize("kind", ["btree", "learned_plr", "hash"])
+@pytest.mark.parametrize("pdist", [_lib.PROBES_UNIFORM, _lib.PROBES_ZIPF])
+@pytest.mark.parametrize("kw,m", [
+    ({"radix_bits": 10, "window": 1 << 12}, 300007),  # 74 windows in one batch, the last one ragged
+    ({"radix_bits": 12, "window": 1 << 10}, 300007),  # 64 windows per batch (the partition cap), 5 batches
+    ({"radix_bits": 12, "passes": 2, "window": 5000}, 300007),
+    ({"radix_bits": 14, "passes": 2, "window": 30000}, 300007),  # 16 windows per batch, two segmented passes
+    ({"radix_bits": 16, "window": 1 << 16}, 300007),  # 4 windows per batch
+    ({"radix_bits": 4, "window": 1}, 40003),  # one probe per window, 16384 windows per batch
+    ({"radix_bits": 8, "window": 3}, 40003),
+])
+def test_partitioned_window_batches(gj, orc, gpu, kind, pdist, kw, m):
+    # windows below 2^22 probes are partitioned a batch at a time, each window on its own
+    print(f"[case] window batches {kind} probes={pdist} {kw} m={m}", flush=True)
+    n = 1 << 18
+    keys = gj.gen_build_keys(n, 8, _lib.KEYS_UNIFORM, 0, 0x29)
+    col = gj.DeviceArray.from_numpy(keys)
+    params = gj.index_params(learned_layout=_lib.LAYOUT_LINES) if kind == "learned_plr" else None
+    idx = gj.Index(kind, col, host_column=keys, params=params)
+    probes, exp = gj.gen_probes(keys, m, pdist, 1.0, 0.1 if pdist == _lib.PROBES_UNIFORM else 0.0, 0, 0x2A)
+    dp = gj.DeviceArray.from_numpy(probes)
+    out = gj.DeviceArray(m, np.uint32)
+    out.fill(0x5A)
+    idx.lookup(dp, out, approach="partitioning", **kw)
+    gj.synchronize()
+    assert np.array_equal(out.numpy(), exp)
+    assert np.array_equal(orc.lookup_lower_bound(keys, probes), exp)
+    assert idx.stats()["dropped_tuples"] == 0
+    idx.close()
+
+
 def _setup(gj, kind, n, kb, dist=_lib.KEYS_UNIFORM):
     n = (n // 32) * 32  # harmonia needs a multiple of 32
     keys = gj.gen_build_keys(n, kb, dist, 0, 0x21 + kb)
