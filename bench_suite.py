@@ -386,9 +386,8 @@ def join_aggregate_leg(gj, stream, cpu=True, fact_rows=60_000_000, reps=6):
             stream.synchronize()
             ms = e[0].elapsed_ms(e[reps]) / reps
             # bytes the kernels must read: every row's date, each qualifying row's fkey, price
-            # and discount, each match's flag (late materialisation); and the survey's
-            # definition, all four fact columns per row plus an 8 B gather and the flag per
-            # qualifying row
+            # and discount, each match's flag (late materialisation); and, for comparison, all
+            # four fact columns per row plus an 8 B gather and the flag per qualifying row
             alg = fact_rows * 4 + copies[0]["n_qual"] * 13 + copies[0]["want"][2]
             alg_all = fact_rows * 17 + copies[0]["n_qual"] * 9
             label = kind if strategy == "lane_refill" else f"{kind}_{strategy}"
