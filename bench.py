@@ -91,8 +91,10 @@ def load_l2(kernel_tag: str):
 
 
 def _index_params_str(p) -> str:
-    return (f"lanes={p.harmonia_group_lanes} btree_lds={p.btree_lds_bytes} rs_bits={p.rs_radix_bits} rs_lds_bits={p.rs_lds_bits} "
-            f"rs_err={p.rs_max_error} leaf_bits={p.learned_leaf_bits} max_err={p.learned_max_error}")
+    return (f"lanes={p.harmonia_group_lanes} btree_node={p.btree_node_bytes} btree_lds={p.btree_lds_bytes} "
+            f"rs_bits={p.rs_radix_bits} rs_lds_bits={p.rs_lds_bits} "
+            f"rs_err={p.rs_max_error} layout={p.learned_layout} line_keys={p.learned_line_keys} "
+            f"leaf_bits={p.learned_leaf_bits} max_err={p.learned_max_error}")
 
 
 def main():

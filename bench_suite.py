@@ -234,12 +234,13 @@ def index_sweep_leg(gj, G, stream, d_out):
 
 def other_leg(gj, G, stream, keys, d_keys, d_out):
     """Sorted probes (the reference's -s true), dense keys (its --dataset dense), the
-    hash-join driver (a table built over the column on every call), and uneven probe
-    batches against one resident index through per-batch launches, a hipGraph and the
-    persistent kernel.  The headline's 2^26 keys and 2^28 probes."""
+    hash-join driver (a table built over the column on every call), the B-tree's node
+    sizes, and uneven probe batches against one resident index through per-batch
+    launches, a hipGraph and the persistent kernel.  The headline's 2^26 keys and 2^28
+    probes."""
     log = lambda *a: print("[extras]", *a, file=sys.stderr, flush=True)
     m = d_out.count
-    res = {"sorted_probes": {}, "dense_keys": {}, "hash_join_driver": {}, "lookup_plans": {}}
+    res = {"sorted_probes": {}, "dense_keys": {}, "hash_join_driver": {}, "lookup_plans": {}, "btree_node_bytes": {}}
 
     def run(section, label, idx, dp, exp, lp):
         ms = _timed(gj, lambda: idx.lookup(dp, d_out, m=m, params=lp, stream=stream), stream, 3, 1)
@@ -274,6 +275,12 @@ def other_leg(gj, G, stream, keys, d_keys, d_out):
     idx = gj.Index("btree", d_keys, host_column=keys, stream=stream)
     run("hash_join_driver", "table_build_and_probe", idx, dp, exp, gj.lookup_params("hj"))
     idx.close()
+    # the B-tree's node size (and leaf chunk): half a line, one line (default), two lines
+    for nb in (64, 128, 256):
+        idx = gj.Index("btree", d_keys, host_column=keys, stream=stream, params=gj.index_params(btree_node_bytes=nb))
+        run("btree_node_bytes", str(nb), idx, dp, exp, gj.lookup_params())
+        res["btree_node_bytes"][str(nb)]["depth"] = idx.stats()["depth"]
+        idx.close()
 
     rng = np.random.default_rng(0x15)
     sizes, total = [], 0

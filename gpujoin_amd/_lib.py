@@ -54,7 +54,7 @@ class IndexParams(C.Structure):
         ("mlp_hidden", C.c_uint32), ("mlp_steps", C.c_uint32), ("mlp_batch", C.c_uint32), ("mlp_lr", C.c_float),
         ("seed", C.c_uint64), ("learned_root_segments", C.c_uint32), ("learned_flags", C.c_uint32),
         ("learned_layout", C.c_uint32), ("learned_line_keys", C.c_uint32), ("rs_lds_bits", C.c_uint32),
-        ("placement", C.c_uint32),
+        ("placement", C.c_uint32), ("btree_node_bytes", C.c_uint32),
     ]
 
 

@@ -17,6 +17,7 @@ void fill_defaults(gj_index_params& p) {
   if (!p.rs_lds_bits) p.rs_lds_bits = 13;
   if (!p.harmonia_group_lanes) p.harmonia_group_lanes = 8;  // one 128-byte line per load
   if (!p.btree_lds_bytes) p.btree_lds_bytes = 16384;
+  if (!p.btree_node_bytes) p.btree_node_bytes = 128;
   // learned_leaf_bits 0: chosen from the column size at build
   if (!p.learned_max_error) p.learned_max_error = 128;
   if (!p.mlp_hidden) p.mlp_hidden = 32;

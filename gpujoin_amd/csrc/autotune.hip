@@ -86,8 +86,9 @@ extern "C" int gj_autotune_lookup(const gj_index* index, const void* d_probe_sam
 
 // ---- index autotuner -------------------------------------------------------
 // Picks the index's own parameters: threads per lookup (Harmonia's lanes per
-// node search), the LDS-resident part of the B-tree, RadixSpline's radix bits
-// and error bound, the learned index's leaf count and error bound.  Each
+// node search), the B-tree's node size and LDS-resident levels, RadixSpline's radix
+// bits and error bound, the learned index's layout, line load, leaf count and error
+// bound.  Each
 // candidate is built over the caller's column and timed with the plain driver
 // on the probe sample; the candidates are destroyed again.
 namespace {
@@ -106,7 +107,13 @@ std::vector<gj_index_params> index_candidates(gj_index_kind kind, uint64_t n, in
         if (g != 32 || key_bytes == 4) add([&](gj_index_params& p) { p.harmonia_group_lanes = g; });
       break;
     case GJ_BTREE:
-      for (uint32_t b : {4096u, 16384u, 32768u}) add([&](gj_index_params& p) { p.btree_lds_bytes = b; });
+      // node size (64 B, one 128 B line, 256 B) x the LDS budget of the upper levels
+      for (uint32_t nb : {64u, 128u, 256u})
+        for (uint32_t b : {4096u, 16384u, 32768u})
+          add([&](gj_index_params& p) {
+            p.btree_node_bytes = nb;
+            p.btree_lds_bytes = b;
+          });
       break;
     case GJ_RADIX_SPLINE:
       // radix bits whose table LDS holds (12, 13) and the larger global tables (16, 18) x the
@@ -123,6 +130,14 @@ std::vector<gj_index_params> index_candidates(gj_index_kind kind, uint64_t n, in
       // the default (compact leaves where the build admits them), then explicit counts
       // of 16-byte leaves around the default
       add([&](gj_index_params& p) { p.learned_leaf_bits = 0; });
+      // the layouts: the column itself, and lines filled to 8 or 12 keys (the default 10
+      // is the candidate above where the auto layout picks lines)
+      add([&](gj_index_params& p) { p.learned_layout = GJ_LAYOUT_COLUMN; });
+      for (uint32_t lk : {8u, 12u})
+        add([&](gj_index_params& p) {
+          p.learned_layout = GJ_LAYOUT_LINES;
+          p.learned_line_keys = key_bytes == 4 ? 2 * lk : lk;
+        });
       const int a = std::clamp(static_cast<int>(ilog2_ceil(n)) - 10, 16, 24);  // the build's default leaf bits
       for (int d : {-1, 0, 1})
         for (uint32_t e : {64u, 128u, 256u}) {

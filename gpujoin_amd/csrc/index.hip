@@ -59,9 +59,10 @@ __global__ void btree_fill_kernel(const K* __restrict__ col, uint64_t n, K* __re
   }
 }
 
-BTreeLayout btree_layout(uint64_t n, int key_bytes, uint32_t lds_budget_bytes) {
+BTreeLayout btree_layout(uint64_t n, int key_bytes, uint32_t lds_budget_bytes, uint32_t node_bytes) {
   BTreeLayout L{};
-  const uint64_t NK = 128 / static_cast<uint64_t>(key_bytes);
+  L.node_bytes = node_bytes;
+  const uint64_t NK = node_bytes / static_cast<uint64_t>(key_bytes);
   // C_h = ceil(n / NK^(h+1)): nodes at height h (h = 0: column chunks)
   std::vector<uint64_t> cnt;
   uint64_t span = NK;
@@ -83,8 +84,8 @@ BTreeLayout btree_layout(uint64_t n, int key_bytes, uint32_t lds_budget_bytes) {
   L.total_nodes = off;
   uint64_t bytes = 0;
   for (uint32_t lvl = 0; lvl < height; ++lvl) {
-    if (bytes + L.level_cnt[lvl] * 128 > lds_budget_bytes) break;
-    bytes += L.level_cnt[lvl] * 128;
+    if (bytes + L.level_cnt[lvl] * node_bytes > lds_budget_bytes) break;
+    bytes += L.level_cnt[lvl] * node_bytes;
     L.lds_levels = lvl + 1;
     L.lds_nodes = L.level_off[lvl] + L.level_cnt[lvl];
   }
@@ -137,9 +138,12 @@ void build_bsearch(gj_index& ix, hipStream_t s) {
 
 template <class K>
 void build_btree(gj_index& ix, hipStream_t s) {
-  ix.bt = btree_layout(ix.n, sizeof(K), ix.params.btree_lds_bytes);
-  const uint64_t NK = 128 / sizeof(K);
-  ix.btree_nodes.alloc(ix.bt.total_nodes * 128);
+  // the node size is a parameter of the B-tree kind; the learned kinds' fallback tree keeps 128
+  const uint32_t nb = ix.kind == GJ_BTREE ? ix.params.btree_node_bytes : 128u;
+  GJ_REQUIRE(nb == 64 || nb == 128 || nb == 256, GJ_ERR_INVALID, "btree_node_bytes must be 64, 128 or 256");
+  ix.bt = btree_layout(ix.n, sizeof(K), ix.params.btree_lds_bytes, nb);
+  const uint64_t NK = nb / sizeof(K);
+  ix.btree_nodes.alloc(ix.bt.total_nodes * nb);
   uint64_t span = NK;  // keys under one child of a height-1 node
   for (uint32_t h = 1; h <= ix.bt.levels; ++h) {
     const uint32_t lvl = ix.bt.levels - h;

@@ -13,6 +13,7 @@
 namespace gj {
 
 struct BTreeLayout {
+  uint32_t node_bytes = 128;  // bytes per node and per leaf chunk (64, 128 or 256)
   uint32_t levels = 0;      // inner levels
   uint32_t lds_levels = 0;  // top levels staged in LDS
   uint64_t lds_nodes = 0;
@@ -21,8 +22,8 @@ struct BTreeLayout {
   uint64_t level_cnt[kMaxLevels + 1] = {};
 };
 
-// Host-side layout of the implicit B-tree over n keys with NK keys per node.
-BTreeLayout btree_layout(uint64_t n, int key_bytes, uint32_t lds_budget_bytes);
+// Host-side layout of the implicit B-tree over n keys with node_bytes / key_bytes keys per node.
+BTreeLayout btree_layout(uint64_t n, int key_bytes, uint32_t lds_budget_bytes, uint32_t node_bytes = 128);
 
 }  // namespace gj
 
@@ -201,9 +202,11 @@ struct gj_index {
     return d;
   }
 
-  template <class K>
-  gj::BTreeDev<K> btree_dev() const {
-    gj::BTreeDev<K> d{};
+  // NB: the node size the tree was built with (bt.node_bytes); the learned kinds'
+  // fallback tree is always built with 128
+  template <class K, int NB = 128>
+  gj::BTreeDev<K, NB> btree_dev() const {
+    gj::BTreeDev<K, NB> d{};
     d.col = static_cast<const K*>(d_col);
     d.n = n;
     d.nodes = btree_nodes.as<const K>();

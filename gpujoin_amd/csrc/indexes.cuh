@@ -66,12 +66,14 @@ struct BSearchDev {
 // of lane (gbase + r), and a ballot counts the keys below that lane's probe.
 // All G loads of a level are issued before any compare, so a level costs one
 // memory latency.  This is the CDNA form of Harmonia's leader rotation
-// (src/harmonia.cuh:561-599) with one full line per load instruction.
+// (src/harmonia.cuh:561-599) with one full line per load instruction.  NB is the
+// node size in bytes (G = NB / 16 lanes per group): 128 by default, 64 or 256 for
+// the B-tree's other node sizes.
 // ---------------------------------------------------------------------------
-template <class K, int G>
+template <class K, int G, int NB = 128>
 __device__ __forceinline__ uint32_t group_count_less(const K* level_base, uint64_t my_node, K my_x, int gbase,
                                                      int gl) {
-  constexpr int NK = 128 / static_cast<int>(sizeof(K));
+  constexpr int NK = NB / static_cast<int>(sizeof(K));
   constexpr int KPL = NK / G;
   static_assert(KPL == Piece<K>::kKeys, "one 16-byte piece per lane");
   Piece<K> pc[G];
@@ -93,11 +95,11 @@ __device__ __forceinline__ uint32_t group_count_less(const K* level_base, uint64
   return result;
 }
 
-// Equality search of each lane's probe in its 128-byte column chunk.
-template <class K, int G>
+// Equality search of each lane's probe in its NB-byte column chunk.
+template <class K, int G, int NB = 128>
 __device__ __forceinline__ uint32_t group_find_equal(const K* col, uint64_t n, uint64_t my_chunk, K my_x,
                                                      bool my_live, int gbase, int gl) {
-  constexpr int NK = 128 / static_cast<int>(sizeof(K));
+  constexpr int NK = NB / static_cast<int>(sizeof(K));
   constexpr int KPL = NK / G;
   Piece<K> pc[G];
   K xs[G];
@@ -130,17 +132,19 @@ __device__ __forceinline__ uint32_t group_find_equal(const K* col, uint64_t n, u
 // B+-tree, re-laid out for CDNA4.  Keeps the reference's clustered-index
 // semantics (btree_index over the sorted column; leaves are column chunks,
 // src/btree.cuh:242-246, 504-531) with a static implicit layout:
-//   * a node is one 128-byte line of NK = 128 / sizeof(K) keys (16 x u64 / 32 x u32);
-//   * the leaves are the column itself, cut into chunks of NK keys (one line each);
+//   * a node is NB bytes, NK = NB / sizeof(K) keys: one 128-byte line by default
+//     (16 x u64 / 32 x u32), or half a line / two lines (NB = 64 / 256);
+//   * the leaves are the column itself, cut into chunks of NK keys (NB bytes each);
 //   * slot s of node i holds the max key of child i*NK + s (key_max past the end),
 //     so the child index is the count of separators below the probe and no
 //     child pointers are stored (the reference stores pointers, src/btree.cuh:58-63);
 //   * levels are stored top first; the top lds_levels levels are staged in LDS.
 // ---------------------------------------------------------------------------
-template <class K>
+template <class K, int NB = 128>
 struct BTreeDev {
-  static constexpr int NK = 128 / static_cast<int>(sizeof(K));
-  static constexpr int G = 8;
+  static_assert(NB == 64 || NB == 128 || NB == 256, "B-tree nodes of 64, 128 or 256 bytes");
+  static constexpr int NK = NB / static_cast<int>(sizeof(K));
+  static constexpr int G = NB / 16;  // lanes per node search, 16 bytes each
   const K* col;
   uint64_t n;
   const K* nodes;  // all inner levels, top level first, NK keys per node
@@ -150,8 +154,8 @@ struct BTreeDev {
   uint64_t level_off[kMaxLevels];      // first node of level l (top = 0)
   uint64_t level_cnt[kMaxLevels + 1];  // nodes of level l; level_cnt[levels] = column chunks
 
-  __host__ __device__ size_t lds_bytes() const { return lds_nodes * 128; }
-  __device__ void stage(uint8_t* lds) const { stage_to_lds(lds, nodes, lds_nodes * 128); }
+  __host__ __device__ size_t lds_bytes() const { return lds_nodes * NB; }
+  __device__ void stage(uint8_t* lds) const { stage_to_lds(lds, nodes, lds_nodes * NB); }
 
   __device__ __forceinline__ void descend(uint64_t& node, bool& live, uint32_t c, uint32_t lvl) const {
     const uint64_t child = node * NK + c;
@@ -165,11 +169,11 @@ struct BTreeDev {
     uint32_t lvl = 0;
     const K* lds_nodes_p = reinterpret_cast<const K*>(lds);
     for (; lvl < lds_levels; ++lvl) {
-      const uint32_t c = group_count_less<K, G>(lds_nodes_p + level_off[lvl] * NK, node, x, gbase, gl);
+      const uint32_t c = group_count_less<K, G, NB>(lds_nodes_p + level_off[lvl] * NK, node, x, gbase, gl);
       descend(node, live, c, lvl);
     }
     for (; lvl < levels; ++lvl) {
-      const uint32_t c = group_count_less<K, G>(nodes + level_off[lvl] * NK, node, x, gbase, gl);
+      const uint32_t c = group_count_less<K, G, NB>(nodes + level_off[lvl] * NK, node, x, gbase, gl);
       descend(node, live, c, lvl);
     }
     return node;
@@ -181,7 +185,7 @@ struct BTreeDev {
     const int gl = ln & (G - 1);
     bool live = active;
     const uint64_t chunk = find_chunk(x, live, lds, gbase, gl);
-    return group_find_equal<K, G>(col, n, chunk, x, live, gbase, gl);
+    return group_find_equal<K, G, NB>(col, n, chunk, x, live, gbase, gl);
   }
 };
 

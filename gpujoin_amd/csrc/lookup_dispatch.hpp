@@ -95,7 +95,14 @@ template <class K, class F>
 void with_index_dev(const gj_index& ix, F&& f) {
   switch (ix.kind) {
     case GJ_BINARY_SEARCH: f(ix.bsearch_dev<K>()); return;
-    case GJ_BTREE: f(ix.btree_dev<K>()); return;
+    case GJ_BTREE:
+      if (ix.bt.node_bytes == 64)
+        f(ix.btree_dev<K, 64>());
+      else if (ix.bt.node_bytes == 256)
+        f(ix.btree_dev<K, 256>());
+      else
+        f(ix.btree_dev<K>());
+      return;
     case GJ_NO_OP: f(NoOpDev<K>{}); return;
     case GJ_HARMONIA:
       switch (ix.params.harmonia_group_lanes) {

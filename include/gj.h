@@ -103,6 +103,9 @@ typedef struct gj_index_params {
                                      (32 KiB) */
   uint32_t placement;             /* GJ_PLACE_DEVICE (default) or GJ_PLACE_HOST: where the index's own
                                      structures live (the column is the caller's and stays where it is) */
+  uint32_t btree_node_bytes;      /* GJ_BTREE: bytes per node and per leaf chunk of the column, 64, 128
+                                     (default: one cache line) or 256; the learned kinds' fallback B-tree
+                                     always uses 128 */
 } gj_index_params;
 
 /* placement: the reference keeps its indexes in host memory by default and reads them over

@@ -374,3 +374,48 @@ def test_what_training_gains(gj, orc, gpu, dist):
         tr, un = rows[(_lib.LAYOUT_COLUMN, "trained")], rows[(_lib.LAYOUT_COLUMN, "untrained")]
         assert tr[0]["fallback_keys"] < un[0]["fallback_keys"]
         assert tr[1] > 1.3 * un[1]
+
+
+def _btree_levels(n, nk):
+    """Inner levels of the implicit B-tree: the chunk count divided by nk until one node."""
+    c, levels = -(-n // nk), 0
+    while True:
+        c = -(-c // nk)
+        levels += 1
+        if c <= 1:
+            return levels
+
+
+@pytest.mark.parametrize("nb", [64, 128, 256])
+@pytest.mark.parametrize("kb", [4, 8])
+@pytest.mark.parametrize("n", [1, 7, 100, 4097, 300007])
+def test_btree_node_sizes(gj, orc, gpu, nb, kb, n):
+    # nodes (and leaf chunks) of half a line, one line and two lines, with and without
+    # LDS-resident upper levels, through the plain, bws and partitioned drivers
+    print(f"[case] btree node={nb} kb={kb} n={n}", flush=True)
+    dt = np.uint64 if kb == 8 else np.uint32
+    rng = np.random.default_rng(nb + kb * 7 + n)
+    keys = np.unique(rng.integers(1, np.iinfo(dt).max - 1, size=n + n // 4 + 8, dtype=dt))[:n]
+    probes = np.concatenate([keys, keys + 1, keys - 1, rng.integers(0, np.iinfo(dt).max, size=2000, dtype=dt),
+                             np.array([0, np.iinfo(dt).max], dtype=dt)])
+    want = orc.lookup_lower_bound(keys, probes)
+    col = gj.DeviceArray.from_numpy(keys)
+    dp = gj.DeviceArray.from_numpy(probes)
+    out = gj.DeviceArray(probes.size, np.uint32)
+    for lds in (1, 16384):
+        idx = gj.Index("btree", col, host_column=keys, params=gj.index_params(btree_node_bytes=nb, btree_lds_bytes=lds))
+        st = idx.stats()
+        assert st["depth"] == _btree_levels(keys.size, nb // kb), st
+        assert (st["lds_levels"] == 0) == (lds == 1)
+        for approach, kw in (("plain", {}), ("bws", {}), ("partitioning", {"radix_bits": 8, "window": 3000})):
+            out.fill(0x5A)
+            idx.lookup(dp, out, approach=approach, **kw)
+            gj.synchronize()
+            assert np.array_equal(out.numpy(), want), (lds, approach)
+        idx.close()
+
+
+def test_btree_node_size_rejected(gj, gpu):
+    keys = np.arange(1, 1001, dtype=np.uint64)
+    with pytest.raises(gj.GjError):
+        gj.Index("btree", gj.DeviceArray.from_numpy(keys), host_column=keys, params=gj.index_params(btree_node_bytes=96))
