@@ -1,5 +1,6 @@
 """The legs of the default bench run beyond the headline (bench.py puts their results
-under "configs" in its JSON line; `bench.py --only C3|C5|host` runs one leg alone).
+under "configs" in its JSON line; `bench.py --only C3|sweep|C5|host|other` runs one leg
+alone).
 
   C1_u32             2^20 unique u32 keys, 2^24 probes: row ids + u64 checksum, every index
   C2b_zipf           the headline's 2^26 u64 keys, 2^28 Zipf(1.0) probes, every index
@@ -9,7 +10,8 @@ under "configs" in its JSON line; `bench.py --only C3|C5|host` runs one leg alon
                      learned in both layouts, fallback keys counted
   C5_join_aggregate  2e6-row dimension, 6e7 fact rows, every strategy, the three batch modes
   host_placement     index structures (and the column) in page-locked host memory
-  other              sorted probes, dense keys, the hash-join driver, lookup plans
+  other              sorted probes, dense keys, the hash-join driver, the B-tree's node sizes,
+                     lookup plans
 
 Every measurement checks its row ids or sums against the generator (or the oracle for
 the aggregates).
