@@ -96,40 +96,55 @@ __device__ __forceinline__ void agg_drain(const Dev& d, const uint8_t* lds, AggQ
 // Rows of one fact batch: the wave takes U x 64 rows per step (row
 // base + k * 64 + lane), starting at r0 and advancing by `step`, below r1.  All
 // date loads of a step are issued together, then the fkey / price / discount
-// loads of every qualifying row, so a step costs two memory latencies however
-// many slices qualify; qualifying rows queue in LDS and are looked up 64 at a time.
+// loads of every qualifying row, and right behind those the next step's date
+// loads, so the two memory latencies of consecutive steps overlap; qualifying rows
+// queue in LDS and are looked up 64 at a time.
 template <int U, class Dev>
 __device__ __forceinline__ void agg_rows(const Dev& d, const uint8_t* lds, AggQueue& q, const FactBatchDev& b,
                                          uint64_t r0, uint64_t r1, uint64_t step, int32_t lo, int32_t hi,
                                          const uint8_t* __restrict__ flag, AggAcc& acc) {
   const int ln = lane();
   const uint64_t lt = (ln == 0) ? 0ull : (~0ull >> (64 - ln));
-  for (uint64_t base = r0; base < r1; base += step) {
-    int32_t dt[U];
+  int32_t dt[U];
 #pragma unroll
-    for (int k = 0; k < U; ++k) {
-      const uint64_t i = base + static_cast<uint64_t>(k * kWave + ln);
-      dt[k] = i < r1 ? load_stream(b.date + i) : 0;
-    }
+  for (int k = 0; k < U; ++k) {
+    const uint64_t i = r0 + static_cast<uint64_t>(k * kWave + ln);
+    dt[k] = i < r1 ? load_stream(b.date + i) : 0;
+  }
+  for (uint64_t base = r0; base < r1; base += step) {
     uint32_t qm = 0;  // bit k: row of slice k qualifies
 #pragma unroll
     for (int k = 0; k < U; ++k) {
       const uint64_t i = base + static_cast<uint64_t>(k * kWave + ln);
       qm |= ((i < r1 && dt[k] >= lo && dt[k] < hi) ? 1u : 0u) << k;
     }
-    if (ballot(qm != 0) == 0) continue;  // wave-uniform
+    const bool any = ballot(qm != 0) != 0;  // wave-uniform
     uint32_t fk[U];
-    int64_t sv[U];
+    int64_t pr[U];
+    int32_t dc[U];
+    if (any) {
 #pragma unroll
-    for (int k = 0; k < U; ++k) {
-      fk[k] = 0;
-      sv[k] = 0;
-      if ((qm >> k) & 1u) {
-        const uint64_t i = base + static_cast<uint64_t>(k * kWave + ln);
-        fk[k] = static_cast<uint32_t>(b.fkey[i]);
-        sv[k] = b.price[i] * (100 - static_cast<int64_t>(b.disc[i]));
+      for (int k = 0; k < U; ++k) {
+        fk[k] = 0;
+        pr[k] = 0;
+        dc[k] = 0;
+        if ((qm >> k) & 1u) {
+          const uint64_t i = base + static_cast<uint64_t>(k * kWave + ln);
+          fk[k] = static_cast<uint32_t>(b.fkey[i]);
+          pr[k] = b.price[i];
+          dc[k] = b.disc[i];
+        }
       }
     }
+#pragma unroll
+    for (int k = 0; k < U; ++k) {  // the next step's dates, behind this step's column loads
+      const uint64_t i = base + step + static_cast<uint64_t>(k * kWave + ln);
+      dt[k] = i < r1 ? load_stream(b.date + i) : 0;
+    }
+    if (!any) continue;
+    int64_t sv[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) sv[k] = pr[k] * (100 - static_cast<int64_t>(dc[k]));
 #pragma unroll
     for (int k = 0; k < U; ++k) {
       const bool qual = (qm >> k) & 1u;
