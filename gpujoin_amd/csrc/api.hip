@@ -123,6 +123,9 @@ int gj_index_build(gj_index_kind kind, const void* h_sorted_keys, const void* d_
       GJ_HIP_CHECK(hipStreamSynchronize(s));
     }
     GJ_REQUIRE(ix->params.placement <= GJ_PLACE_HOST, GJ_ERR_INVALID, "placement must be 0 or 1");
+    // the partitioner's dropped-tuple counter (allocated before the build, which may partition)
+    ix->dropped.alloc(16);
+    fill_bytes(ix->dropped.p, 0, ix->dropped.bytes, s);
     build_index(*ix, s);
     if (ix->params.placement == GJ_PLACE_HOST) {
       GJ_HIP_CHECK(hipStreamSynchronize(s));
@@ -132,8 +135,6 @@ int gj_index_build(gj_index_kind kind, const void* h_sorted_keys, const void* d_
         b->to_host();
       ix->stats.placement = GJ_PLACE_HOST;
     }
-    ix->dropped.alloc(16);
-    fill_bytes(ix->dropped.p, 0, ix->dropped.bytes, s);
     GJ_HIP_CHECK(hipStreamSynchronize(s));
     ix->h_col = nullptr;  // the host view is only valid during construct
     ix->stats.kind = kind;

@@ -419,3 +419,32 @@ def test_btree_node_size_rejected(gj, gpu):
     keys = np.arange(1, 1001, dtype=np.uint64)
     with pytest.raises(gj.GjError):
         gj.Index("btree", gj.DeviceArray.from_numpy(keys), host_column=keys, params=gj.index_params(btree_node_bytes=96))
+
+
+@pytest.mark.parametrize("kb", [4, 8])
+def test_hash_large_column(gj, orc, gpu, kb):
+    # a 2^21-key table (4 M slots of 16 bytes): the hash index and the hash-join driver,
+    # which rebuilds the table on every call
+    n = (1 << 21) + 12345
+    dt = np.uint64 if kb == 8 else np.uint32
+    rng = np.random.default_rng(0x4A5 + kb)
+    keys = np.unique(rng.integers(1, np.iinfo(dt).max - 1, size=n + n // 8, dtype=dt))[:n]
+    probes = np.concatenate([rng.choice(keys, 300000), rng.integers(0, np.iinfo(dt).max, size=30000, dtype=dt),
+                             keys[:3], keys[-3:]])
+    want = orc.lookup_lower_bound(keys, probes)
+    col = gj.DeviceArray.from_numpy(keys)
+    dp = gj.DeviceArray.from_numpy(probes)
+    out = gj.DeviceArray(probes.size, np.uint32)
+    idx = gj.Index("hash", col, host_column=keys)
+    out.fill(0x5A)
+    idx.lookup(dp, out)
+    gj.synchronize()
+    assert np.array_equal(out.numpy(), want)
+    assert idx.stats()["dropped_tuples"] == 0
+    bt = gj.Index("btree", col, host_column=keys)
+    for rep in range(2):  # the driver rebuilds the table on every call
+        out.fill(0x5A)
+        bt.lookup(dp, out, approach="hj")
+        gj.synchronize()
+        assert np.array_equal(out.numpy(), want), rep
+    assert bt.stats()["dropped_tuples"] == 0
