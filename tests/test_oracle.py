@@ -179,6 +179,41 @@ def test_murmur3_finaliser(orc):
     assert len({orc.murmur3_64(int(x)) for x in xs}) == xs.size
 
 
+# The multiplicative inverses of MurmurHash3's finaliser constants (mod 2^64 for fmix64,
+# mod 2^32 for fmix32), as published with the finaliser's inversion; an xorshift by s
+# bits is undone by xoring the shifts by s, 2s, 3s, ...
+_INV64 = (0x4F74430C22A54005, 0x9CB4B2F8129337DB)  # of 0xff51afd7ed558ccd, 0xc4ceb9fe1a85ec53
+_INV32 = (0xA5CB9243, 0x7ED1B41D)  # of 0x85ebca6b, 0xc2b2ae35
+
+
+def _unshift(h, s, bits):
+    out, t = h, h >> s
+    while t:
+        out ^= t
+        t >>= s
+    return out & ((1 << bits) - 1)
+
+
+def _unfmix(h, shifts, inv, bits):
+    m = (1 << bits) - 1
+    h = _unshift(h, shifts[2], bits)
+    h = (h * inv[1]) & m
+    h = _unshift(h, shifts[1], bits)
+    h = (h * inv[0]) & m
+    return _unshift(h, shifts[0], bits)
+
+
+def test_murmur3_finaliser_inverts(orc):
+    # undoing each step with independently published constants recovers the input, which a
+    # mistyped multiplier or shift in the oracle's finalisers would break
+    rng = np.random.default_rng(0x3E)
+    for x in [0, 1, 2, 3, 0x123456789ABCDEF, 2 ** 63, 2 ** 64 - 1] + [int(v) for v in
+                                                                      rng.integers(0, 2 ** 63, 500, dtype=np.uint64)]:
+        assert _unfmix(orc.murmur3_64(x), (33, 33, 33), _INV64, 64) == x
+        x32 = x & 0xFFFFFFFF
+        assert _unfmix(orc.murmur3_32(x32), (16, 13, 16), _INV32, 32) == x32
+
+
 def test_hash_join_equals_lower_bound(orc):
     rng = np.random.default_rng(6)
     for kb in (4, 8):
