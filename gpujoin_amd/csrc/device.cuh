@@ -119,6 +119,27 @@ __host__ __device__ __forceinline__ uint64_t murmur3_64(uint64_t k) {
   return k;
 }
 
+// murmur3 32-bit finaliser (src/cuda_utils.cuh:65-79).
+__host__ __device__ __forceinline__ uint32_t murmur3_32(uint32_t k) {
+  k ^= k >> 16;
+  k *= 0x85ebca6bu;
+  k ^= k >> 13;
+  k *= 0xc2b2ae35u;
+  k ^= k >> 16;
+  return k;
+}
+
+// The hash of a K-wide key: the reference hashes with the finaliser of the key's own width
+// (murmur3_hash's 32- and 64-bit overloads, src/cuda_utils.cuh:65-100).
+template <class K>
+__host__ __device__ __forceinline__ uint64_t key_hash(uint64_t k) {
+  if constexpr (sizeof(K) == 4) {
+    return murmur3_32(static_cast<uint32_t>(k));
+  } else {
+    return murmur3_64(k);
+  }
+}
+
 // Block-wide copy of `count` 16-byte units from global memory into LDS.
 __device__ __forceinline__ void stage_to_lds(void* lds, const void* src, uint64_t bytes) {
   const u32x4* s = reinterpret_cast<const u32x4*>(src);

@@ -218,7 +218,7 @@ __global__ void ht_insert_kernel(const K* __restrict__ col, uint64_t n, HashEntr
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
     const unsigned long long k = static_cast<uint64_t>(col[i]);
-    uint64_t s = murmur3_64(k) & mask;
+    uint64_t s = key_hash<K>(k) & mask;
     for (;;) {
       const unsigned long long prev = atomicCAS(&table[s].key, ~0ull, k);
       if (prev == ~0ull || prev == k) {

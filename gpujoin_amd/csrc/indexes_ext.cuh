@@ -808,7 +808,8 @@ struct LinesDev {
 // ---------------------------------------------------------------------------
 // Linear-probing hash table over the column (linear_probing_hashtable,
 // src/linear_probing_hashtable.cuh:14-98): capacity 2^ceil(log2(n / 0.5)),
-// slot = murmur3_64(key) & (capacity - 1), step +1, empty = all-ones key.
+// slot = murmur3(key) & (capacity - 1) with the finaliser of the key's width, step +1,
+// empty = all-ones key.
 // An entry is 16 bytes {u64 key, u32 row id, pad}; a probe reads one entry per
 // 16-byte load (the reference's ld.global.cs of a longlong2, :62).
 // ---------------------------------------------------------------------------
@@ -830,7 +831,7 @@ struct HashDev {
   __device__ __forceinline__ uint32_t lookup(K x, bool active, const uint8_t*) const {
     const uint64_t k = static_cast<uint64_t>(x);
     if (!active || k == ~0ull) return kNotFound;  // all-ones is the empty marker, never a stored key
-    uint64_t s = murmur3_64(k) & mask;
+    uint64_t s = key_hash<K>(k) & mask;
     for (uint64_t probes = 0; probes <= mask; ++probes) {
       const u32x4 e = *reinterpret_cast<const u32x4*>(table + s);
       const uint64_t ek = (static_cast<uint64_t>(e.y) << 32) | e.x;

@@ -645,7 +645,11 @@ uint64_t orc_hash_capacity(uint64_t n) {
 }
 
 /* insert (:37-48) every column key with its position, then lookup (:50-72) every probe.
- * Slots hash the key zero-extended to 64 bits. */
+ * Slots hash the key with the finaliser of its width (murmur3_hash's 32- and 64-bit
+ * overloads, src/cuda_utils.cuh:65-100). */
+static uint64_t orc_key_hash(uint64_t k, int kb) {
+  return kb == 4 ? (uint64_t)orc_murmur3_32((uint32_t)k) : orc_murmur3_64(k);
+}
 void orc_hash_join(const void* col, uint64_t n, int kb, const void* probes, uint64_t m, uint32_t* out) {
   uint64_t cap = orc_hash_capacity(n);
   uint64_t* tk = malloc(cap * sizeof(uint64_t));
@@ -654,7 +658,7 @@ void orc_hash_join(const void* col, uint64_t n, int kb, const void* probes, uint
   for (uint64_t i = 0; i < cap; ++i) tk[i] = empty;
   for (uint64_t i = 0; i < n; ++i) {
     uint64_t k = kat(col, i, kb);
-    uint64_t s = orc_murmur3_64(k) & (cap - 1);
+    uint64_t s = orc_key_hash(k, kb) & (cap - 1);
     for (;;) {
       if (tk[s] == empty || tk[s] == k) {
         tk[s] = k;
@@ -666,7 +670,7 @@ void orc_hash_join(const void* col, uint64_t n, int kb, const void* probes, uint
   }
   for (uint64_t j = 0; j < m; ++j) {
     uint64_t k = kat(probes, j, kb);
-    uint64_t s = orc_murmur3_64(k) & (cap - 1);
+    uint64_t s = orc_key_hash(k, kb) & (cap - 1);
     uint32_t r = ORC_NOT_FOUND;
     for (;;) {
       if (tk[s] == k && k != empty) { r = tv[s]; break; }
