@@ -84,7 +84,9 @@ __device__ __forceinline__ uint32_t xcd_logical_block() {
 // workgroup-wide step apart, so the key range in flight on an XCD (hence the
 // index nodes and column lines its L2 must hold) spans a few partitions rather
 // than one partition per workgroup.  The tuple count is read from *d_total when
-// given (it is known only on the device).
+// given (it is known only on the device).  payload == nullptr: every tuple, padding
+// included, is looked up and its row id written to out[t] in tuple order (full-line
+// stores), for unpartition_kernel to bring back to probe order.
 template <class Dev, class K>
 __global__ __launch_bounds__(256) void lookup_tuples_kernel(Dev d, const K* __restrict__ keys,
                                                             const uint32_t* __restrict__ payload,
@@ -105,6 +107,13 @@ __global__ __launch_bounds__(256) void lookup_tuples_kernel(Dev d, const K* __re
   for (uint64_t base = lo + static_cast<uint64_t>(blockIdx.x / groups) * blockDim.x + w * kWave; base < hi;
        base += step) {
     const uint64_t i = base + static_cast<uint64_t>(lane());
+    if (!payload) {  // wave-uniform
+      const bool active = i < hi;
+      const K x = active ? load_stream(keys + i) : K(0);
+      const uint32_t r = d.lookup(x, active, lds);
+      if (active) store_stream(out + i, r);
+      continue;
+    }
     uint32_t dst = 0xFFFFFFFFu;
     K x = K(0);
     if (i < hi) {

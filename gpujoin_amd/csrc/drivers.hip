@@ -10,16 +10,68 @@ namespace gj {
 
 namespace {
 
-// Number of significant bits of the largest key.  The reference derives
-// ignore_bits = floor(log2(max key)) - radix_bits (src/index_lookup_new.cu:63-67),
-// which leaves the top significant bit out of the digit; using the bit width
-// keeps the digit on the top radix_bits significant bits.
 // Below one fallback key in this many build keys, the plain driver marks and
 // re-looks-up fallback probes; above, it runs the B-tree inline.  A wave with a
 // mark costs one B-tree lookup, so marking wins while fewer than about one wave
 // in six (64 / 400) holds a fallback probe.
 constexpr uint64_t kMarkFallbackRatio = 400;
 
+// Row ids back to probe order after a batch of windows: the partitioner recorded where
+// each probe went (inv1; with two passes a first-pass position, which inv2 maps to the
+// final one) and the tuple lookup wrote row ids in tuple order, so
+// out[i] = res[inv2[inv1[i]]].  Each XCD takes one contiguous eighth of the probes, and
+// its workgroups sweep it together, one workgroup tile apart, so the windows in flight
+// on an XCD (whose results its L2 must hold for the gathers) are one or two.  A position
+// outside the arrays (a tuple the partitioner dropped and counted) reads as not found.
+constexpr int kUnpartIn = 4;
+__global__ __launch_bounds__(256) void unpartition_kernel(const uint32_t* __restrict__ inv1,
+                                                          const uint32_t* __restrict__ inv2,
+                                                          const uint32_t* __restrict__ res, uint64_t res_n,
+                                                          uint64_t m, uint32_t* __restrict__ out) {
+  const uint32_t groups = (gridDim.x % 8 == 0) ? 8u : 1u;  // XCDs
+  const uint64_t tile = static_cast<uint64_t>(blockDim.x) * kUnpartIn;
+  const uint64_t chunk = ((m + groups - 1) / groups + tile - 1) / tile * tile;
+  const uint64_t lo = static_cast<uint64_t>(blockIdx.x % groups) * chunk;
+  const uint64_t hi = lo + chunk < m ? lo + chunk : m;
+  const uint64_t step = static_cast<uint64_t>(gridDim.x / groups) * tile;
+  for (uint64_t i0 = lo + static_cast<uint64_t>(blockIdx.x / groups) * tile + threadIdx.x; i0 < hi; i0 += step) {
+    uint32_t t[kUnpartIn];
+#pragma unroll
+    for (int u = 0; u < kUnpartIn; ++u) {
+      const uint64_t i = i0 + static_cast<uint64_t>(u) * blockDim.x;
+      t[u] = i < hi ? load_stream(inv1 + i) : 0xFFFFFFFFu;
+    }
+    if (inv2) {
+#pragma unroll
+      for (int u = 0; u < kUnpartIn; ++u) t[u] = t[u] < m ? inv2[t[u]] : 0xFFFFFFFFu;
+    }
+    uint32_t r[kUnpartIn];
+#pragma unroll
+    for (int u = 0; u < kUnpartIn; ++u) r[u] = t[u] < res_n ? res[t[u]] : kNotFound;
+#pragma unroll
+    for (int u = 0; u < kUnpartIn; ++u) {
+      const uint64_t i = i0 + static_cast<uint64_t>(u) * blockDim.x;
+      if (i < hi) store_stream(out + i, r[u]);
+    }
+  }
+}
+
+// out[i] = res[inv2[inv1[i]]] for i < m (inv2 optional); res holds res_n entries.
+inline void launch_unpartition(const uint32_t* inv1, const uint32_t* inv2, const uint32_t* res, uint64_t res_n,
+                               uint64_t m, uint32_t* out, hipStream_t s, int cu) {
+  if (m == 0) return;
+  uint64_t blocks = std::min<uint64_t>(static_cast<uint64_t>(cu) * 4, ceil_div(m, 256 * kUnpartIn));
+  if (blocks >= 8) blocks = blocks / 8 * 8;  // whole XCD rounds for the XCD-major sweep
+  blocks = std::max<uint64_t>(blocks, 1);
+  hipLaunchKernelGGL(unpartition_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, inv1, inv2, res, res_n,
+                     m, out);
+  GJ_HIP_CHECK(hipGetLastError());
+}
+
+// Number of significant bits of the largest key.  The reference derives
+// ignore_bits = floor(log2(max key)) - radix_bits (src/index_lookup_new.cu:63-67),
+// which leaves the top significant bit out of the digit; using the bit width
+// keeps the digit on the top radix_bits significant bits.
 uint32_t key_bit_width(uint64_t max_key) { return max_key ? 64u - static_cast<uint32_t>(__builtin_clzll(max_key)) : 0u; }
 
 // Windows below kWindowBatch probes are partitioned a batch at a time (each window on
@@ -81,25 +133,33 @@ void lookup_partitioned(const gj_index& ix, const K* probes, uint64_t m, uint32_
     const uint64_t parts = ceil_div(batch, window) * fan;
     const uint64_t bcap = windows_capacity(batch, window, sizeof(K), bits);
     GJ_REQUIRE(bcap < 0xFFFFFFFFull, GJ_ERR_OVERFLOW, "window batch too large");
-    // scratch: keys | payloads | start (parts + 1) | count (parts) | partitioner scratch
-    const uint64_t b_pay = up(bcap * sizeof(K)), b_start = b_pay + up(bcap * 4), b_count = b_start + up((parts + 1) * 8),
+    const bool two = windows_two_pass(bits, passes);
+    // scratch: keys | row ids in tuple order | inv1 | inv2 (two passes) | start (parts + 1) |
+    // count (parts) | partitioner scratch
+    const uint64_t b_res = up(bcap * sizeof(K)), b_inv1 = b_res + up(bcap * 4), b_inv2 = b_inv1 + up(batch * 4),
+                   b_start = b_inv2 + (two ? up(batch * 4) : 0), b_count = b_start + up((parts + 1) * 8),
                    b_part = b_count + up(parts * 8),
                    b_bytes = b_part + up(windows_scratch(batch, window, sizeof(K), bits, passes));
     scratch.reserve(b_bytes);
     uint8_t* bb = scratch.as<uint8_t>();
     K* bkeys = reinterpret_cast<K*>(bb);
-    uint32_t* bpay = reinterpret_cast<uint32_t*>(bb + b_pay);
+    uint32_t* bres = reinterpret_cast<uint32_t*>(bb + b_res);
+    uint32_t* binv1 = reinterpret_cast<uint32_t*>(bb + b_inv1);
+    uint32_t* binv2 = two ? reinterpret_cast<uint32_t*>(bb + b_inv2) : nullptr;
     uint64_t* bstart = reinterpret_cast<uint64_t*>(bb + b_start);
     uint64_t* bcount = reinterpret_cast<uint64_t*>(bb + b_count);
     for (uint64_t b0 = 0; b0 < m; b0 += batch) {
       const uint64_t bm = std::min<uint64_t>(batch, m - b0);
-      // the payload is the position in the batch, so the row ids go to out + b0
-      partition_windows<K>(probes + b0, bm, window, bits, ignore, passes, bkeys, bpay, bstart, bcount, bb + b_part,
-                           ix.dropped.as<unsigned long long>(), s);
+      const uint64_t cap_b = windows_capacity(bm, window, sizeof(K), bits);
+      // the row ids of a batch's windows stay within the XCD that looks them up: row ids
+      // in tuple order, then gathered back to probe order through the recorded positions
+      partition_windows<K>(probes + b0, bm, window, bits, ignore, passes, bkeys, binv1, binv2, bstart, bcount,
+                           bb + b_part, ix.dropped.as<unsigned long long>(), s);
       with_index_dev<K>(ix, [&](auto d) {
-        launch_tuples(d, bkeys, bpay, bstart + ceil_div(bm, window) * fan, windows_capacity(bm, window, sizeof(K), bits),
-                      out + b0, tp, s, ix.cu);
+        launch_tuples(d, bkeys, static_cast<const uint32_t*>(nullptr), bstart + ceil_div(bm, window) * fan, cap_b,
+                      bres, tp, s, ix.cu);
       });
+      launch_unpartition(binv1, binv2, bres, cap_b, bm, out + b0, s, ix.cu);
     }
     return;
   }

@@ -148,13 +148,17 @@ uint64_t hash_capacity(uint64_t n);
 uint64_t partition_capacity(uint64_t m, int key_bytes, uint32_t bits);
 uint64_t partition_scratch(uint64_t m, int key_bytes, uint32_t bits, uint32_t passes);
 // A batch of m probes cut into windows of `window`, each window partitioned on its own
-// (output partition w * 2^bits + digit); the payload is the position in the batch.
+// (output partition w * 2^bits + digit).  No payloads: inv1[i] is where probe i went;
+// when windows_two_pass(), that is a position in the first pass's output, and
+// inv2[inv1[i]] its final position (inv2 is null otherwise).  inv1 and inv2 hold m
+// entries each.
 uint64_t windows_capacity(uint64_t m, uint64_t window, int key_bytes, uint32_t bits);
 uint64_t windows_scratch(uint64_t m, uint64_t window, int key_bytes, uint32_t bits, uint32_t passes);
+bool windows_two_pass(uint32_t bits, uint32_t passes);
 template <class K>
 void partition_windows(const K* keys, uint64_t m, uint64_t window, uint32_t bits, uint32_t ignore, uint32_t passes,
-                       K* out_keys, uint32_t* out_payload, uint64_t* start, uint64_t* count, uint8_t* scratch,
-                       unsigned long long* dropped, hipStream_t s);
+                       K* out_keys, uint32_t* inv1, uint32_t* inv2, uint64_t* start, uint64_t* count,
+                       uint8_t* scratch, unsigned long long* dropped, hipStream_t s);
 template <class K>
 void partition_keys(const K* keys, uint64_t m, uint32_t payload_base, uint32_t bits, uint32_t ignore, uint32_t passes,
                     bool stable, K* out_keys, uint32_t* out_payload, uint64_t* start, uint64_t* count, uint8_t* scratch,

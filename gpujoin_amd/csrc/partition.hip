@@ -182,6 +182,111 @@ __global__ __launch_bounds__(1024) void part_scan_starts_kernel(const uint64_t* 
   if (threadIdx.x == blockDim.x - 1) start[fan] = s[threadIdx.x];  // padded total
 }
 
+// The same starts over many partitions (a batch of windows brings up to 2^18, where one
+// block took 420 µs): block b owns the kScanSpan partitions [b * kScanSpan, ...).
+// part_span_sums_kernel writes each span's padded total to sums[b];
+// part_span_starts_kernel adds the totals of the spans before its own and scans its span
+// through LDS (loads and stores coalesced).
+constexpr uint32_t kScanThreads = 256;
+constexpr uint32_t kScanSpan = kScanThreads * kScanBatch;  // 2048 partitions per block
+constexpr uint32_t kOneBlockScanMax = 8192;                // up to here the one-block kernel
+constexpr uint64_t kScanSumsBytes = 8192;                  // sums of up to 1024 spans (2^21 partitions)
+
+__device__ __forceinline__ uint64_t padded_count(uint64_t c, uint32_t align) { return (c + align - 1) / align * align; }
+
+// Sum over the block of one value per thread; every thread gets the total.
+__device__ __forceinline__ uint64_t span_block_sum(uint64_t v, uint64_t* s) {
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (uint32_t off = kScanThreads / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+    __syncthreads();
+  }
+  const uint64_t t = s[0];
+  __syncthreads();
+  return t;
+}
+
+__global__ __launch_bounds__(kScanThreads) void part_span_sums_kernel(const uint64_t* __restrict__ count, uint32_t fan,
+                                                                     uint32_t align, uint64_t* __restrict__ sums) {
+  __shared__ uint64_t s[kScanThreads];
+  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kScanSpan;
+  uint64_t c[kScanBatch];
+#pragma unroll
+  for (uint32_t u = 0; u < kScanBatch; ++u) {
+    const uint64_t p = base + u * kScanThreads + threadIdx.x;
+    c[u] = p < fan ? count[p] : 0;
+  }
+  uint64_t local = 0;
+#pragma unroll
+  for (uint32_t u = 0; u < kScanBatch; ++u) local += padded_count(c[u], align);
+  const uint64_t t = span_block_sum(local, s);
+  if (threadIdx.x == 0) sums[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(kScanThreads) void part_span_starts_kernel(const uint64_t* __restrict__ count,
+                                                                       uint32_t fan, uint32_t align,
+                                                                       const uint64_t* __restrict__ sums,
+                                                                       uint64_t* __restrict__ start) {
+  __shared__ uint64_t v[kScanSpan];
+  __shared__ uint64_t s[kScanThreads];
+  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kScanSpan;
+  uint64_t before = 0;
+  for (uint32_t b = threadIdx.x; b < blockIdx.x; b += kScanThreads) before += sums[b];
+#pragma unroll
+  for (uint32_t u = 0; u < kScanBatch; ++u) {
+    const uint64_t p = base + u * kScanThreads + threadIdx.x;
+    v[u * kScanThreads + threadIdx.x] = p < fan ? padded_count(count[p], align) : 0;
+  }
+  const uint64_t off = span_block_sum(before, s);  // also orders the stores to v before the reads below
+  // thread t scans v[t * kScanBatch, (t + 1) * kScanBatch)
+  uint64_t local = 0;
+#pragma unroll
+  for (uint32_t u = 0; u < kScanBatch; ++u) local += v[threadIdx.x * kScanBatch + u];
+  s[threadIdx.x] = local;
+  __syncthreads();
+  for (uint32_t o = 1; o < kScanThreads; o <<= 1) {  // Hillis-Steele inclusive scan
+    const uint64_t y = threadIdx.x >= o ? s[threadIdx.x - o] : 0;
+    __syncthreads();
+    s[threadIdx.x] += y;
+    __syncthreads();
+  }
+  uint64_t run = off + s[threadIdx.x] - local;
+#pragma unroll
+  for (uint32_t u = 0; u < kScanBatch; ++u) {
+    const uint64_t x = v[threadIdx.x * kScanBatch + u];
+    v[threadIdx.x * kScanBatch + u] = run;
+    run += x;
+  }
+  __syncthreads();
+#pragma unroll
+  for (uint32_t u = 0; u < kScanBatch; ++u) {
+    const uint64_t p = base + u * kScanThreads + threadIdx.x;
+    if (p < fan) start[p] = v[u * kScanThreads + threadIdx.x];
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == kScanThreads - 1) start[fan] = off + s[kScanThreads - 1];
+}
+
+// start[0 .. fan] from count[0 .. fan) (start[fan] = padded total); sums: kScanSumsBytes
+// of scratch, used above kOneBlockScanMax partitions.
+inline void launch_scan_starts(const uint64_t* count, uint64_t fan, uint32_t align, uint64_t* start, uint64_t* sums,
+                               hipStream_t s) {
+  if (fan <= kOneBlockScanMax) {
+    hipLaunchKernelGGL(part_scan_starts_kernel, dim3(1), dim3(1024), 0, s, count, static_cast<uint32_t>(fan), align,
+                       start);
+    GJ_HIP_CHECK(hipGetLastError());
+    return;
+  }
+  const uint64_t spans = ceil_div(fan, kScanSpan);
+  GJ_REQUIRE(spans * 8 <= kScanSumsBytes, GJ_ERR_INVALID, "too many partitions for the starts scan");
+  hipLaunchKernelGGL(part_span_sums_kernel, dim3(static_cast<unsigned>(spans)), dim3(kScanThreads), 0, s, count,
+                     static_cast<uint32_t>(fan), align, sums);
+  GJ_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(part_span_starts_kernel, dim3(static_cast<unsigned>(spans)), dim3(kScanThreads), 0, s, count,
+                     static_cast<uint32_t>(fan), align, sums, start);
+  GJ_HIP_CHECK(hipGetLastError());
+}
+
 // Marks the padding slots after every partition (fewer than `align` each) with payload
 // 0xFFFFFFFF and key_max; one thread per partition, grid stride (a batch of windows has
 // up to 2^18 partitions).
@@ -194,7 +299,7 @@ __global__ void part_pad_kernel(const uint64_t* __restrict__ start, const uint64
     const uint64_t stop = start[p] + (count[p] + align - 1) / align * align;
     for (uint64_t i = end; i < stop; ++i) {
       keys[i] = key_max<K>();
-      payload[i] = 0xFFFFFFFFu;
+      if (payload) payload[i] = 0xFFFFFFFFu;
     }
   }
 }
@@ -215,8 +320,8 @@ template <class K>
 __global__ __launch_bounds__(kPartBlock) void part_scatter_kernel(
     const K* __restrict__ keys, const uint32_t* __restrict__ in_payload, uint32_t payload_base, uint64_t m,
     uint64_t chunk, uint32_t shift, uint32_t bits, const uint32_t* __restrict__ block_off,
-    const uint64_t* __restrict__ start, K* __restrict__ out_keys, uint32_t* __restrict__ out_payload, uint64_t cap,
-    unsigned long long* __restrict__ dropped, Seg sg) {
+    const uint64_t* __restrict__ start, K* __restrict__ out_keys, uint32_t* __restrict__ out_payload,
+    uint32_t* __restrict__ inv, uint64_t cap, unsigned long long* __restrict__ dropped, Seg sg) {
   extern __shared__ uint32_t sm[];
   const uint32_t fan = 1u << bits;
   uint32_t* cnt = sm;                        // [kPartWaves][fan]: counts, then cursors
@@ -287,10 +392,11 @@ __global__ __launch_bounds__(kPartBlock) void part_scatter_kernel(
         const uint32_t pos = at + static_cast<uint32_t>(__popcll(peers & lt));
         if (pos < cap) {  // always true for a consistent histogram
           out_keys[pos] = k[j];
-          out_payload[pos] = pay[j];
+          if (out_payload) out_payload[pos] = pay[j];
         } else {
           drop(dropped);
         }
+        if (inv) inv[wbase + static_cast<uint64_t>(j * kWave + ln)] = pos;
         if ((peers & lt) == 0) mine[d] = at + static_cast<uint32_t>(__popcll(peers));
       }
       __builtin_amdgcn_wave_barrier();
@@ -397,8 +503,8 @@ template <class K, bool Stable>
 __global__ __launch_bounds__(wc_block(Stable)) void part_scatter_wc_kernel(
     const K* __restrict__ keys, const uint32_t* __restrict__ in_payload, uint32_t payload_base, uint64_t m,
     uint64_t chunk, uint32_t shift, uint32_t bits, uint32_t seg_bits, const uint32_t* __restrict__ block_off,
-    const uint64_t* __restrict__ start, K* __restrict__ out_keys, uint32_t* __restrict__ out_payload, uint64_t cap,
-    unsigned long long* __restrict__ dropped) {
+    const uint64_t* __restrict__ start, K* __restrict__ out_keys, uint32_t* __restrict__ out_payload,
+    uint32_t* __restrict__ inv, uint64_t cap, unsigned long long* __restrict__ dropped, Seg sg) {
   constexpr int kBlock = wc_block(Stable);
   constexpr int kWaves = kBlock / kWave;
   constexpr int kItems = kWcTile / kBlock;  // items per lane per tile
@@ -419,17 +525,17 @@ __global__ __launch_bounds__(wc_block(Stable)) void part_scatter_wc_kernel(
   uint32_t* cnt = endp + fan;                        // [kRows][fan]
   uint16_t* segp = reinterpret_cast<uint16_t*>(cnt + kRows * fan);  // segment -> partition
 
-  const uint64_t c = xcd_logical_block();
+  const uint64_t c = sg.segs ? static_cast<uint64_t>(blockIdx.x) : xcd_logical_block();  // as part_hist_kernel
+  uint64_t lo, hi, pbase;
+  if (!seg_range(sg, c, chunk, m, fan, lo, hi, pbase)) return;  // block-uniform, before any barrier
   for (uint32_t p = threadIdx.x; p < fan; p += blockDim.x) {
-    const uint32_t r = static_cast<uint32_t>(start[p]) + block_off[c * fan + p];
+    const uint32_t r = static_cast<uint32_t>(start[pbase + p]) + block_off[c * fan + p];
     cur[p] = r;
     run0[p] = r;
 #pragma unroll
     for (int v = 0; v < kRows; ++v) cnt[v * fan + p] = 0;
   }
   __syncthreads();
-  const uint64_t lo = c * chunk;
-  const uint64_t hi = std::min<uint64_t>(m, lo + chunk);
   const int w = threadIdx.x / kWave;
   const int ln = lane();
   const uint64_t lt = (ln == 0) ? 0ull : (~0ull >> (64 - ln));
@@ -551,6 +657,7 @@ __global__ __launch_bounds__(wc_block(Stable)) void part_scatter_wc_kernel(
       } else if (valid) {
         drop(dropped);
       }
+      if (inv && valid) inv[i] = pos[j];
     }
     __syncthreads();
     // store the complete segments, L lanes each
@@ -572,7 +679,7 @@ __global__ __launch_bounds__(wc_block(Stable)) void part_scatter_wc_kernel(
           pp = sp[t];
         }
         out_keys[at] = kk;
-        out_payload[at] = pp;
+        if (out_payload) out_payload[at] = pp;
       }
     }
     __syncthreads();
@@ -603,7 +710,7 @@ __global__ __launch_bounds__(wc_block(Stable)) void part_scatter_wc_kernel(
     if (at >= run0[p] && at < cur[p] && at >= cap) drop(dropped);
     if (at >= run0[p] && at < cur[p] && at < cap) {
       out_keys[at] = ck[p * L + q];
-      out_payload[at] = cp[p * L + q];
+      if (out_payload) out_payload[at] = cp[p * L + q];
     }
   }
 }
@@ -719,10 +826,11 @@ PartPlan plan(uint64_t m, uint32_t bits, int cu, uint32_t per_cu) {
   return p;
 }
 
-// Histogram rows for the plan with the most blocks any pass uses.
+// The starts scan's span sums, then the histogram rows for the plan with the most
+// blocks any pass uses.
 uint64_t hist_bytes(uint64_t m, uint32_t bits, int cu) {
   const PartPlan p = plan(m, bits, cu, kMaxBlocksPerCu);
-  return ceil_div(static_cast<uint64_t>(p.blocks) * p.fan * 4 + 256, 256) * 256;
+  return kScanSumsBytes + ceil_div(static_cast<uint64_t>(p.blocks) * p.fan * 4 + 256, 256) * 256;
 }
 
 // Segment size (log2 of slots) of the write-combining scatter: the largest with
@@ -749,11 +857,12 @@ uint64_t seg_max_chunks(uint64_t m, uint32_t bits) {
   return ceil_div(m, kPartTile) + (1ull << seg_high_bits(bits));
 }
 
-// Histogram region of a partitioning (either pass's rows).
+// Histogram region of a partitioning (the span sums, then either pass's rows).
 uint64_t hist_region(uint64_t m, uint32_t bits, int cu) {
   if (bits <= kMaxSinglePassBits) return hist_bytes(m, bits, cu);
   const uint64_t seg_rows = seg_max_chunks(m, bits) * (1ull << seg_low_bits(bits)) * 4;
-  return std::max<uint64_t>(hist_bytes(m, seg_high_bits(bits), cu), ceil_div(seg_rows + 256, 256) * 256);
+  return std::max<uint64_t>(hist_bytes(m, seg_high_bits(bits), cu),
+                            kScanSumsBytes + ceil_div(seg_rows + 256, 256) * 256);
 }
 
 // histograms | (2 passes: first-pass keys and payloads [| segment starts, counts, chunk
@@ -775,7 +884,8 @@ uint64_t scratch_for(uint64_t m, int key_bytes, uint32_t bits, uint32_t passes, 
 template <class K>
 void partition_pass(const K* keys, const uint32_t* in_payload, uint32_t payload_base, uint64_t m, uint32_t shift,
                     uint32_t bits, uint32_t align, bool stable, K* out_keys, uint32_t* out_payload, uint64_t* start,
-                    uint64_t* count, uint32_t* hist, unsigned long long* dropped, int cu, hipStream_t s) {
+                    uint64_t* count, uint32_t* hist, uint64_t* sums, unsigned long long* dropped, int cu,
+                    hipStream_t s) {
   const uint32_t sb = wc_seg_bits(bits, sizeof(K), stable);
   size_t slds = 0;
   uint32_t per_cu = bits > 10 ? 2 : 4;  // the register-tile scatter holds 5 x 4 B per partition
@@ -791,8 +901,7 @@ void partition_pass(const K* keys, const uint32_t* in_payload, uint32_t payload_
   GJ_HIP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(part_colscan_kernel, dim3((p.fan + 3) / 4), dim3(256), 0, s, hist, p.blocks, p.fan, count);
   GJ_HIP_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(part_scan_starts_kernel, dim3(1), dim3(1024), 0, s, count, p.fan, align, start);
-  GJ_HIP_CHECK(hipGetLastError());
+  launch_scan_starts(count, p.fan, align, start, sums, s);
   if (align > 1) {
     hipLaunchKernelGGL(part_pad_kernel<K>, dim3(pad_grid(p.fan)), dim3(256), 0, s, start, count, p.fan, align,
                        out_keys, out_payload);
@@ -805,7 +914,7 @@ void partition_pass(const K* keys, const uint32_t* in_payload, uint32_t payload_
         GJ_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
                                          hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(slds)));
       hipLaunchKernelGGL(kernel, dim3(p.blocks), dim3(wc_block(stable)), slds, s, keys, in_payload, payload_base, m,
-                         p.chunk, shift, bits, sb, hist, start, out_keys, out_payload, cap, dropped);
+                         p.chunk, shift, bits, sb, hist, start, out_keys, out_payload, nullptr, cap, dropped, Seg{});
       GJ_HIP_CHECK(hipGetLastError());
     };
     if (stable)
@@ -825,7 +934,7 @@ void partition_pass(const K* keys, const uint32_t* in_payload, uint32_t payload_
     GJ_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&part_scatter_kernel<K>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(rlds)));
   hipLaunchKernelGGL(part_scatter_kernel<K>, dim3(p.blocks), dim3(kPartBlock), rlds, s, keys, in_payload, payload_base,
-                     m, p.chunk, shift, bits, hist, start, out_keys, out_payload, cap, dropped, Seg{});
+                     m, p.chunk, shift, bits, hist, start, out_keys, out_payload, nullptr, cap, dropped, Seg{});
   GJ_HIP_CHECK(hipGetLastError());
 }
 
@@ -839,44 +948,60 @@ uint64_t partition_scratch(uint64_t m, int key_bytes, uint32_t bits, uint32_t pa
 
 // The second pass above 2^12 partitions: the first pass left segment h (its partition
 // on the high bits) at [start1[h], start1[h] + count1[h]) of keys / payload; each segment
-// is partitioned, stably, on the low `bits` bits (digit (k >> shift) & (2^bits - 1)) into
-// output partitions h * 2^bits + p, with 128-byte aligned starts.  Chunks of one tile
-// never straddle segments, so every histogram row and cursor belongs to one segment.
+// is partitioned on the low `bits` bits (digit (k >> shift) & (2^bits - 1)) into output
+// partitions h * 2^bits + p, with 128-byte aligned starts.  Chunks never straddle
+// segments, so every histogram row and cursor belongs to one segment.  Stable: tiles of
+// kPartTile items through the register-tile scatter.  Unordered: chunks of kSegWcChunk
+// items through the write-combining scatter where its buffers fit (the register tiles
+// write each wave's few items per partition as partial lines: 52 B of writes per 12 B
+// tuple at 2^8 partitions per 2^16-item segment).
+constexpr uint64_t kSegWcChunk = 8 * kWcTile;
+
 template <class K>
 void partition_segmented(const K* keys, const uint32_t* payload, uint64_t m, uint32_t shift, uint32_t bits,
                          uint32_t segs, const uint64_t* start1, const uint64_t* count1, uint64_t* chunk0,
-                         uint32_t align, K* out_keys, uint32_t* out_payload, uint64_t* start, uint64_t* count,
-                         uint32_t* hist, unsigned long long* dropped, hipStream_t s) {
+                         uint32_t align, bool stable, K* out_keys, uint32_t* out_payload, uint32_t* inv,
+                         uint64_t* start, uint64_t* count, uint32_t* hist, uint64_t* sums,
+                         unsigned long long* dropped, hipStream_t s) {
   const uint32_t fan = 1u << bits;
   const uint64_t total_fan = static_cast<uint64_t>(segs) * fan;
-  const uint64_t blocks = ceil_div(m, kPartTile) + segs;  // an upper bound on the chunks
+  const uint32_t sb = stable ? 0u : wc_seg_bits(bits, sizeof(K), false);
+  const uint64_t chunk = sb ? kSegWcChunk : static_cast<uint64_t>(kPartTile);
+  const uint64_t blocks = ceil_div(m, chunk) + segs;  // an upper bound on the chunks
   GJ_REQUIRE(blocks < (1ull << 31), GJ_ERR_INVALID, "too many partition chunks");
-  hipLaunchKernelGGL(seg_chunks_kernel, dim3(1), dim3(1024), 0, s, count1, segs, static_cast<uint64_t>(kPartTile),
-                     chunk0);
+  hipLaunchKernelGGL(seg_chunks_kernel, dim3(1), dim3(1024), 0, s, count1, segs, chunk, chunk0);
   GJ_HIP_CHECK(hipGetLastError());
   const Seg sg{chunk0, start1, count1, segs};
-  hipLaunchKernelGGL(part_hist_kernel<K>, dim3(static_cast<unsigned>(blocks)), dim3(kPartBlock), fan * sizeof(uint32_t),
-                     s, keys, m, static_cast<uint64_t>(kPartTile), shift, bits, hist, sg);
+  hipLaunchKernelGGL(part_hist_kernel<K>, dim3(static_cast<unsigned>(blocks)), dim3(sb ? 1024 : kPartBlock),
+                     fan * sizeof(uint32_t), s, keys, m, chunk, shift, bits, hist, sg);
   GJ_HIP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(seg_colscan_kernel, dim3(static_cast<unsigned>(ceil_div(total_fan, 256))), dim3(256), 0, s, hist,
                      chunk0, segs, fan, count);
   GJ_HIP_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(part_scan_starts_kernel, dim3(1), dim3(1024), 0, s, count, static_cast<uint32_t>(total_fan), align,
-                     start);
-  GJ_HIP_CHECK(hipGetLastError());
+  launch_scan_starts(count, total_fan, align, start, sums, s);
   if (align > 1) {
     hipLaunchKernelGGL(part_pad_kernel<K>, dim3(pad_grid(total_fan)), dim3(256), 0, s, start, count,
                        static_cast<uint32_t>(total_fan), align, out_keys, out_payload);
     GJ_HIP_CHECK(hipGetLastError());
   }
   const uint64_t cap = m + (align > 1 ? total_fan * align : 0ull);
+  if (sb) {
+    const size_t slds = wc_lds_bytes(fan, sb, sizeof(K), false);
+    const void* fn = reinterpret_cast<const void*>(&part_scatter_wc_kernel<K, false>);
+    if (slds > 65536)
+      GJ_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(slds)));
+    hipLaunchKernelGGL((part_scatter_wc_kernel<K, false>), dim3(static_cast<unsigned>(blocks)), dim3(wc_block(false)),
+                       slds, s, keys, payload, 0u, m, chunk, shift, bits, sb, hist, start, out_keys, out_payload, inv,
+                       cap, dropped, sg);
+    GJ_HIP_CHECK(hipGetLastError());
+    return;
+  }
   const size_t rlds = static_cast<size_t>(kPartWaves + 1) * fan * sizeof(uint32_t);  // <= 80 KiB of the 160
   if (rlds > 65536)
     GJ_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&part_scatter_kernel<K>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(rlds)));
   hipLaunchKernelGGL(part_scatter_kernel<K>, dim3(static_cast<unsigned>(blocks)), dim3(kPartBlock), rlds, s, keys,
-                     payload, 0u, m, static_cast<uint64_t>(kPartTile), shift, bits, hist, start, out_keys,
-                     out_payload, cap, dropped, sg);
+                     payload, 0u, m, chunk, shift, bits, hist, start, out_keys, out_payload, inv, cap, dropped, sg);
   GJ_HIP_CHECK(hipGetLastError());
 }
 
@@ -886,7 +1011,9 @@ namespace {
 // segmented pass, so each window is partitioned on its own (output partition
 // w * 2^bits + digit, window-major) while the batch takes one launch per kernel.  Two
 // passes (or a fan-out above 2^12): a segmented pass per window on the high bits, then a
-// segmented pass per (window, high digit) on the low bits.
+// segmented pass per (window, high digit) on the low bits.  No payloads: each pass
+// records, in input order, where it put every item (inv1 for the first pass, inv2 for
+// the second), which the lookup driver follows back to probe order.
 uint32_t win_high_bits(uint32_t bits, uint32_t passes) {
   if (bits > kMaxSinglePassBits) return seg_high_bits(bits);
   return (passes == 2 && bits >= 2) ? (bits + 1) / 2 : 0u;  // 0: one pass
@@ -895,7 +1022,7 @@ uint32_t win_high_bits(uint32_t bits, uint32_t passes) {
 struct WinLayout {
   uint64_t windows, segs_b;  // windows; segments of the last pass
   uint32_t hb, lb;           // digit bits of the first pass (0: one pass) and of the last
-  uint64_t o_tk, o_tp, o_wstart, o_wcount, o_sa, o_ca, o_chunk0, bytes;
+  uint64_t o_tk, o_wstart, o_wcount, o_sa, o_ca, o_chunk0, bytes;
 };
 
 WinLayout win_layout(uint64_t m, uint64_t window, int key_bytes, uint32_t bits, uint32_t passes) {
@@ -908,11 +1035,9 @@ WinLayout win_layout(uint64_t m, uint64_t window, int key_bytes, uint32_t bits, 
   const uint64_t tiles = ceil_div(m, static_cast<uint64_t>(kPartTile));
   const uint64_t rows_a = L.hb ? (tiles + L.windows) << L.hb : 0;
   const uint64_t rows_b = (tiles + L.segs_b) << L.lb;
-  uint64_t o = up(std::max(rows_a, rows_b) * 4);  // histogram rows of either pass
+  uint64_t o = kScanSumsBytes + up(std::max(rows_a, rows_b) * 4);  // span sums, histogram rows of either pass
   L.o_tk = o;
   if (L.hb) o += up((m + 32) * key_bytes);
-  L.o_tp = o;
-  if (L.hb) o += up((m + 32) * 4);
   L.o_wstart = o;
   o += up((L.windows + 2) * 8);
   L.o_wcount = o;
@@ -937,14 +1062,18 @@ uint64_t windows_scratch(uint64_t m, uint64_t window, int key_bytes, uint32_t bi
   return win_layout(m, window, key_bytes, bits, passes).bytes;
 }
 
+bool windows_two_pass(uint32_t bits, uint32_t passes) { return win_high_bits(bits, passes) != 0; }
+
 template <class K>
 void partition_windows(const K* keys, uint64_t m, uint64_t window, uint32_t bits, uint32_t ignore, uint32_t passes,
-                       K* out_keys, uint32_t* out_payload, uint64_t* start, uint64_t* count, uint8_t* scratch,
-                       unsigned long long* dropped, hipStream_t s) {
+                       K* out_keys, uint32_t* inv1, uint32_t* inv2, uint64_t* start, uint64_t* count,
+                       uint8_t* scratch, unsigned long long* dropped, hipStream_t s) {
   if (m == 0) return;
   const WinLayout L = win_layout(m, window, sizeof(K), bits, passes);
+  GJ_REQUIRE(inv1 && (!L.hb || inv2), GJ_ERR_INVALID, "partition_windows: missing position arrays");
   GJ_REQUIRE((L.segs_b << L.lb) < (1ull << 31), GJ_ERR_INVALID, "too many partitions in a batch of windows");
-  uint32_t* hist = reinterpret_cast<uint32_t*>(scratch);
+  auto* sums = reinterpret_cast<uint64_t*>(scratch);
+  uint32_t* hist = reinterpret_cast<uint32_t*>(scratch + kScanSumsBytes);
   auto* wstart = reinterpret_cast<uint64_t*>(scratch + L.o_wstart);
   auto* wcount = reinterpret_cast<uint64_t*>(scratch + L.o_wcount);
   auto* chunk0 = reinterpret_cast<uint64_t*>(scratch + L.o_chunk0);
@@ -954,24 +1083,23 @@ void partition_windows(const K* keys, uint64_t m, uint64_t window, uint32_t bits
   const uint32_t align = part_align(sizeof(K));
   if (!L.hb) {
     partition_segmented<K>(keys, nullptr, m, ignore, bits, static_cast<uint32_t>(L.windows), wstart, wcount, chunk0,
-                           align, out_keys, out_payload, start, count, hist, dropped, s);
+                           align, false, out_keys, nullptr, inv1, start, count, hist, sums, dropped, s);
     return;
   }
   K* tk = reinterpret_cast<K*>(scratch + L.o_tk);
-  auto* tp = reinterpret_cast<uint32_t*>(scratch + L.o_tp);
   auto* sa = reinterpret_cast<uint64_t*>(scratch + L.o_sa);
   auto* ca = reinterpret_cast<uint64_t*>(scratch + L.o_ca);
   partition_segmented<K>(keys, nullptr, m, ignore + L.lb, L.hb, static_cast<uint32_t>(L.windows), wstart, wcount,
-                         chunk0, 1, tk, tp, sa, ca, hist, dropped, s);
-  partition_segmented<K>(tk, tp, m, ignore, L.lb, static_cast<uint32_t>(L.segs_b), sa, ca, chunk0, align, out_keys,
-                         out_payload, start, count, hist, dropped, s);
+                         chunk0, 1, false, tk, nullptr, inv1, sa, ca, hist, sums, dropped, s);
+  partition_segmented<K>(tk, nullptr, m, ignore, L.lb, static_cast<uint32_t>(L.segs_b), sa, ca, chunk0, align, false,
+                         out_keys, nullptr, inv2, start, count, hist, sums, dropped, s);
 }
 template void partition_windows<uint32_t>(const uint32_t*, uint64_t, uint64_t, uint32_t, uint32_t, uint32_t,
-                                          uint32_t*, uint32_t*, uint64_t*, uint64_t*, uint8_t*, unsigned long long*,
-                                          hipStream_t);
+                                          uint32_t*, uint32_t*, uint32_t*, uint64_t*, uint64_t*, uint8_t*,
+                                          unsigned long long*, hipStream_t);
 template void partition_windows<uint64_t>(const uint64_t*, uint64_t, uint64_t, uint32_t, uint32_t, uint32_t,
-                                          uint64_t*, uint32_t*, uint64_t*, uint64_t*, uint8_t*, unsigned long long*,
-                                          hipStream_t);
+                                          uint64_t*, uint32_t*, uint32_t*, uint64_t*, uint64_t*, uint8_t*,
+                                          unsigned long long*, hipStream_t);
 
 // Full partitioning into `out` (1 or 2 passes; fan-outs above 2^12 always take two, the
 // second one segmented); scratch must hold partition_scratch() bytes; start must hold
@@ -980,7 +1108,8 @@ template <class K>
 void partition_keys(const K* keys, uint64_t m, uint32_t payload_base, uint32_t bits, uint32_t ignore, uint32_t passes,
                     bool stable, K* out_keys, uint32_t* out_payload, uint64_t* start, uint64_t* count, uint8_t* scratch,
                     unsigned long long* dropped, int cu, hipStream_t s) {
-  uint32_t* hist = reinterpret_cast<uint32_t*>(scratch);
+  auto* sums = reinterpret_cast<uint64_t*>(scratch);
+  uint32_t* hist = reinterpret_cast<uint32_t*>(scratch + kScanSumsBytes);
   const uint32_t align = part_align(sizeof(K));
   if (bits > kMaxSinglePassBits) {
     const uint32_t hb = seg_high_bits(bits), lb = seg_low_bits(bits);
@@ -993,14 +1122,14 @@ void partition_keys(const K* keys, uint64_t m, uint32_t payload_base, uint32_t b
     auto* count1 = start1 + segs + 2;
     auto* chunk0 = count1 + segs + 2;
     partition_pass<K>(keys, nullptr, payload_base, m, ignore + lb, hb, 1, stable, tk, tp, start1, count1, hist,
-                      dropped, cu, s);
-    partition_segmented<K>(tk, tp, m, ignore, lb, segs, start1, count1, chunk0, align, out_keys, out_payload, start,
-                           count, hist, dropped, s);
+                      sums, dropped, cu, s);
+    partition_segmented<K>(tk, tp, m, ignore, lb, segs, start1, count1, chunk0, align, stable, out_keys, out_payload,
+                           nullptr, start, count, hist, sums, dropped, s);
     return;
   }
   if (passes == 1 || bits < 2) {
     partition_pass<K>(keys, nullptr, payload_base, m, ignore, bits, align, stable, out_keys, out_payload, start, count,
-                      hist, dropped, cu, s);
+                      hist, sums, dropped, cu, s);
     return;
   }
   // pass 1: the high half of the digit, unpadded, into scratch; pass 2: the full digit.
@@ -1009,9 +1138,9 @@ void partition_keys(const K* keys, uint64_t m, uint32_t payload_base, uint32_t b
   K* tk = reinterpret_cast<K*>(tmp);
   uint32_t* tp = reinterpret_cast<uint32_t*>(tmp + ceil_div((m + 32) * sizeof(K), 256) * 256);
   partition_pass<K>(keys, nullptr, payload_base, m, ignore + (bits - b_hi), b_hi, 1, stable, tk, tp, start, count,
-                    hist, dropped, cu, s);
-  partition_pass<K>(tk, tp, 0, m, ignore, bits, align, stable, out_keys, out_payload, start, count, hist, dropped, cu,
-                    s);
+                    hist, sums, dropped, cu, s);
+  partition_pass<K>(tk, tp, 0, m, ignore, bits, align, stable, out_keys, out_payload, start, count, hist, sums,
+                    dropped, cu, s);
 }
 template void partition_keys<uint32_t>(const uint32_t*, uint64_t, uint32_t, uint32_t, uint32_t, uint32_t, bool,
                                        uint32_t*, uint32_t*, uint64_t*, uint64_t*, uint8_t*, unsigned long long*, int,

@@ -116,6 +116,9 @@ def test_partition_rejects_bad_args(gj, gpu):
     ({"radix_bits": 12, "passes": 2, "window": 5000}, 300007),
     ({"radix_bits": 14, "passes": 2, "window": 30000}, 300007),  # 16 windows per batch, two segmented passes
     ({"radix_bits": 16, "window": 1 << 16}, 300007),  # 4 windows per batch
+    # 33 windows, 4 write-combining chunks of 8 tiles each per window; 8448 partitions
+    # (the multi-block starts scan)
+    ({"radix_bits": 8, "window": 1 << 16}, 2_100_007),
     ({"radix_bits": 4, "window": 1}, 40003),  # one probe per window, 16384 windows per batch
     ({"radix_bits": 8, "window": 3}, 40003),
 ])
