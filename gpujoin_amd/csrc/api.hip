@@ -129,7 +129,7 @@ int gj_index_build(gj_index_kind kind, const void* h_sorted_keys, const void* d_
     build_index(*ix, s);
     if (ix->params.placement == GJ_PLACE_HOST) {
       GJ_HIP_CHECK(hipStreamSynchronize(s));
-      for (dbuf* b : {&ix->samples, &ix->btree_nodes, &ix->hm_keys, &ix->hm_children, &ix->rs_radix, &ix->rs_knots,
+      for (dbuf* b : {&ix->samples, &ix->btree_nodes, &ix->hm_keys, &ix->rs_radix, &ix->rs_knots,
                       &ix->lm_leaves, &ix->lm_mlp, &ix->lm_knots, &ix->lm_seps, &ix->lm_subs, &ix->lm_groups,
                       &ix->lm_lines, &ix->lm_line_leaves, &ix->ht})
         b->to_host();

@@ -30,7 +30,8 @@ unsigned grid_for(uint64_t work, unsigned block, unsigned cap = 65536) {
 //   key s of node i of level l        = largest key under child 33 i + s of level l+1
 //                                     = col[min(n, (33 i + s + 1) * span(l+1)) - 1]
 // with span(l) = 32 * 33^(levels below l) column keys per node
-// (get_largest_key / populate_inner_nodes, :245-295).
+// (get_largest_key / populate_inner_nodes, :245-295).  Only the separators are
+// stored; the lookup computes the children from the level starts.
 // ---------------------------------------------------------------------------
 struct HmLevels {
   uint32_t depth;
@@ -44,17 +45,6 @@ __device__ __forceinline__ uint32_t hm_level_of(const HmLevels& L, uint64_t g) {
   uint32_t l = 0;
   while (l + 1 < L.depth && L.first[l + 1] <= g) ++l;
   return l;
-}
-
-__global__ void hm_children_kernel(HmLevels L, uint64_t inner_nodes, uint64_t total, uint32_t* __restrict__ children) {
-  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-  for (uint64_t g = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; g < total; g += stride) {
-    const uint32_t l = hm_level_of(L, g);
-    uint64_t v;
-    if (l + 1 < L.depth) v = L.first[l + 1] + 33ull * (g - L.first[l]);
-    else v = 32ull * (g - inner_nodes);
-    children[g] = static_cast<uint32_t>(v);
-  }
 }
 
 template <class K>
@@ -111,12 +101,8 @@ void build_harmonia(gj_index& ix, hipStream_t s) {
   ix.hm_depth = L.depth;
   ix.hm_inner_nodes = L.first[L.depth - 1];
   ix.hm_nodes = first;
+  for (uint32_t l = 0; l < L.depth; ++l) ix.hm_first[l] = L.first[l];
   GJ_REQUIRE(ix.hm_nodes < 0xFFFFFFFFull && ix.n <= 0xFFFFFFFFull, GJ_ERR_OVERFLOW, "harmonia: too many nodes");
-  ix.hm_children.alloc((ix.hm_nodes + 4) * sizeof(uint32_t));  // padded for 16-byte LDS staging
-  fill_bytes(ix.hm_children.p, 0, ix.hm_children.bytes, s);
-  hipLaunchKernelGGL(hm_children_kernel, dim3(grid_for(ix.hm_nodes, 256)), dim3(256), 0, s, L, ix.hm_inner_nodes,
-                     ix.hm_nodes, ix.hm_children.as<uint32_t>());
-  GJ_HIP_CHECK(hipGetLastError());
   if (ix.hm_inner_nodes) {
     const uint64_t slots = ix.hm_inner_nodes * 32;
     ix.hm_keys.alloc(slots * sizeof(K));
@@ -128,8 +114,8 @@ void build_harmonia(gj_index& ix, hipStream_t s) {
   const uint64_t budget = ix.params.btree_lds_bytes;
   uint64_t bytes = 0;
   for (uint32_t l = 0; l + 1 < L.depth; ++l) {
-    const uint64_t add = L.count[l] * (32 * sizeof(K) + 4);
-    if (bytes + add + 16 > budget) break;
+    const uint64_t add = L.count[l] * 32 * sizeof(K);
+    if (bytes + add > budget) break;
     bytes += add;
     ix.hm_lds_levels = l + 1;
     ix.hm_lds_nodes = L.first[l] + L.count[l];
@@ -254,7 +240,7 @@ template void build_hash_table<uint32_t>(const gj_index&, dbuf&, uint64_t&, hipS
 template void build_hash_table<uint64_t>(const gj_index&, dbuf&, uint64_t&, hipStream_t);
 
 uint64_t extended_memory(const gj_index& ix) {
-  return ix.hm_keys.bytes + ix.hm_children.bytes + ix.rs_radix.bytes + ix.rs_knots.bytes +
+  return ix.hm_keys.bytes + ix.rs_radix.bytes + ix.rs_knots.bytes +
          ix.lm_leaves.bytes + ix.lm_mlp.bytes + ix.lm_knots.bytes + ix.lm_seps.bytes + ix.lm_subs.bytes +
          ix.lm_groups.bytes + ix.lm_lines.bytes + ix.lm_line_leaves.bytes + ix.ht.bytes;
 }

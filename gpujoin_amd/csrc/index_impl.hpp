@@ -48,8 +48,9 @@ struct gj_index {
   gj::BTreeLayout bt;
 
   // Harmonia
-  gj::dbuf hm_keys, hm_children;
+  gj::dbuf hm_keys;
   uint64_t hm_inner_nodes = 0, hm_lds_nodes = 0, hm_nodes = 0;
+  uint64_t hm_first[gj::kMaxLevels] = {};  // first node of each level
   uint32_t hm_depth = 0, hm_lds_levels = 0;
 
   // RadixSpline
@@ -114,8 +115,8 @@ struct gj_index {
     d.col = static_cast<const K*>(d_col);
     d.n = n;
     d.keys = hm_keys.as<const K>();
-    d.children = hm_children.as<const uint32_t>();
     d.inner_nodes = hm_inner_nodes;
+    for (int i = 0; i < gj::kMaxLevels; ++i) d.first[i] = hm_first[i];
     d.depth = hm_depth;
     d.lds_levels = hm_lds_levels;
     d.lds_nodes = hm_lds_nodes;

@@ -42,14 +42,17 @@ __device__ __forceinline__ LaneKeys<K, KPL> load_lane_keys(const K* p) {
 // Harmonia (harmonia_index, src/indexes.cuh:257-307; harmonia_tree,
 // src/harmonia.cuh:48-992).  Layout kept from the reference: 32 keys per node
 // (fanout 33), inner keys breadth first and padded with key_max, leaf keys =
-// the column, children[node] = first child (inner) or first row id (leaf),
-// so next = children[pos] + lb (src/harmonia.cuh:748-766).  Re-laid out for
-// CDNA4: children are uint32, the top levels (keys and children) sit in LDS
-// (the reference uses 42 KiB of __constant__, src/harmonia.cuh:45-46), and a
-// node is searched by a group of G lanes of the wave64 that fetch one
-// 128-byte line per load and read a u64 node's second line only when every
-// key of the first line is below the probe (the reference's early exit,
-// src/harmonia.cuh:581-589).
+// the column, next = children[pos] + lb (src/harmonia.cuh:748-766).  Re-laid
+// out for CDNA4: the build fills every node of a level but the last, so the
+// reference's children[] prefix sums have a closed form,
+//   children[node p of inner level l] = first[l + 1] + 33 (p - first[l]),
+// which the lookup computes instead of reading a 4-byte entry per level (at
+// 2^26 keys a 16.5 MiB array whose bottom-level reads miss L2); the top key
+// levels sit in LDS (the reference uses 42 KiB of __constant__,
+// src/harmonia.cuh:45-46); a node is searched by a group of G lanes of the
+// wave64 that fetch one 128-byte line per load and read a u64 node's second
+// line only when every key of the first line is below the probe (the
+// reference's early exit, src/harmonia.cuh:581-589).
 // ---------------------------------------------------------------------------
 template <class K, int G>
 struct HarmoniaDev {
@@ -62,20 +65,15 @@ struct HarmoniaDev {
   const K* col;
   uint64_t n;
   const K* keys;              // inner nodes x MK
-  const uint32_t* children;   // every node (inner and leaf)
   uint64_t inner_nodes;
   uint32_t depth;             // levels including the leaf level
   uint32_t lds_levels;        // top inner levels staged in LDS
   uint64_t lds_nodes;
+  uint64_t first[kMaxLevels];  // first node of level l (top = 0); first[depth - 1] = inner_nodes
 
-  __host__ __device__ size_t lds_key_bytes() const { return lds_nodes * MK * sizeof(K); }
-  __host__ __device__ size_t lds_bytes() const {
-    return lds_nodes ? lds_key_bytes() + ((lds_nodes * 4 + 15) / 16) * 16 : 0;
-  }
+  __host__ __device__ size_t lds_bytes() const { return lds_nodes * MK * sizeof(K); }
   __device__ void stage(uint8_t* lds) const {
-    if (!lds_nodes) return;
-    stage_to_lds(lds, keys, lds_key_bytes());
-    stage_to_lds(lds + lds_key_bytes(), children, ((lds_nodes * 4 + 15) / 16) * 16);
+    if (lds_nodes) stage_to_lds(lds, keys, lds_bytes());
   }
 
   // Count of node keys below each lane's probe (and whether the key at that
@@ -144,16 +142,14 @@ struct HarmoniaDev {
     uint64_t pos = 0;
     uint32_t d = 0;
     const K* lds_keys = reinterpret_cast<const K*>(lds);
-    const uint32_t* lds_children = reinterpret_cast<const uint32_t*>(lds + lds_key_bytes());
     for (; d < lds_levels; ++d) {
       const uint32_t lb = node_search(lds_keys, pos * MK, x, gbase, gl, eq);
-      const uint64_t next = static_cast<uint64_t>(lds_children[pos]) + lb;
+      const uint64_t next = first[d + 1] + 33ull * (pos - first[d]) + lb;
       pos = live ? next : 0;
     }
     for (; d + 1 < depth; ++d) {
-      const uint32_t first = children[pos];
       const uint32_t lb = node_search(keys, pos * MK, x, gbase, gl, eq);
-      const uint64_t next = static_cast<uint64_t>(first) + lb;
+      const uint64_t next = first[d + 1] + 33ull * (pos - first[d]) + lb;
       pos = live ? next : 0;
     }
     const uint64_t leaf = live ? pos - inner_nodes : 0;
