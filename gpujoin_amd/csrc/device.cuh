@@ -140,6 +140,34 @@ __host__ __device__ __forceinline__ uint64_t key_hash(uint64_t k) {
   }
 }
 
+// Inverses of the finalisers (each step undone in reverse order; the multipliers'
+// inverses mod 2^64 and 2^32): the hash-table build partitions hashed keys and
+// recovers each key from its hash.
+__host__ __device__ __forceinline__ uint64_t murmur3_64_inverse(uint64_t h) {
+  h ^= h >> 33;
+  h *= 0x9cb4b2f8129337dbull;  // inverse of 0xc4ceb9fe1a85ec53
+  h ^= h >> 33;
+  h *= 0x4f74430c22a54005ull;  // inverse of 0xff51afd7ed558ccd
+  h ^= h >> 33;
+  return h;
+}
+__host__ __device__ __forceinline__ uint32_t murmur3_32_inverse(uint32_t h) {
+  h ^= h >> 16;
+  h *= 0x7ed1b41du;  // inverse of 0xc2b2ae35
+  h ^= (h >> 13) ^ (h >> 26);
+  h *= 0xa5cb9243u;  // inverse of 0x85ebca6b
+  h ^= h >> 16;
+  return h;
+}
+template <class K>
+__host__ __device__ __forceinline__ uint64_t key_unhash(uint64_t h) {
+  if constexpr (sizeof(K) == 4) {
+    return murmur3_32_inverse(static_cast<uint32_t>(h));
+  } else {
+    return murmur3_64_inverse(h);
+  }
+}
+
 // Block-wide copy of `count` 16-byte units from global memory into LDS.
 __device__ __forceinline__ void stage_to_lds(void* lds, const void* src, uint64_t bytes) {
   const u32x4* s = reinterpret_cast<const u32x4*>(src);

@@ -183,7 +183,7 @@ void lookup_hash_join(const gj_index& ix, const K* probes, uint64_t m, uint32_t*
                       hipStream_t s) {
   uint64_t cap = 0;
   dbuf& table = ix.scratch_for(s).b;
-  build_hash_table<K>(ix, table, cap, s);
+  build_hash_table<K>(ix, table, cap, ix.scratch_for(s).a, s);
   HashDev<K> d{};
   d.table = table.as<const HashEntry>();
   d.mask = cap - 1;

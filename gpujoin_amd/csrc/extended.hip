@@ -216,6 +216,114 @@ __global__ void ht_insert_kernel(const K* __restrict__ col, uint64_t n, HashEntr
   }
 }
 
+// Atomic-free build of the table for large columns.  The hashed keys h = key_hash(k)
+// are radix-partitioned on bits [kHtRangeBits, cap_bits) of h, that is by the table
+// range of 2^kHtRangeBits slots that holds each key's home slot h & (cap - 1).  One
+// workgroup builds a range in LDS, linear probing with LDS compare-and-swap and the
+// key recovered from its hash, then writes the whole range out in full lines, empty
+// slots included (so the table needs no fill).  A key whose probe run passes the end of
+// its range is moved to the front of the range's tuples and counted; ht_spill_kernel
+// then inserts those keys into the finished table.  Every
+// key sits at or after its home slot with no empty slot between, which is all a
+// lookup (HashDev) relies on; only the order inside a run can differ from an
+// insert-built table.
+constexpr uint32_t kHtRangeBits = 13;  // 8192 slots: 96 KiB of LDS (8 B key + 4 B row id)
+constexpr uint32_t kHtRangeBlock = 1024;
+
+template <class K>
+__global__ __launch_bounds__(kHtRangeBlock) void ht_range_build_kernel(K* __restrict__ hkeys, uint32_t* __restrict__ rows,
+                                                                      const uint64_t* __restrict__ start,
+                                                                      const uint64_t* __restrict__ count,
+                                                                      uint32_t* __restrict__ spills,
+                                                                      HashEntry* __restrict__ table) {
+  constexpr uint32_t S = 1u << kHtRangeBits;
+  extern __shared__ unsigned long long lk[];  // [S] keys, then [S] row ids
+  uint32_t* lv = reinterpret_cast<uint32_t*>(lk + S);
+  __shared__ uint32_t nspill;
+  const uint32_t b = blockIdx.x;
+  for (uint32_t j = threadIdx.x; j < S; j += blockDim.x) {
+    lk[j] = ~0ull;
+    lv[j] = 0xFFFFFFFFu;
+  }
+  if (threadIdx.x == 0) nspill = 0;
+  __syncthreads();
+  const uint64_t lo = start[b], n = count[b];
+  for (uint64_t c0 = 0; c0 < n; c0 += blockDim.x) {
+    const uint64_t t = c0 + threadIdx.x;
+    const bool valid = t < n;
+    K h = K(0);
+    uint32_t r = 0;
+    if (valid) {
+      h = hkeys[lo + t];
+      r = rows[lo + t];
+    }
+    // every read of this chunk comes before the chunk's spill stores, which reuse
+    // slots already read (a spill's slot is below the count of tuples read so far)
+    __syncthreads();
+    if (valid) {
+      const unsigned long long k = key_unhash<K>(static_cast<uint64_t>(h));
+      bool placed = false;
+      for (uint32_t j = static_cast<uint32_t>(h) & (S - 1); j < S; ++j) {
+        const unsigned long long prev = atomicCAS(&lk[j], ~0ull, k);
+        if (prev == ~0ull || prev == k) {
+          lv[j] = r;
+          placed = true;
+          break;
+        }
+      }
+      if (!placed) {
+        const uint32_t q = atomicAdd(&nspill, 1u);
+        hkeys[lo + q] = h;
+        rows[lo + q] = r;
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) spills[b] = nspill;
+  u32x4* dst = reinterpret_cast<u32x4*>(table + (static_cast<uint64_t>(b) << kHtRangeBits));
+  for (uint32_t j = threadIdx.x; j < S; j += blockDim.x) {
+    u32x4 e;
+    e.x = static_cast<uint32_t>(lk[j]);
+    e.y = static_cast<uint32_t>(lk[j] >> 32);
+    e.z = lv[j];
+    e.w = 0xFFFFFFFFu;
+    store_stream(dst + j, e);
+  }
+}
+
+// The keys each range could not hold: one workgroup per range inserts them as
+// ht_insert_kernel does, starting at the range's end (every slot from such a key's home
+// to the end of its range was taken when it spilled).
+template <class K>
+__global__ __launch_bounds__(256) void ht_spill_kernel(const K* __restrict__ hkeys, const uint32_t* __restrict__ rows,
+                                                       const uint64_t* __restrict__ start,
+                                                       const uint32_t* __restrict__ spills, HashEntry* table,
+                                                       uint64_t mask) {
+  const uint64_t b = blockIdx.x;
+  const uint32_t ns = spills[b];
+  for (uint32_t q = threadIdx.x; q < ns; q += blockDim.x) {
+    const uint64_t h = static_cast<uint64_t>(hkeys[start[b] + q]);
+    const uint32_t r = rows[start[b] + q];
+    const unsigned long long k = key_unhash<K>(h);
+    uint64_t s = ((b + 1) << kHtRangeBits) & mask;
+    for (;;) {
+      const unsigned long long prev = atomicCAS(&table[s].key, ~0ull, k);
+      if (prev == ~0ull || prev == k) {
+        table[s].value = r;
+        break;
+      }
+      s = (s + 1) & mask;
+    }
+  }
+}
+
+template <class K>
+__global__ void ht_hash_kernel(const K* __restrict__ col, uint64_t n, K* __restrict__ h) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+    store_stream(h + i, static_cast<K>(key_hash<K>(static_cast<uint64_t>(load_stream(col + i)))));
+}
+
 }  // namespace
 
 // calculate_table_size, src/linear_probing_hashtable.cuh:74-79
@@ -225,19 +333,59 @@ uint64_t hash_capacity(uint64_t n) {
   return 1ull << bits;
 }
 
+// Tables of 2^14 .. 2^(kHtRangeBits + 16) slots are built by range (the partitioner
+// takes up to 2^16 partitions), smaller and larger ones by inserts.
+bool hash_build_by_range(uint64_t cap) {
+  const uint32_t bits = ilog2_floor(cap);
+  return bits > kHtRangeBits && bits <= kHtRangeBits + 16;
+}
+
 template <class K>
-void build_hash_table(const gj_index& ix, dbuf& table, uint64_t& cap, hipStream_t s) {
+void build_hash_table(const gj_index& ix, dbuf& table, uint64_t& cap, dbuf& temp, hipStream_t s) {
   GJ_REQUIRE(!(sizeof(K) == 8 && ix.max_key == ~0ull), GJ_ERR_INVALID,
              "hash: the key 2^64-1 is the empty-slot marker and cannot be stored");
   cap = hash_capacity(ix.n);
   table.reserve(cap * sizeof(HashEntry));
-  fill_bytes(table.p, 0xFF, cap * sizeof(HashEntry), s);
-  hipLaunchKernelGGL(ht_insert_kernel<K>, dim3(grid_for(ix.n, 256)), dim3(256), 0, s, static_cast<const K*>(ix.d_col),
-                     ix.n, table.as<HashEntry>(), cap - 1);
+  if (!hash_build_by_range(cap)) {
+    fill_bytes(table.p, 0xFF, cap * sizeof(HashEntry), s);
+    hipLaunchKernelGGL(ht_insert_kernel<K>, dim3(grid_for(ix.n, 256)), dim3(256), 0, s,
+                       static_cast<const K*>(ix.d_col), ix.n, table.as<HashEntry>(), cap - 1);
+    GJ_HIP_CHECK(hipGetLastError());
+    return;
+  }
+  const uint32_t bits = ilog2_floor(cap) - kHtRangeBits;  // ranges = partitions
+  const uint64_t ranges = 1ull << bits, n = ix.n;
+  const uint64_t tcap = partition_capacity(n, sizeof(K), bits);
+  // temp: hashes | partitioned hashes | row ids | start (ranges + 1) | count | spills | partitioner scratch
+  auto up = [](uint64_t b) { return (b + 255) / 256 * 256; };
+  const uint64_t o_pk = up(n * sizeof(K)), o_pr = o_pk + up(tcap * sizeof(K)), o_st = o_pr + up(tcap * 4),
+                 o_ct = o_st + up((ranges + 1) * 8), o_sp = o_ct + up(ranges * 8), o_ps = o_sp + up(ranges * 4),
+                 bytes = o_ps + up(partition_scratch(n, sizeof(K), bits, 1));
+  temp.reserve(bytes);
+  uint8_t* base = temp.as<uint8_t>();
+  K* hk = reinterpret_cast<K*>(base);
+  K* pk = reinterpret_cast<K*>(base + o_pk);
+  uint32_t* pr = reinterpret_cast<uint32_t*>(base + o_pr);
+  uint64_t* st = reinterpret_cast<uint64_t*>(base + o_st);
+  uint64_t* ct = reinterpret_cast<uint64_t*>(base + o_ct);
+  uint32_t* sp = reinterpret_cast<uint32_t*>(base + o_sp);
+  hipLaunchKernelGGL(ht_hash_kernel<K>, dim3(grid_for(n, 256)), dim3(256), 0, s, static_cast<const K*>(ix.d_col), n, hk);
+  GJ_HIP_CHECK(hipGetLastError());
+  // payload = row id; the digit is the range of the home slot
+  partition_keys<K>(hk, n, 0, bits, kHtRangeBits, 1, false, pk, pr, st, ct, base + o_ps,
+                    ix.dropped.as<unsigned long long>(), ix.cu, s);
+  const size_t lds = (1u << kHtRangeBits) * (8 + 4);
+  const void* fn = reinterpret_cast<const void*>(&ht_range_build_kernel<K>);
+  GJ_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+  hipLaunchKernelGGL(ht_range_build_kernel<K>, dim3(static_cast<unsigned>(ranges)), dim3(kHtRangeBlock), lds, s, pk, pr,
+                     st, ct, sp, table.as<HashEntry>());
+  GJ_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(ht_spill_kernel<K>, dim3(static_cast<unsigned>(ranges)), dim3(256), 0, s, pk, pr, st, sp,
+                     table.as<HashEntry>(), cap - 1);
   GJ_HIP_CHECK(hipGetLastError());
 }
-template void build_hash_table<uint32_t>(const gj_index&, dbuf&, uint64_t&, hipStream_t);
-template void build_hash_table<uint64_t>(const gj_index&, dbuf&, uint64_t&, hipStream_t);
+template void build_hash_table<uint32_t>(const gj_index&, dbuf&, uint64_t&, dbuf&, hipStream_t);
+template void build_hash_table<uint64_t>(const gj_index&, dbuf&, uint64_t&, dbuf&, hipStream_t);
 
 uint64_t extended_memory(const gj_index& ix) {
   return ix.hm_keys.bytes + ix.rs_radix.bytes + ix.rs_knots.bytes +
@@ -255,7 +403,12 @@ void build_extended(gj_index& ix, hipStream_t s) {
       build_btree<K>(ix, s);
       build_learned<K>(ix, s);
       break;
-    case GJ_HASH: build_hash_table<K>(ix, ix.ht, ix.ht_cap, s); break;
+    case GJ_HASH: {
+      dbuf temp;
+      build_hash_table<K>(ix, ix.ht, ix.ht_cap, temp, s);
+      GJ_HIP_CHECK(hipStreamSynchronize(s));  // temp is freed on return
+      break;
+    }
     default:
       throw error(GJ_ERR_UNSUPPORTED, std::string("index kind not built: ") + gj_index_type_name(ix.kind));
   }

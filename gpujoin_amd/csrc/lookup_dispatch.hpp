@@ -138,9 +138,10 @@ void with_index_dev(const gj_index& ix, F&& f) {
   throw error(GJ_ERR_INVALID, "unknown index kind");
 }
 
-// Hash table over the column into `table` (capacity written to cap).
+// Hash table over the column into `table` (capacity written to cap); temp: working
+// memory of the partitioned build (grown as needed).
 template <class K>
-void build_hash_table(const gj_index& ix, dbuf& table, uint64_t& cap, hipStream_t s);
+void build_hash_table(const gj_index& ix, dbuf& table, uint64_t& cap, dbuf& temp, hipStream_t s);
 uint64_t hash_capacity(uint64_t n);
 
 // Radix partitioner (partition.hip).  stable: input order kept inside every

@@ -448,3 +448,69 @@ def test_hash_large_column(gj, orc, gpu, kb):
         gj.synchronize()
         assert np.array_equal(out.numpy(), want), rep
     assert bt.stats()["dropped_tuples"] == 0
+
+
+def _unfmix(h, kb):
+    """Keys whose murmur3 finaliser (of the key's width) is h: each step of the finaliser
+    undone in reverse order, with the multipliers' inverses mod 2^64 / 2^32."""
+    bits = 8 * kb
+    dt = np.uint64 if kb == 8 else np.uint32
+    shifts, inv = ((33, 33, 33), (0x4F74430C22A54005, 0x9CB4B2F8129337DB)) if kb == 8 else \
+        ((16, 13, 16), (0xA5CB9243, 0x7ED1B41D))
+
+    def unshift(x, s):
+        out, t = x.copy(), x >> dt(s)
+        while np.any(t):
+            out ^= t
+            t >>= dt(s)
+        return out
+
+    x = h.astype(dt)
+    with np.errstate(over="ignore"):
+        x = unshift(x, shifts[2])
+        x = x * dt(inv[1])
+        x = unshift(x, shifts[1])
+        x = x * dt(inv[0])
+        x = unshift(x, shifts[0])
+    assert bits in (32, 64)
+    return x
+
+
+
+@pytest.mark.parametrize("kb", [4, 8])
+def test_hash_crowded_ranges(gj, orc, gpu, kb):
+    # keys chosen by their hashes: 12000 home slots in the first 8192-slot range of the
+    # 2^17-slot table and 3000 in its last 200 slots, so the build meets a range with more
+    # keys than slots (its excess runs on through the next ranges) and runs that wrap from
+    # the last slot to the first; 30000 more spread evenly
+    cap = 1 << 17
+    dt = np.uint64 if kb == 8 else np.uint32
+    rng = np.random.default_rng(0x7A + kb)
+
+    def keys_with_homes(lo, hi, count):
+        high = rng.integers(0, np.iinfo(dt).max, size=count, dtype=dt) & ~dt(cap - 1)
+        return _unfmix(high | rng.integers(lo, hi, size=count).astype(dt), kb)
+
+    keys = np.unique(np.concatenate([keys_with_homes(0, 8192, 12000), keys_with_homes(cap - 200, cap, 3000),
+                                     keys_with_homes(0, cap, 30000)]))
+    keys = keys[keys != np.iinfo(np.uint64).max]  # the empty-slot marker
+    fin = orc.murmur3_64 if kb == 8 else orc.murmur3_32
+    homes = np.array([fin(int(k)) & (cap - 1) for k in keys])
+    assert np.count_nonzero(homes < 8192) > 11900 and np.count_nonzero(homes >= cap - 200) > 2900
+    assert (1 << 16) < 2 * keys.size <= cap  # the table the build sizes has 2^17 slots
+    probes = np.concatenate([keys, rng.integers(0, np.iinfo(dt).max, size=20000, dtype=dt)])
+    want = orc.lookup_lower_bound(keys, probes)
+    col = gj.DeviceArray.from_numpy(keys)
+    dp = gj.DeviceArray.from_numpy(probes)
+    out = gj.DeviceArray(probes.size, np.uint32)
+    idx = gj.Index("hash", col, host_column=keys)
+    out.fill(0x5A)
+    idx.lookup(dp, out)
+    gj.synchronize()
+    assert np.array_equal(out.numpy(), want)
+    bs = gj.Index("binary_search", col, host_column=keys)
+    out.fill(0x5A)
+    bs.lookup(dp, out, approach="hj")
+    gj.synchronize()
+    assert np.array_equal(out.numpy(), want)
+    assert idx.stats()["dropped_tuples"] == 0 and bs.stats()["dropped_tuples"] == 0
