@@ -76,22 +76,19 @@ __device__ __forceinline__ uint32_t xcd_logical_block() {
   return (blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8;
 }
 
-// Lookup over a partitioned relation: tuple t has key keys[t] and original
-// probe index payload[t]; the row id is scattered back to the original
-// position (partitioned_lookup_kernel, src/index_lookup_partitioning.cu:218-248).
-// Padding slots carry payload 0xFFFFFFFF.  Each XCD takes one contiguous eighth
-// of the tuples, and the workgroups of an XCD sweep it together, one
-// workgroup-wide step apart, so the key range in flight on an XCD (hence the
-// index nodes and column lines its L2 must hold) spans a few partitions rather
-// than one partition per workgroup.  The tuple count is read from *d_total when
-// given (it is known only on the device).  payload == nullptr: every tuple, padding
-// included, is looked up and its row id written to out[t] in tuple order (full-line
-// stores), for unpartition_kernel to bring back to probe order.
+// Lookup over a partitioned relation (partitioned_lookup_kernel,
+// src/index_lookup_partitioning.cu:218-248): tuple t has key keys[t], and its row id
+// goes to res[t], in tuple order (full-line stores); the driver then gathers the row ids
+// back to probe order.  Padding tuples (key_max) are looked up too, their row ids never
+// read.  Each XCD takes one contiguous eighth of the tuples, and the workgroups of an
+// XCD sweep it together, one workgroup-wide step apart, so the key range in flight on
+// an XCD (hence the index nodes and column lines its L2 must hold) spans a few
+// partitions rather than one partition per workgroup.  The tuple count is read from
+// *d_total when given (it is known only on the device).
 template <class Dev, class K>
 __global__ __launch_bounds__(256) void lookup_tuples_kernel(Dev d, const K* __restrict__ keys,
-                                                            const uint32_t* __restrict__ payload,
                                                             const uint64_t* __restrict__ d_total, uint64_t total_max,
-                                                            uint32_t* __restrict__ out) {
+                                                            uint32_t* __restrict__ res) {
   extern __shared__ __attribute__((aligned(16))) uint8_t lds[];
   d.stage(lds);
   __syncthreads();
@@ -107,22 +104,10 @@ __global__ __launch_bounds__(256) void lookup_tuples_kernel(Dev d, const K* __re
   for (uint64_t base = lo + static_cast<uint64_t>(blockIdx.x / groups) * blockDim.x + w * kWave; base < hi;
        base += step) {
     const uint64_t i = base + static_cast<uint64_t>(lane());
-    if (!payload) {  // wave-uniform
-      const bool active = i < hi;
-      const K x = active ? load_stream(keys + i) : K(0);
-      const uint32_t r = d.lookup(x, active, lds);
-      if (active) store_stream(out + i, r);
-      continue;
-    }
-    uint32_t dst = 0xFFFFFFFFu;
-    K x = K(0);
-    if (i < hi) {
-      dst = load_stream(payload + i);
-      x = load_stream(keys + i);
-    }
-    const bool active = dst != 0xFFFFFFFFu;
+    const bool active = i < hi;
+    const K x = active ? load_stream(keys + i) : K(0);
     const uint32_t r = d.lookup(x, active, lds);
-    if (active) out[dst] = r;
+    if (active) store_stream(res + i, r);
   }
 }
 

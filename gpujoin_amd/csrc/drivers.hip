@@ -74,21 +74,25 @@ inline void launch_unpartition(const uint32_t* inv1, const uint32_t* inv2, const
 // keeps the digit on the top radix_bits significant bits.
 uint32_t key_bit_width(uint64_t max_key) { return max_key ? 64u - static_cast<uint32_t>(__builtin_clzll(max_key)) : 0u; }
 
-// Windows below kWindowBatch probes are partitioned a batch at a time (each window on
-// its own, all windows of the batch in one launch per kernel): one window per launch
-// costs about 36 µs of launches and small kernels, which bound 2^16-probe windows at
-// 1.8 G lookups/s.  A batch holds up to kBatchProbes probes and kMaxBatchPartitions
-// partitions.
-constexpr uint64_t kWindowBatch = 1ull << 22;
+// Windows are partitioned a batch at a time (each window on its own, all windows of the
+// batch in one launch per kernel): one window per launch costs about 36 µs of launches
+// and small kernels, which bound 2^16-probe windows at 1.8 G lookups/s.  A batch holds
+// up to kBatchProbes probes (or one larger window) and kMaxBatchPartitions partitions.
 constexpr uint64_t kBatchProbes = 1ull << 24;
 constexpr uint64_t kMaxBatchPartitions = 1ull << 18;
 
-// partitioning_approach (src/index_lookup_partitioning.cu:377-513): the probes
-// are cut into windows; per window a stable radix partition on the top key bits
-// (payload = global probe index), then a lookup over the partitioned tuples that
-// scatters row ids back to probe order.  The reference keeps two streams with a
-// device-wide sync per window pair; here the windows are stream-ordered on the
-// caller's stream and the tuple count never leaves the device.
+// partitioning_approach (src/index_lookup_partitioning.cu:377-513): the probes are cut
+// into windows; per window a radix partition on the top key bits, then a lookup over
+// the partitioned tuples, with each probe's row id written to its position in probe
+// order.  The reference carries each probe's position as the tuple payload and
+// scatters the row ids back (tids[payload] = tid, :218-248), a 4-byte store to a random
+// position of the window's output that the memory system writes back as a partial line;
+// here the partitioner records, in probe order, where it put each probe, the tuple
+// lookup writes the row ids in tuple order (full lines), and unpartition_kernel
+// gathers them back (2^26 keys, 2^28 probes, fan-out 2^10, 2^22-probe windows: 27.0 G
+// lookups/s against 19.5 with the scatter).  The reference keeps two streams with a
+// device-wide sync per window pair; here the windows are stream-ordered on the caller's
+// stream and the tuple counts never leave the device.
 template <class K>
 void lookup_partitioned(const gj_index& ix, const K* probes, uint64_t m, uint32_t* out, const gj_lookup_params& p,
                         hipStream_t s) {
@@ -102,75 +106,46 @@ void lookup_partitioned(const gj_index& ix, const K* probes, uint64_t m, uint32_
     ignore = w > bits ? w - bits : 0u;
   }
   GJ_REQUIRE(ignore < 8 * sizeof(K), GJ_ERR_INVALID, "ignore_bits must be below the key width");
+  if (m == 0) return;
   const uint64_t window = std::min<uint64_t>(p.window ? p.window : (1ull << 24), m);
   const uint64_t fan = 1ull << bits;
-  const uint64_t cap = partition_capacity(window, sizeof(K), bits);
-  GJ_REQUIRE(cap < 0xFFFFFFFFull, GJ_ERR_OVERFLOW, "window too large");
-  // scratch: keys | payloads | start (fan + 1) | count (fan) | partitioner scratch
-  auto up = [](uint64_t b) { return (b + 255) / 256 * 256; };
-  const uint64_t o_keys = 0;
-  const uint64_t o_pay = o_keys + up(cap * sizeof(K));
-  const uint64_t o_start = o_pay + up(cap * 4);
-  const uint64_t o_count = o_start + up((fan + 1) * 8);
-  const uint64_t o_part = o_count + up(fan * 8);
-  const uint64_t bytes = o_part + up(partition_scratch(window, sizeof(K), bits, passes));
-  dbuf& scratch = ix.scratch_for(s).a;
-  scratch.reserve(bytes);
-  uint8_t* base = scratch.as<uint8_t>();
-  K* tkeys = reinterpret_cast<K*>(base + o_keys);
-  uint32_t* tpay = reinterpret_cast<uint32_t*>(base + o_pay);
-  uint64_t* start = reinterpret_cast<uint64_t*>(base + o_start);
-  uint64_t* count = reinterpret_cast<uint64_t*>(base + o_count);
   // a column in host memory: one workgroup per CU for the lookup, so each XCD
   // sweeps a narrow key range and the host pages in flight stay within reach of
   // the GPU's address translation (2x on a 2^26-key column)
   gj_lookup_params tp = p;
   if ((ix.col_in_host || ix.params.placement == GJ_PLACE_HOST) && !tp.grid_blocks)
     tp.grid_blocks = static_cast<uint32_t>(ix.cu);
-  if (window < kWindowBatch && m > window) {
-    const uint64_t per = std::max<uint64_t>(1, std::min<uint64_t>(kBatchProbes / window, kMaxBatchPartitions >> bits));
-    const uint64_t batch = std::min<uint64_t>(per * window, m);
-    const uint64_t parts = ceil_div(batch, window) * fan;
-    const uint64_t bcap = windows_capacity(batch, window, sizeof(K), bits);
-    GJ_REQUIRE(bcap < 0xFFFFFFFFull, GJ_ERR_OVERFLOW, "window batch too large");
-    const bool two = windows_two_pass(bits, passes);
-    // scratch: keys | row ids in tuple order | inv1 | inv2 (two passes) | start (parts + 1) |
-    // count (parts) | partitioner scratch
-    const uint64_t b_res = up(bcap * sizeof(K)), b_inv1 = b_res + up(bcap * 4), b_inv2 = b_inv1 + up(batch * 4),
-                   b_start = b_inv2 + (two ? up(batch * 4) : 0), b_count = b_start + up((parts + 1) * 8),
-                   b_part = b_count + up(parts * 8),
-                   b_bytes = b_part + up(windows_scratch(batch, window, sizeof(K), bits, passes));
-    scratch.reserve(b_bytes);
-    uint8_t* bb = scratch.as<uint8_t>();
-    K* bkeys = reinterpret_cast<K*>(bb);
-    uint32_t* bres = reinterpret_cast<uint32_t*>(bb + b_res);
-    uint32_t* binv1 = reinterpret_cast<uint32_t*>(bb + b_inv1);
-    uint32_t* binv2 = two ? reinterpret_cast<uint32_t*>(bb + b_inv2) : nullptr;
-    uint64_t* bstart = reinterpret_cast<uint64_t*>(bb + b_start);
-    uint64_t* bcount = reinterpret_cast<uint64_t*>(bb + b_count);
-    for (uint64_t b0 = 0; b0 < m; b0 += batch) {
-      const uint64_t bm = std::min<uint64_t>(batch, m - b0);
-      const uint64_t cap_b = windows_capacity(bm, window, sizeof(K), bits);
-      // the row ids of a batch's windows stay within the XCD that looks them up: row ids
-      // in tuple order, then gathered back to probe order through the recorded positions
-      partition_windows<K>(probes + b0, bm, window, bits, ignore, passes, bkeys, binv1, binv2, bstart, bcount,
-                           bb + b_part, ix.dropped.as<unsigned long long>(), s);
-      with_index_dev<K>(ix, [&](auto d) {
-        launch_tuples(d, bkeys, static_cast<const uint32_t*>(nullptr), bstart + ceil_div(bm, window) * fan, cap_b,
-                      bres, tp, s, ix.cu);
-      });
-      launch_unpartition(binv1, binv2, bres, cap_b, bm, out + b0, s, ix.cu);
-    }
-    return;
-  }
-  for (uint64_t w0 = 0; w0 < m; w0 += window) {
-    const uint64_t w = std::min<uint64_t>(window, m - w0);
-    // unordered: the payload carries each probe's position back to `out`
-    partition_keys<K>(probes + w0, w, static_cast<uint32_t>(w0), bits, ignore, passes, false, tkeys, tpay, start,
-                      count, base + o_part, ix.dropped.as<unsigned long long>(), ix.cu, s);
+  const uint64_t per = std::max<uint64_t>(1, std::min<uint64_t>(kBatchProbes / window, kMaxBatchPartitions >> bits));
+  const uint64_t batch = std::min<uint64_t>(per * window, m);
+  const uint64_t parts = ceil_div(batch, window) * fan;
+  const uint64_t bcap = windows_capacity(batch, window, sizeof(K), bits);
+  GJ_REQUIRE(bcap < 0xFFFFFFFFull, GJ_ERR_OVERFLOW, "window too large");
+  const bool two = windows_two_pass(bits, passes);
+  // scratch: keys | row ids in tuple order | inv1 | inv2 (two passes) | start (parts + 1) |
+  // count (parts) | partitioner scratch
+  auto up = [](uint64_t b) { return (b + 255) / 256 * 256; };
+  const uint64_t o_res = up(bcap * sizeof(K)), o_inv1 = o_res + up(bcap * 4), o_inv2 = o_inv1 + up(batch * 4),
+                 o_start = o_inv2 + (two ? up(batch * 4) : 0), o_count = o_start + up((parts + 1) * 8),
+                 o_part = o_count + up(parts * 8),
+                 bytes = o_part + up(windows_scratch(batch, window, sizeof(K), bits, passes));
+  dbuf& scratch = ix.scratch_for(s).a;
+  scratch.reserve(bytes);
+  uint8_t* base = scratch.as<uint8_t>();
+  K* tkeys = reinterpret_cast<K*>(base);
+  uint32_t* res = reinterpret_cast<uint32_t*>(base + o_res);
+  uint32_t* inv1 = reinterpret_cast<uint32_t*>(base + o_inv1);
+  uint32_t* inv2 = two ? reinterpret_cast<uint32_t*>(base + o_inv2) : nullptr;
+  uint64_t* start = reinterpret_cast<uint64_t*>(base + o_start);
+  uint64_t* count = reinterpret_cast<uint64_t*>(base + o_count);
+  for (uint64_t b0 = 0; b0 < m; b0 += batch) {
+    const uint64_t bm = std::min<uint64_t>(batch, m - b0);
+    const uint64_t cap_b = windows_capacity(bm, window, sizeof(K), bits);
+    partition_windows<K>(probes + b0, bm, window, bits, ignore, passes, tkeys, inv1, inv2, start, count,
+                         base + o_part, ix.dropped.as<unsigned long long>(), s);
     with_index_dev<K>(ix, [&](auto d) {
-      launch_tuples(d, tkeys, tpay, start + fan, partition_capacity(w, sizeof(K), bits), out, tp, s, ix.cu);
+      launch_tuples(d, tkeys, start + ceil_div(bm, window) * fan, cap_b, res, tp, s, ix.cu);
     });
+    launch_unpartition(inv1, inv2, res, cap_b, bm, out + b0, s, ix.cu);
   }
 }
 

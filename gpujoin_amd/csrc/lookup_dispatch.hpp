@@ -54,8 +54,8 @@ void launch_marked(const BTreeDev<K>& d, const K* probes, uint64_t m, uint32_t* 
 
 // total_max bounds the tuple count; d_total (optional) holds the exact count on the device.
 template <class Dev, class K>
-void launch_tuples(const Dev& d, const K* keys, const uint32_t* payload, const uint64_t* d_total, uint64_t total_max,
-                   uint32_t* out, const gj_lookup_params& p, hipStream_t s, int cu) {
+void launch_tuples(const Dev& d, const K* keys, const uint64_t* d_total, uint64_t total_max, uint32_t* res,
+                   const gj_lookup_params& p, hipStream_t s, int cu) {
   if (total_max == 0) return;
   const unsigned block = lookup_block(p);
   const size_t lds = d.lds_bytes();
@@ -65,7 +65,7 @@ void launch_tuples(const Dev& d, const K* keys, const uint32_t* payload, const u
   if (blocks >= 8) blocks = blocks / 8 * 8;  // whole XCD rounds for the XCD-major chunk map
   blocks = std::max<uint64_t>(blocks, 1);
   hipLaunchKernelGGL((lookup_tuples_kernel<Dev, K>), dim3(static_cast<unsigned>(blocks)), dim3(block), lds, s, d,
-                     keys, payload, d_total, total_max, out);
+                     keys, d_total, total_max, res);
   GJ_HIP_CHECK(hipGetLastError());
 }
 
