@@ -806,8 +806,13 @@ struct LinesDev {
 // src/linear_probing_hashtable.cuh:14-98): capacity 2^ceil(log2(n / 0.5)),
 // slot = murmur3(key) & (capacity - 1) with the finaliser of the key's width, step +1,
 // empty = all-ones key.
-// An entry is 16 bytes {u64 key, u32 row id, pad}; a probe reads one entry per
-// 16-byte load (the reference's ld.global.cs of a longlong2, :62).
+// An entry is 16 bytes {u64 key, u32 row id, pad}.  The reference reads one entry per
+// 16-byte load and steps on (ld.global.cs of a longlong2, :62), so a wave waits for its
+// longest probe run, one dependent load per slot.  Here groups of kWinG lanes read whole
+// 128-byte lines of 8 entries (32 bytes per lane) and serve their probes in kWinG
+// rounds, every open round requesting its line before any line is examined: a probe
+// resolves within the line of its home slot unless its run passes the line's end.
+// Tables below 8 slots take the entry-by-entry walk.
 // ---------------------------------------------------------------------------
 struct HashEntry {
   unsigned long long key;
@@ -818,15 +823,16 @@ static_assert(sizeof(HashEntry) == 16, "16-byte entry");
 
 template <class K>
 struct HashDev {
+  static constexpr uint32_t kLineEntries = 8;
+  static constexpr uint32_t kEntriesPerLane = kLineEntries / kWinG;  // 2
   const HashEntry* table;
-  uint64_t mask;
+  uint64_t mask;  // capacity - 1
 
   __host__ __device__ size_t lds_bytes() const { return 0; }
   __device__ void stage(uint8_t*) const {}
 
-  __device__ __forceinline__ uint32_t lookup(K x, bool active, const uint8_t*) const {
-    const uint64_t k = static_cast<uint64_t>(x);
-    if (!active || k == ~0ull) return kNotFound;  // all-ones is the empty marker, never a stored key
+  __device__ __forceinline__ uint32_t lookup_entries(uint64_t k, bool live) const {
+    if (!live) return kNotFound;
     uint64_t s = key_hash<K>(k) & mask;
     for (uint64_t probes = 0; probes <= mask; ++probes) {
       const u32x4 e = *reinterpret_cast<const u32x4*>(table + s);
@@ -836,6 +842,76 @@ struct HashDev {
       s = (s + 1) & mask;
     }
     return kNotFound;
+  }
+
+  __device__ __forceinline__ uint32_t lookup(K x, bool active, const uint8_t*) const {
+    const uint64_t k = static_cast<uint64_t>(x);
+    const bool live = active && k != ~0ull;  // all-ones is the empty marker, never a stored key
+    if (mask < kLineEntries - 1) return lookup_entries(k, live);  // wave-uniform
+    constexpr int G = kWinG;
+    const int ln = lane();
+    const int gbase = ln & ~(G - 1);
+    const int gl = ln & (G - 1);
+    uint64_t slot = live ? key_hash<K>(k) & mask : 0;
+    uint64_t lines_left = (mask + 1) / kLineEntries;  // a table without an empty slot ends the walk
+    bool open = live;
+    uint32_t found = kNotFound;
+    for (;;) {
+      const uint64_t open_mask = ballot(open);
+      if (open_mask == 0) break;  // wave-uniform exit
+      u32x4 e[G][kEntriesPerLane];
+#pragma unroll
+      for (int r = 0; r < G; ++r) {
+        const uint64_t sr = shfl64(slot, gbase + r);
+        if ((open_mask >> (gbase + r)) & 1ull) {
+          const u32x4* p = reinterpret_cast<const u32x4*>(table + (sr & ~static_cast<uint64_t>(kLineEntries - 1)) +
+                                                          static_cast<uint64_t>(gl) * kEntriesPerLane);
+#pragma unroll
+          for (uint32_t t = 0; t < kEntriesPerLane; ++t) e[r][t] = p[t];
+        }
+      }
+      bool hit = false, stop = false;
+      uint32_t value = 0;
+#pragma unroll
+      for (int r = 0; r < G; ++r) {
+        if (!((open_mask >> (gbase + r)) & 1ull)) continue;
+        const uint64_t kr = shfl64(k, gbase + r);
+        const uint32_t off = static_cast<uint32_t>(shfl64(slot, gbase + r)) & (kLineEntries - 1);
+        uint32_t v = 0;
+        bool m = false, em = false;
+#pragma unroll
+        for (uint32_t t = 0; t < kEntriesPerLane; ++t) {
+          const uint32_t j = static_cast<uint32_t>(gl) * kEntriesPerLane + t;  // entry in the line
+          const uint64_t ek = (static_cast<uint64_t>(e[r][t].y) << 32) | e[r][t].x;
+          const bool at = j >= off;
+          if (at && ek == kr) {
+            m = true;
+            v = e[r][t].z;
+          }
+          em |= at && ek == ~0ull;
+        }
+        // a present key comes before every empty slot of its run, so a match decides
+        const uint32_t mb = group_bits<G>(ballot(m), gbase);
+        const uint32_t eb = group_bits<G>(ballot(em), gbase);
+        const uint32_t vr = shfl32(v, gbase + (mb ? __ffs(mb) - 1 : 0));
+        if (gl == r) {
+          hit = mb != 0;
+          stop = hit || eb != 0;
+          value = vr;
+        }
+      }
+      if (open) {
+        if (hit) {
+          found = value;
+          open = false;
+        } else if (stop || --lines_left == 0) {
+          open = false;
+        } else {
+          slot = ((slot | (kLineEntries - 1)) + 1) & mask;  // the next line's first entry
+        }
+      }
+    }
+    return found;
   }
 };
 
