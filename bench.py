@@ -114,10 +114,11 @@ def main():
     ap.add_argument("--tune-log2", type=int, default=24, help="probe sample size of the autotuner")
     ap.add_argument("--tune-index", action="store_true",
                     help="pick each index's own parameters with the index autotuner before building it")
-    ap.add_argument("--only", choices=("C3", "sweep", "C5", "host", "other"), default="",
+    ap.add_argument("--only", choices=("C3", "sweep", "C5", "host", "host_sweep", "other"), default="",
                     help="run one leg of the other configurations alone and print its JSON (for rocprofv3 runs): "
                          "C3 the partitioned driver, sweep the index-size sweep, C5 the join-aggregate, host the "
-                         "host-memory placement, other sorted/dense probes, the hash-join driver and lookup plans")
+                         "host-memory placement, host_sweep columns of 2^26..2^31 keys in host memory (not in the "
+                         "default run), other sorted/dense probes, the hash-join driver and lookup plans")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -132,6 +133,10 @@ def main():
     if args.only == "C5":
         import bench_suite as suite
         print(json.dumps({"C5_join_aggregate": suite.join_aggregate_leg(gj, gj.Stream(), cpu=not args.no_cpu)}))
+        return
+    if args.only == "host_sweep":
+        import bench_suite as suite
+        print(json.dumps({"host_sweep": suite.host_sweep_leg(gj, gj.Stream())}))
         return
     n = 1 << args.log2_n
     m = 1 << args.log2_m

@@ -1,6 +1,6 @@
 """The legs of the default bench run beyond the headline (bench.py puts their results
-under "configs" in its JSON line; `bench.py --only C3|sweep|C5|host|other` runs one leg
-alone).
+under "configs" in its JSON line; `bench.py --only C3|sweep|C5|host|host_sweep|other` runs
+one leg alone).
 
   C1_u32             2^20 unique u32 keys, 2^24 probes: row ids + u64 checksum, every index
   C2b_zipf           the headline's 2^26 u64 keys, 2^28 Zipf(1.0) probes, every index
@@ -10,6 +10,8 @@ alone).
                      learned in both layouts, fallback keys counted
   C5_join_aggregate  2e6-row dimension, 6e7 fact rows, every strategy, the three batch modes
   host_placement     index structures (and the column) in page-locked host memory
+  host_sweep         (not in the default run) dense columns of 2^26..2^31 keys in host memory,
+                     partitioned lookups, after the reference's TLB study
   other              sorted probes, dense keys, the hash-join driver, the B-tree's node sizes,
                      lookup plans
 
@@ -340,6 +342,52 @@ def host_placement_leg(gj, stream, keys, d_keys, d_out, mh=1 << 24):
                 log(f"host placement {label}: {mh / ms * 1e-6:.3f} G lookups/s match={ok}")
             idx.close()
     hcol.free()
+    return out
+
+
+def host_sweep_leg(gj, stream, sizes=(26, 28, 30, 31), m=1 << 26):
+    """The host placement at scale, after the reference's TLB study
+    (scripts/bench_tlb_misses.sh): dense keys (key = row id, its --dataset dense) in a
+    column held in page-locked host memory, 2^26 keys (0.5 GiB) to 2^31 (16 GiB; row ids
+    are u32, so a column stays below 2^32 keys), the index structures in host memory too,
+    2^26 uniform probes in HBM through the partitioned driver (fan-out 2^12, one window) as
+    the script runs them, and the learned index also through the plain driver.  Not part
+    of the default run (`bench.py --only host_sweep`)."""
+    log = lambda *a: print("[extras]", *a, file=sys.stderr, flush=True)
+    rng = np.random.default_rng(0x27)
+    out = {"workload": "dense u64 keys in page-locked host memory (2^26..2^31 keys), index structures in host "
+                       "memory, 2^26 uniform probes in HBM", "rows": {}}
+    d_out = gj.DeviceArray(m, np.uint32)
+    chunk = 1 << 26
+    for lg in sizes:
+        n = 1 << lg
+        hcol = gj.HostArray(n, np.uint64)
+        view = hcol.view()
+        for c in range(0, n, chunk):
+            view[c:c + chunk] = np.arange(c, min(n, c + chunk), dtype=np.uint64)
+        probes = rng.integers(0, n, size=m, dtype=np.uint64)
+        exp = probes.astype(np.uint32)  # dense keys: key k sits at row k
+        dp = gj.DeviceArray.from_numpy(probes, stream)
+        for kind in ("btree", "learned_plr", "harmonia", "radix_spline", "binary_search"):
+            t0 = time.time()
+            idx = gj.Index(kind, hcol, host_column=view, stream=stream,
+                           params=gj.index_params(placement=_lib.PLACE_HOST))
+            build_s = time.time() - t0
+            st = idx.stats()
+            runs = [("partitioning", {"radix_bits": 12, "window": m})]
+            if kind == "learned_plr":
+                runs.append(("plain", {}))
+            for appr, kw in runs:
+                lp = gj.lookup_params(appr, **kw)
+                ms = _timed(gj, lambda: idx.lookup(dp, d_out, m=m, params=lp, stream=stream), stream, 2, 1)
+                ok = bool(np.array_equal(d_out.numpy(m), exp))
+                label = f"2^{lg}_{kind}_{appr}"
+                out["rows"][label] = {"lookups_per_s": m / ms * 1e3, "rowids_match": ok, "build_s": build_s,
+                                      "column_bytes": 8 * n, "index_bytes_in_host": st["memory_bytes"]}
+                log(f"host sweep {label}: {m / ms * 1e-6:.3f} G lookups/s match={ok} build {build_s:.1f}s")
+            idx.close()
+        del dp, probes, exp, view
+        hcol.free()
     return out
 
 

@@ -53,3 +53,37 @@ def test_auto_layout_in_host_memory_is_column(gj, orc, gpu):
         gj.synchronize()
         assert np.array_equal(out.numpy(), exp), placement
         idx.close()
+
+
+@pytest.mark.parametrize("kind", ["btree", "learned_plr", "harmonia"])
+def test_large_column_in_host_memory(gj, gpu, kind):
+    # an 8 GiB dense column (2^30 keys, past 2^32-byte offsets) in page-locked host memory
+    # with the index structures there too, as in the reference's host-memory sweep
+    # (scripts/bench_tlb_misses.sh); dense keys, so key k sits at row k, and misses are
+    # the keys past the column's end
+    n = 1 << 30
+    print(f"[case] host column 2^30 dense kind={kind}", flush=True)
+    hcol = gj.HostArray(n, np.uint64)
+    view = hcol.view()
+    chunk = 1 << 26
+    for c in range(0, n, chunk):
+        view[c:c + chunk] = np.arange(c, c + chunk, dtype=np.uint64)
+    rng = np.random.default_rng(0x85)
+    m = 1 << 22
+    probes = rng.integers(0, n + n // 8, size=m, dtype=np.uint64)
+    probes[:4] = [0, n - 1, n, 2 ** 63]
+    exp = np.where(probes < n, probes, gj.NOT_FOUND).astype(np.uint32)
+    idx = gj.Index(kind, hcol, host_column=view, params=gj.index_params(placement=_lib.PLACE_HOST))
+    dp = gj.DeviceArray.from_numpy(probes)
+    out = gj.DeviceArray(m, np.uint32)
+    for approach, kw in (("partitioning", {"radix_bits": 12}), ("plain", {})):
+        if approach == "plain" and kind != "learned_plr":
+            continue  # a plain B-tree or Harmonia walk over host memory takes seconds here
+        out.fill(0x5A)
+        idx.lookup(dp, out, approach=approach, **kw)
+        gj.synchronize()
+        assert np.array_equal(out.numpy(), exp), approach
+    assert idx.stats()["dropped_tuples"] == 0
+    idx.close()
+    del view
+    hcol.free()
