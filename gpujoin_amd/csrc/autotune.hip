@@ -34,11 +34,15 @@ extern "C" int gj_autotune_lookup(const gj_index* index, const void* d_probe_sam
       p.approach = GJ_BWS;
       cand.push_back(p);
     }
-    for (uint32_t bits : {8u, 10u, 12u}) {
+    // fan-outs above 2^12 always take two passes, and pay only with large windows (binary
+    // search over 2^26 keys: 14.4 G lookups/s at 2^16 partitions, 12.4 at 2^10)
+    for (uint32_t bits : {8u, 10u, 12u, 14u, 16u}) {
       for (uint64_t window : {1ull << 20, 1ull << 22, 1ull << 24, 1ull << 26}) {
         if (window > 4 * m && window != (1ull << 20)) continue;
+        if (bits > 12 && window < (1ull << 24)) continue;
         for (uint32_t passes : {1u, 2u}) {
           if (passes == 2 && bits < 10) continue;
+          if (passes == 1 && bits > 12) continue;
           gj_lookup_params p = base();
           p.approach = GJ_PARTITIONING;
           p.radix_bits = bits;
