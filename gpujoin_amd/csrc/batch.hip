@@ -679,6 +679,8 @@ __global__ __launch_bounds__(kAggBlock) void agg_filter_write_kernel(FactBatchDe
   const int ln = lane();
   const uint64_t lt = (ln == 0) ? 0ull : (~0ull >> (64 - ln));
   uint32_t run = offsets[blockIdx.x];
+  // (loading the next step's dates during this step, as the lane-refill row loop does,
+  // took the partitioned strategy from 203 to 169 G rows/s: not done here)
   for (uint64_t t0 = c0; t0 < c1; t0 += kFilterIn * blockDim.x) {
     int32_t dt[kFilterIn];
 #pragma unroll
