@@ -729,22 +729,25 @@ __global__ __launch_bounds__(kAggBlock) void agg_lookup_entries_kernel(Dev d, Fa
                                                                        uint32_t* __restrict__ ent_row,
                                                                        uint32_t* __restrict__ ent_dim) {
   extern __shared__ __attribute__((aligned(16))) uint8_t lds[];
-  __shared__ uint32_t wc[kAggBlock / kWave];
+  __shared__ uint32_t used;  // entries of the chunk written so far
   d.stage(lds);
+  if (threadIdx.x == 0) used = 0;
   __syncthreads();
   const uint64_t c0 = static_cast<uint64_t>(blockIdx.x) * kFilterChunk;
   const uint64_t c1 = c0 + kFilterChunk < b.rows ? c0 + kFilterChunk : b.rows;
-  const int w = threadIdx.x / kWave;
   const int ln = lane();
   const uint64_t lt = (ln == 0) ? 0ull : (~0ull >> (64 - ln));
   const uint32_t first = offsets[blockIdx.x];
-  uint32_t run = first;
-  // each row's date is loaded one tile ahead, so it arrives during the tile before's lookups
-  int32_t next = c0 + threadIdx.x < c1 ? load_stream(b.date + c0 + threadIdx.x) : 0;
-  for (uint64_t t0 = c0; t0 < c1; t0 += blockDim.x) {
-    const uint64_t i = t0 + threadIdx.x;
+  // each wave walks its own 64-row slices of the chunk; each row's date is loaded one
+  // slice ahead, so it arrives during the slice before's lookups; a wave reserves its
+  // entries with one LDS atomic (the join pass sums them in any order), so the waves of
+  // a workgroup never wait for one another
+  const uint64_t step = blockDim.x;
+  uint64_t i = c0 + threadIdx.x;
+  int32_t next = i < c1 ? load_stream(b.date + i) : 0;
+  for (uint64_t t0 = c0 + (threadIdx.x / kWave) * kWave; t0 < c1; t0 += step, i += step) {
     const int32_t dt = next;
-    next = i + blockDim.x < c1 ? load_stream(b.date + i + blockDim.x) : 0;
+    next = i + step < c1 ? load_stream(b.date + i + step) : 0;
     const bool qual = i < c1 && dt >= lo && dt < hi;
     bool match = false;
     uint32_t r = kNotFound;
@@ -754,22 +757,18 @@ __global__ __launch_bounds__(kAggBlock) void agg_lookup_entries_kernel(Dev d, Fa
       match = qual && r != kNotFound;
     }
     const uint64_t mask = ballot(match);
-    if (ln == 0) wc[w] = static_cast<uint32_t>(__popcll(mask));
-    __syncthreads();
-    uint32_t before = 0, total = 0;
-    for (int v = 0; v < kAggBlock / kWave; ++v) {
-      before += v < w ? wc[v] : 0u;
-      total += wc[v];
-    }
+    if (mask == 0) continue;  // wave-uniform
+    uint32_t at0 = 0;
+    if (ln == 0) at0 = atomicAdd(&used, static_cast<uint32_t>(__popcll(mask)));
+    at0 = first + shfl32(at0, 0);
     if (match) {
-      const uint32_t at = run + before + static_cast<uint32_t>(__popcll(mask & lt));
+      const uint32_t at = at0 + static_cast<uint32_t>(__popcll(mask & lt));
       ent_row[at] = static_cast<uint32_t>(i);
       ent_dim[at] = r;
     }
-    run += total;
-    __syncthreads();  // wc is rewritten by the next tile
   }
-  if (threadIdx.x == 0) matched[blockIdx.x] = run - first;
+  __syncthreads();
+  if (threadIdx.x == 0) matched[blockIdx.x] = used;
 }
 
 // The lookup-first strategy, second pass (ij_join_kernel, src/tpch/tpch_14_ij.cu:416-461):
