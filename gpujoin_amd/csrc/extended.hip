@@ -217,8 +217,8 @@ __global__ void ht_insert_kernel(const K* __restrict__ col, uint64_t n, HashEntr
 }
 
 // Atomic-free build of the table for large columns.  The hashed keys h = key_hash(k)
-// are radix-partitioned on bits [kHtRangeBits, cap_bits) of h, that is by the table
-// range of 2^kHtRangeBits slots that holds each key's home slot h & (cap - 1).  One
+// are radix-partitioned on bits [range_bits, cap_bits) of h, that is by the table
+// range of 2^range_bits slots that holds each key's home slot h & (cap - 1).  One
 // workgroup builds a range in LDS, linear probing with LDS compare-and-swap and the
 // key recovered from its hash, then writes the whole range out in full lines, empty
 // slots included (so the table needs no fill).  A key whose probe run passes the end of
@@ -227,7 +227,11 @@ __global__ void ht_insert_kernel(const K* __restrict__ col, uint64_t n, HashEntr
 // key sits at or after its home slot with no empty slot between, which is all a
 // lookup (HashDev) relies on; only the order inside a run can differ from an
 // insert-built table.
-constexpr uint32_t kHtRangeBits = 13;  // 8192 slots: 96 KiB of LDS (8 B key + 4 B row id)
+// A range of 2^range_bits slots takes 12 B of LDS per slot (8 B key + 4 B row id): 2^12
+// (48 KiB, two workgroups per CU) where the partitioner's 2^16 partitions cover the table,
+// 2^13 (96 KiB) for the largest tables.
+constexpr uint32_t kHtRangeMinBits = 12;
+constexpr uint32_t kHtRangeMaxBits = 13;
 constexpr uint32_t kHtRangeBlock = 1024;
 
 template <class K>
@@ -235,8 +239,9 @@ __global__ __launch_bounds__(kHtRangeBlock) void ht_range_build_kernel(K* __rest
                                                                       const uint64_t* __restrict__ start,
                                                                       const uint64_t* __restrict__ count,
                                                                       uint32_t* __restrict__ spills,
-                                                                      HashEntry* __restrict__ table) {
-  constexpr uint32_t S = 1u << kHtRangeBits;
+                                                                      HashEntry* __restrict__ table,
+                                                                      uint32_t range_bits) {
+  const uint32_t S = 1u << range_bits;
   extern __shared__ unsigned long long lk[];  // [S] keys, then [S] row ids
   uint32_t* lv = reinterpret_cast<uint32_t*>(lk + S);
   __shared__ uint32_t nspill;
@@ -280,7 +285,7 @@ __global__ __launch_bounds__(kHtRangeBlock) void ht_range_build_kernel(K* __rest
   }
   __syncthreads();
   if (threadIdx.x == 0) spills[b] = nspill;
-  u32x4* dst = reinterpret_cast<u32x4*>(table + (static_cast<uint64_t>(b) << kHtRangeBits));
+  u32x4* dst = reinterpret_cast<u32x4*>(table + (static_cast<uint64_t>(b) << range_bits));
   for (uint32_t j = threadIdx.x; j < S; j += blockDim.x) {
     u32x4 e;
     e.x = static_cast<uint32_t>(lk[j]);
@@ -298,14 +303,14 @@ template <class K>
 __global__ __launch_bounds__(256) void ht_spill_kernel(const K* __restrict__ hkeys, const uint32_t* __restrict__ rows,
                                                        const uint64_t* __restrict__ start,
                                                        const uint32_t* __restrict__ spills, HashEntry* table,
-                                                       uint64_t mask) {
+                                                       uint64_t mask, uint32_t range_bits) {
   const uint64_t b = blockIdx.x;
   const uint32_t ns = spills[b];
   for (uint32_t q = threadIdx.x; q < ns; q += blockDim.x) {
     const uint64_t h = static_cast<uint64_t>(hkeys[start[b] + q]);
     const uint32_t r = rows[start[b] + q];
     const unsigned long long k = key_unhash<K>(h);
-    uint64_t s = ((b + 1) << kHtRangeBits) & mask;
+    uint64_t s = ((b + 1) << range_bits) & mask;
     for (;;) {
       const unsigned long long prev = atomicCAS(&table[s].key, ~0ull, k);
       if (prev == ~0ull || prev == k) {
@@ -333,11 +338,15 @@ uint64_t hash_capacity(uint64_t n) {
   return 1ull << bits;
 }
 
-// Tables of 2^14 .. 2^(kHtRangeBits + 16) slots are built by range (the partitioner
+// Tables of 2^13 .. 2^(kHtRangeMaxBits + 16) slots are built by range (the partitioner
 // takes up to 2^16 partitions), smaller and larger ones by inserts.
 bool hash_build_by_range(uint64_t cap) {
   const uint32_t bits = ilog2_floor(cap);
-  return bits > kHtRangeBits && bits <= kHtRangeBits + 16;
+  return bits > kHtRangeMinBits && bits <= kHtRangeMaxBits + 16;
+}
+uint32_t hash_range_bits(uint64_t cap) {
+  const uint32_t bits = ilog2_floor(cap);
+  return bits > kHtRangeMinBits + 16 ? bits - 16 : kHtRangeMinBits;
 }
 
 template <class K>
@@ -353,7 +362,8 @@ void build_hash_table(const gj_index& ix, dbuf& table, uint64_t& cap, dbuf& temp
     GJ_HIP_CHECK(hipGetLastError());
     return;
   }
-  const uint32_t bits = ilog2_floor(cap) - kHtRangeBits;  // ranges = partitions
+  const uint32_t rb = hash_range_bits(cap);
+  const uint32_t bits = ilog2_floor(cap) - rb;  // ranges = partitions
   const uint64_t ranges = 1ull << bits, n = ix.n;
   const uint64_t tcap = partition_capacity(n, sizeof(K), bits);
   // temp: hashes | partitioned hashes | row ids | start (ranges + 1) | count | spills | partitioner scratch
@@ -372,16 +382,17 @@ void build_hash_table(const gj_index& ix, dbuf& table, uint64_t& cap, dbuf& temp
   hipLaunchKernelGGL(ht_hash_kernel<K>, dim3(grid_for(n, 256)), dim3(256), 0, s, static_cast<const K*>(ix.d_col), n, hk);
   GJ_HIP_CHECK(hipGetLastError());
   // payload = row id; the digit is the range of the home slot
-  partition_keys<K>(hk, n, 0, bits, kHtRangeBits, 1, false, pk, pr, st, ct, base + o_ps,
+  partition_keys<K>(hk, n, 0, bits, rb, 1, false, pk, pr, st, ct, base + o_ps,
                     ix.dropped.as<unsigned long long>(), ix.cu, s);
-  const size_t lds = (1u << kHtRangeBits) * (8 + 4);
+  const size_t lds = (static_cast<size_t>(1) << rb) * (8 + 4);
   const void* fn = reinterpret_cast<const void*>(&ht_range_build_kernel<K>);
-  GJ_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+  if (lds > 65536)
+    GJ_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
   hipLaunchKernelGGL(ht_range_build_kernel<K>, dim3(static_cast<unsigned>(ranges)), dim3(kHtRangeBlock), lds, s, pk, pr,
-                     st, ct, sp, table.as<HashEntry>());
+                     st, ct, sp, table.as<HashEntry>(), rb);
   GJ_HIP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(ht_spill_kernel<K>, dim3(static_cast<unsigned>(ranges)), dim3(256), 0, s, pk, pr, st, sp,
-                     table.as<HashEntry>(), cap - 1);
+                     table.as<HashEntry>(), cap - 1, rb);
   GJ_HIP_CHECK(hipGetLastError());
 }
 template void build_hash_table<uint32_t>(const gj_index&, dbuf&, uint64_t&, dbuf&, hipStream_t);

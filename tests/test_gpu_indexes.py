@@ -479,10 +479,10 @@ def _unfmix(h, kb):
 
 @pytest.mark.parametrize("kb", [4, 8])
 def test_hash_crowded_ranges(gj, orc, gpu, kb):
-    # keys chosen by their hashes: 12000 home slots in the first 8192-slot range of the
-    # 2^17-slot table and 3000 in its last 200 slots, so the build meets a range with more
-    # keys than slots (its excess runs on through the next ranges) and runs that wrap from
-    # the last slot to the first; 30000 more spread evenly
+    # keys chosen by their hashes: 12000 home slots in the first 8192 slots of the 2^17-slot
+    # table (its first two 4096-slot build ranges) and 3000 in its last 200 slots, so the
+    # build meets ranges with more keys than slots (their excess runs on through the next
+    # ranges) and runs that wrap from the last slot to the first; 30000 more spread evenly
     cap = 1 << 17
     dt = np.uint64 if kb == 8 else np.uint32
     rng = np.random.default_rng(0x7A + kb)
