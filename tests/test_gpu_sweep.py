@@ -5,7 +5,7 @@ src/index_lookup_partitioning.cu:445-447; sweep of scripts/bench_varying_index_s
 
 For each size a 2^22-probe sample with 10 % misses runs through the plain driver
 and the partitioned driver (fan-out 2^10 in one pass, 2^12 in two passes) for the
-B-tree, the learned index and binary search; hits must return the generator's rank
+B-tree, the learned index, binary search and the hash table; hits must return the generator's rank
 and misses must be absent from the host column (checked here by np.searchsorted).
 One Harmonia index is built and probed at 2^30 keys.
 """
@@ -42,8 +42,10 @@ def column(request, gj, gpu):
     del col, dp, out
 
 
-@pytest.mark.parametrize("kind", ["btree", "learned_plr", "binary_search"])
+@pytest.mark.parametrize("kind", ["btree", "learned_plr", "binary_search", "hash"])
 def test_size_sweep_plain_and_partitioned(gj, column, kind):
+    # the hash table's three build paths: by 4096-slot range (2^24 keys), by 8192-slot range
+    # (2^28 keys, a 2^29-slot table), by inserts (2^30 keys, 2^31 slots, 32 GiB)
     s, keys, col, dp, out, exp = column
     idx = gj.Index(kind, col, host_column=keys)
     st = idx.stats()
