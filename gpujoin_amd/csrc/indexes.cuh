@@ -77,19 +77,21 @@ __device__ __forceinline__ uint32_t group_count_less(const K* level_base, uint64
   constexpr int KPL = NK / G;
   static_assert(KPL == Piece<K>::kKeys, "one 16-byte piece per lane");
   Piece<K> pc[G];
-  K xs[G];
 #pragma unroll
   for (int r = 0; r < G; ++r) {
     const uint64_t nd = shfl64(my_node, gbase + r);
-    xs[r] = shfl(my_x, gbase + r);
     pc[r] = load_piece(level_base + nd * NK + gl * KPL);
   }
+  // each round's probe is fetched from its lane when compared (while the lines are in
+  // flight), so only the G lines stay live across the level: 108 -> 64 VGPRs for u64
+  // keys, 4 -> 8 waves per SIMD
   uint32_t result = 0;
 #pragma unroll
   for (int r = 0; r < G; ++r) {
+    const K xr = shfl(my_x, gbase + r);
     int cnt = 0;
 #pragma unroll
-    for (int j = 0; j < KPL; ++j) cnt += group_popc<G>(ballot(pc[r].k[j] < xs[r]), gbase);
+    for (int j = 0; j < KPL; ++j) cnt += group_popc<G>(ballot(pc[r].k[j] < xr), gbase);
     if (gl == r) result = static_cast<uint32_t>(cnt);
   }
   return result;
@@ -102,30 +104,30 @@ __device__ __forceinline__ uint32_t group_find_equal(const K* col, uint64_t n, u
   constexpr int NK = NB / static_cast<int>(sizeof(K));
   constexpr int KPL = NK / G;
   Piece<K> pc[G];
-  K xs[G];
-  int valid[G];
-  uint64_t chunk[G];
 #pragma unroll
   for (int r = 0; r < G; ++r) {
-    chunk[r] = shfl64(my_chunk, gbase + r);
-    xs[r] = shfl(my_x, gbase + r);
-    const uint64_t start = chunk[r] * NK + static_cast<uint64_t>(gl * KPL);
-    valid[r] = (start >= n) ? 0 : ((n - start) >= static_cast<uint64_t>(KPL) ? KPL : static_cast<int>(n - start));
-    pc[r] = load_piece_guarded(col + start, valid[r], key_max<K>());
+    const uint64_t chunk = shfl64(my_chunk, gbase + r);
+    const uint64_t start = chunk * NK + static_cast<uint64_t>(gl * KPL);
+    const int valid = (start >= n) ? 0 : ((n - start) >= static_cast<uint64_t>(KPL) ? KPL : static_cast<int>(n - start));
+    pc[r] = load_piece_guarded(col + start, valid, key_max<K>());
   }
-  uint32_t result = kNotFound;
-  const uint64_t live_mask = ballot(my_live);
+  // as in group_count_less, each round's probe is fetched again when compared; the owning
+  // lane forms the match position from its own chunk.  Slots past the column's end hold the
+  // key_max fill: the lowest matching slot is taken (the column's keys are unique, so a
+  // real match precedes any fill) and a position past the end reads as not found.
+  uint64_t pos = ~0ull;
 #pragma unroll
   for (int r = 0; r < G; ++r) {
-    uint32_t found = kNotFound;
+    const K xr = shfl(my_x, gbase + r);
+    int best = NK;
 #pragma unroll
     for (int j = 0; j < KPL; ++j) {
-      const int f = group_ffs<G>(ballot(j < valid[r] && pc[r].k[j] == xs[r]), gbase);
-      if (f >= 0) found = static_cast<uint32_t>(chunk[r] * NK + static_cast<uint64_t>(f * KPL + j));
+      const int f = group_ffs<G>(ballot(pc[r].k[j] == xr), gbase);
+      if (f >= 0 && f * KPL + j < best) best = f * KPL + j;
     }
-    if (gl == r && ((live_mask >> (gbase + r)) & 1ull)) result = found;
+    if (gl == r && best < NK) pos = my_chunk * NK + static_cast<uint64_t>(best);
   }
-  return result;
+  return (my_live && pos < n) ? static_cast<uint32_t>(pos) : kNotFound;
 }
 
 // ---------------------------------------------------------------------------

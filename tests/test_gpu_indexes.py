@@ -514,3 +514,29 @@ def test_hash_crowded_ranges(gj, orc, gpu, kb):
     gj.synchronize()
     assert np.array_equal(out.numpy(), want)
     assert idx.stats()["dropped_tuples"] == 0 and bs.stats()["dropped_tuples"] == 0
+
+
+@pytest.mark.parametrize("kb", [4, 8])
+@pytest.mark.parametrize("n", [1000, 1008, 4096 + 3])
+def test_btree_key_max_in_last_chunk(gj, orc, gpu, kb, n):
+    # the column ends in the largest key, and its last leaf chunk is partial (n % 16 != 0
+    # for u64), so the chunk's tail holds the same value as fill: a probe of that key must
+    # find the real position, and the fill past the end must never answer
+    dt = np.uint64 if kb == 8 else np.uint32
+    kmax = np.iinfo(dt).max
+    rng = np.random.default_rng(0x99 + n + kb)
+    keys = np.unique(rng.integers(0, kmax - 1, size=n + n // 4, dtype=dt))[:n - 1]
+    keys = np.concatenate([keys, np.array([kmax], dtype=dt)])
+    probes = np.concatenate([np.array([kmax, kmax - 1, keys[-2], 0], dtype=dt), rng.choice(keys, 500)])
+    want = orc.lookup_lower_bound(keys, probes)
+    assert want[0] == keys.size - 1
+    col = gj.DeviceArray.from_numpy(keys)
+    dp = gj.DeviceArray.from_numpy(probes)
+    out = gj.DeviceArray(probes.size, np.uint32)
+    for nb in (64, 128, 256):
+        idx = gj.Index("btree", col, host_column=keys, params=gj.index_params(btree_node_bytes=nb))
+        out.fill(0x5A)
+        idx.lookup(dp, out)
+        gj.synchronize()
+        assert np.array_equal(out.numpy(), want), nb
+        idx.close()
