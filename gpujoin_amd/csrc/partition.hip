@@ -484,6 +484,12 @@ constexpr int kWcTile = 2048;
 // SIMD) to keep loads in flight; the stable one uses 256 (its per-wave count rows
 // grow with the waves) and runs only where two blocks fit per CU.
 constexpr int wc_block(bool stable) { return stable ? 256 : 1024; }
+// Pair: the unordered kernel's LDS leaves room for two blocks per CU, and the kernel is
+// held to 64 VGPRs (8 waves per SIMD) so that both run.  Measured on 2^24-item windows:
+// fan-out 2^8 153 -> 129 us per launch with two blocks against one at 73 VGPRs, although
+// the cap spills 32 bytes; where the LDS admits one block only (fan-out 2^10) the cap
+// only adds the spills (+13 %), so it is not applied there.
+constexpr int wc_min_waves(bool stable, bool pair) { return (!stable && pair) ? 8 : 1; }
 
 __host__ __device__ constexpr uint32_t wc_max_segments(uint32_t fan, uint32_t seg_bits) {
   // a tile completes at most one segment per partition beyond kWcTile / L
@@ -499,8 +505,8 @@ __host__ __device__ constexpr size_t wc_lds_bytes(uint32_t fan, uint32_t seg_bit
          + (static_cast<size_t>(wc_max_segments(fan, seg_bits)) * 2 + 3) / 4 * 4;  // segment list
 }
 
-template <class K, bool Stable>
-__global__ __launch_bounds__(wc_block(Stable)) void part_scatter_wc_kernel(
+template <class K, bool Stable, bool Pair>
+__global__ __launch_bounds__(wc_block(Stable), wc_min_waves(Stable, Pair)) void part_scatter_wc_kernel(
     const K* __restrict__ keys, const uint32_t* __restrict__ in_payload, uint32_t payload_base, uint64_t m,
     uint64_t chunk, uint32_t shift, uint32_t bits, uint32_t seg_bits, const uint32_t* __restrict__ block_off,
     const uint64_t* __restrict__ start, K* __restrict__ out_keys, uint32_t* __restrict__ out_payload,
@@ -546,6 +552,21 @@ __global__ __launch_bounds__(wc_block(Stable)) void part_scatter_wc_kernel(
   const uint32_t g = threadIdx.x >> seg_bits;
   const uint32_t q = threadIdx.x & lmask;
 
+  // The next tile's items are loaded while this tile is counted, placed and stored (the
+  // barriers wait for LDS operations only, so the loads stay in flight across them).
+  K kn[kItems];
+  uint32_t pn[kItems];
+  auto load_tile = [&](uint64_t t) {
+    const uint64_t b = t + static_cast<uint64_t>(w) * (kItems * kWave);
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+      const uint64_t i = b + static_cast<uint64_t>(j * kWave + ln);
+      const bool valid = i < hi;
+      kn[j] = valid ? load_stream(keys + i) : K(0);
+      pn[j] = valid ? (in_payload ? load_stream(in_payload + i) : payload_base + static_cast<uint32_t>(i)) : 0u;
+    }
+  };
+  load_tile(lo);
   for (uint64_t tile = lo; tile < hi; tile += kWcTile) {
     const uint64_t wbase = tile + static_cast<uint64_t>(w) * (kItems * kWave);
     K k[kItems];
@@ -553,11 +574,10 @@ __global__ __launch_bounds__(wc_block(Stable)) void part_scatter_wc_kernel(
     uint32_t rk[kItems];
 #pragma unroll
     for (int j = 0; j < kItems; ++j) {
-      const uint64_t i = wbase + static_cast<uint64_t>(j * kWave + ln);
-      const bool valid = i < hi;
-      k[j] = valid ? load_stream(keys + i) : K(0);
-      pay[j] = valid ? (in_payload ? load_stream(in_payload + i) : payload_base + static_cast<uint32_t>(i)) : 0u;
+      k[j] = kn[j];
+      pay[j] = pn[j];
     }
+    if (tile + kWcTile < hi) load_tile(tile + kWcTile);  // block-uniform
     // count
 #pragma unroll
     for (int j = 0; j < kItems; ++j) {
@@ -918,9 +938,11 @@ void partition_pass(const K* keys, const uint32_t* in_payload, uint32_t payload_
       GJ_HIP_CHECK(hipGetLastError());
     };
     if (stable)
-      launch(&part_scatter_wc_kernel<K, true>);
+      launch(&part_scatter_wc_kernel<K, true, false>);
+    else if (per_cu >= 2)
+      launch(&part_scatter_wc_kernel<K, false, true>);
     else
-      launch(&part_scatter_wc_kernel<K, false>);
+      launch(&part_scatter_wc_kernel<K, false, false>);
     return;
   }
   if (!stable) {
@@ -987,13 +1009,18 @@ void partition_segmented(const K* keys, const uint32_t* payload, uint64_t m, uin
   const uint64_t cap = m + (align > 1 ? total_fan * align : 0ull);
   if (sb) {
     const size_t slds = wc_lds_bytes(fan, sb, sizeof(K), false);
-    const void* fn = reinterpret_cast<const void*>(&part_scatter_wc_kernel<K, false>);
-    if (slds > 65536)
-      GJ_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(slds)));
-    hipLaunchKernelGGL((part_scatter_wc_kernel<K, false>), dim3(static_cast<unsigned>(blocks)), dim3(wc_block(false)),
-                       slds, s, keys, payload, 0u, m, chunk, shift, bits, sb, hist, start, out_keys, out_payload, inv,
-                       cap, dropped, sg);
-    GJ_HIP_CHECK(hipGetLastError());
+    auto launch = [&](auto kernel) {
+      if (slds > 65536)
+        GJ_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(slds)));
+      hipLaunchKernelGGL(kernel, dim3(static_cast<unsigned>(blocks)), dim3(wc_block(false)), slds, s, keys, payload,
+                         0u, m, chunk, shift, bits, sb, hist, start, out_keys, out_payload, inv, cap, dropped, sg);
+      GJ_HIP_CHECK(hipGetLastError());
+    };
+    if (kMaxLds / slds >= 2)
+      launch(&part_scatter_wc_kernel<K, false, true>);
+    else
+      launch(&part_scatter_wc_kernel<K, false, false>);
     return;
   }
   const size_t rlds = static_cast<size_t>(kPartWaves + 1) * fan * sizeof(uint32_t);  // <= 80 KiB of the 160
