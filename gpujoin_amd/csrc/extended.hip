@@ -233,6 +233,7 @@ __global__ void ht_insert_kernel(const K* __restrict__ col, uint64_t n, HashEntr
 constexpr uint32_t kHtRangeMinBits = 12;
 constexpr uint32_t kHtRangeMaxBits = 13;
 constexpr uint32_t kHtRangeBlock = 1024;
+constexpr int kHtRangeItems = 4;
 
 template <class K>
 __global__ __launch_bounds__(kHtRangeBlock) void ht_range_build_kernel(K* __restrict__ hkeys, uint32_t* __restrict__ rows,
@@ -253,33 +254,39 @@ __global__ __launch_bounds__(kHtRangeBlock) void ht_range_build_kernel(K* __rest
   if (threadIdx.x == 0) nspill = 0;
   __syncthreads();
   const uint64_t lo = start[b], n = count[b];
-  for (uint64_t c0 = 0; c0 < n; c0 += blockDim.x) {
-    const uint64_t t = c0 + threadIdx.x;
-    const bool valid = t < n;
-    K h = K(0);
-    uint32_t r = 0;
-    if (valid) {
-      h = hkeys[lo + t];
-      r = rows[lo + t];
+  // A range holds about 2^range_bits / 2 keys: kHtRangeItems per thread read them in one
+  // round of loads.
+  const uint64_t round = static_cast<uint64_t>(kHtRangeItems) * blockDim.x;
+  for (uint64_t c0 = 0; c0 < n; c0 += round) {
+    K h[kHtRangeItems];
+    uint32_t r[kHtRangeItems];
+#pragma unroll
+    for (int i = 0; i < kHtRangeItems; ++i) {
+      const uint64_t t = c0 + static_cast<uint64_t>(i) * blockDim.x + threadIdx.x;
+      h[i] = t < n ? hkeys[lo + t] : K(0);
+      r[i] = t < n ? rows[lo + t] : 0u;
     }
-    // every read of this chunk comes before the chunk's spill stores, which reuse
+    // every read of this round comes before the round's spill stores, which reuse
     // slots already read (a spill's slot is below the count of tuples read so far)
     __syncthreads();
-    if (valid) {
-      const unsigned long long k = key_unhash<K>(static_cast<uint64_t>(h));
+#pragma unroll
+    for (int i = 0; i < kHtRangeItems; ++i) {
+      const uint64_t t = c0 + static_cast<uint64_t>(i) * blockDim.x + threadIdx.x;
+      if (t >= n) continue;
+      const unsigned long long k = key_unhash<K>(static_cast<uint64_t>(h[i]));
       bool placed = false;
-      for (uint32_t j = static_cast<uint32_t>(h) & (S - 1); j < S; ++j) {
+      for (uint32_t j = static_cast<uint32_t>(h[i]) & (S - 1); j < S; ++j) {
         const unsigned long long prev = atomicCAS(&lk[j], ~0ull, k);
         if (prev == ~0ull || prev == k) {
-          lv[j] = r;
+          lv[j] = r[i];
           placed = true;
           break;
         }
       }
       if (!placed) {
         const uint32_t q = atomicAdd(&nspill, 1u);
-        hkeys[lo + q] = h;
-        rows[lo + q] = r;
+        hkeys[lo + q] = h[i];
+        rows[lo + q] = r[i];
       }
     }
   }
