@@ -177,18 +177,21 @@ def test_approaches_match_oracle(gj, orc, gpu, kind, kb, approach, extra):
     assert np.array_equal(out.numpy(), want)
 
 
-@pytest.mark.parametrize("kb,bits", [(8, 8), (8, 10), (4, 10), (4, 11), (8, 14), (8, 16), (4, 16)])
-def test_partitioned_lookup_many_tiles(gj, orc, gpu, kb, bits):
+@pytest.mark.parametrize("kb,bits,passes", [(8, 8, 1), (8, 10, 1), (4, 7, 1), (4, 10, 1), (4, 11, 1), (4, 12, 2),
+                                            (8, 12, 2), (8, 14, 2), (8, 16, 2), (4, 16, 2)])
+def test_partitioned_lookup_many_tiles(gj, orc, gpu, kb, bits, passes):
     # 3e6 probes in one window: each block of the unordered write-combining scatter
-    # walks several tiles and carries open segments between them (u32 keys at
-    # fan-out 2048 use 4-slot segments)
-    print(f"[case] partitioned lookup many tiles kb={kb} bits={bits}", flush=True)
+    # walks several tiles, loading the next one ahead, and carries open segments between
+    # them (u32 keys at fan-out 2048 use 4-slot segments).  Fan-outs whose buffers leave
+    # room for two blocks per CU (u64 up to 2^8, u32 up to 2^7 per pass) take the
+    # register-capped kernel, the others the uncapped one.
+    print(f"[case] partitioned lookup many tiles kb={kb} bits={bits} passes={passes}", flush=True)
     keys, col, idx = _setup(gj, "btree", 1 << 20, kb)
     probes, _ = gj.gen_probes(keys, 3_000_017, _lib.PROBES_UNIFORM, 1.0, 0.05, 0, 0x24 + bits)
     dp = gj.DeviceArray.from_numpy(probes)
     out = gj.DeviceArray(probes.size, np.uint32)
     out.fill(0x5A)
-    idx.lookup(dp, out, approach="partitioning", radix_bits=bits, window=1 << 22)
+    idx.lookup(dp, out, approach="partitioning", radix_bits=bits, passes=passes, window=1 << 22)
     gj.synchronize()
     assert idx.stats()["dropped_tuples"] == 0
     assert np.array_equal(out.numpy(), orc.lookup_lower_bound(keys, probes))
