@@ -55,6 +55,33 @@ def test_auto_layout_in_host_memory_is_column(gj, orc, gpu):
         idx.close()
 
 
+@pytest.mark.parametrize("kind", ["btree", "learned_plr"])
+@pytest.mark.parametrize("column_in_host", [False, True])
+def test_autotuned_lookup_in_host_memory(gj, orc, gpu, kind, column_in_host):
+    # the lookup autotuner over an index whose structures sit in host memory (the
+    # configuration the bench's host-placement leg measures): the configuration it
+    # returns must give the oracle's row ids
+    print(f"[case] autotune host placement kind={kind} column_in_host={column_in_host}", flush=True)
+    keys = gj.gen_build_keys(1 << 20, 8, _lib.KEYS_UNIFORM, 63, 0x86)
+    probes, exp = gj.gen_probes(keys, 1 << 20, _lib.PROBES_UNIFORM, 1.0, 0.1, 63, 0x87)
+    col = gj.HostArray.from_numpy(keys) if column_in_host else gj.DeviceArray.from_numpy(keys)
+    idx = gj.Index(kind, col, host_column=keys, params=gj.index_params(placement=_lib.PLACE_HOST))
+    dp = gj.DeviceArray.from_numpy(probes)
+    out = gj.DeviceArray(probes.size, np.uint32)
+    best, rate = gj.autotune(idx, dp, out)
+    print(f"[autotune host] approach={best.approach} bits={best.radix_bits} passes={best.passes} "
+          f"window={best.window} rate={rate / 1e9:.2f} G/s", flush=True)
+    assert rate > 0
+    out.fill(0x5A)
+    idx.lookup(dp, out, params=best)
+    gj.synchronize()
+    assert np.array_equal(out.numpy(), exp)
+    assert idx.stats()["dropped_tuples"] == 0
+    idx.close()
+    if column_in_host:
+        col.free()
+
+
 @pytest.mark.parametrize("kind", ["btree", "learned_plr", "harmonia"])
 def test_large_column_in_host_memory(gj, gpu, kind):
     # an 8 GiB dense column (2^30 keys, past 2^32-byte offsets) in page-locked host memory
